@@ -1,0 +1,106 @@
+/*
+ * cpsd_abi.h -- C-ABI of the MI355X (gfx950) SD step, libcpsd_hip.so.
+ *
+ * Drop-in entry points (same symbols, arguments and semantics as the
+ * reference; paths relative to the reference's Cpsd-mpi-bamboo/):
+ *
+ *   cpsd()       replaces ccpsdv3/cpsd.c:33-71 (prototype at
+ *                ccpsdv3/main.c:18-56): `inner` steepest-descent steps of
+ *                the orbitals followed by one energy evaluation.  All
+ *                arrays are the caller's host arrays with the reference's
+ *                layouts; on return c1, c2, cpsi (= c2 - c1), dn, hml, lmd,
+ *                E[0..9], *deltae, *deltac, *deltar, r1 hold what the
+ *                reference leaves there.  cpsdwork is not written (the
+ *                scratch lives on the GPU).
+ *   cr_fft3b_()  replaces fftpack3d/fftpack3d.f:22-96 (Fortran ABI, all
+ *                arguments by pointer): in-place unnormalised inverse 3D FFT
+ *                of a half-complex (N/2+1) x N x N array into padded real
+ *                rows of N+2 (pads set to zero).
+ *   rc_fft3f_()  replaces fftpack3d/fftpack3d.f:107-196: the forward
+ *                transform, Im(kx=0) = Im(kx=N/2) = 0.
+ *
+ * Supported inputs: ispin = 1, move = 0, ne[1] = 0, nx = ny = nz = nfft
+ * with nfft in {8,12,16,20,24,32,40,48,64,80,96,128}, the reference's
+ * increments, orbitals that vanish outside the spherical cutoff mask (the
+ * reference keeps them there).  Anything else is reported on stderr and,
+ * unless cpsd_set_abort_on_error(0) was called, aborts the process; the
+ * library never falls back to a CPU path.
+ *
+ * The cpsd_dev_* functions have no counterpart in the reference: they keep
+ * the orbitals resident in HBM across calls for callers that run many SD
+ * steps (the run driver, benchmarks) and avoid the per-call host<->device
+ * copies that the host-pointer contract of cpsd() implies.
+ */
+#ifndef CPSD_ABI_H
+#define CPSD_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ccpsdv3/cpsd.c:33-71 */
+void cpsd(const int inner, const int move, const int nion, const int ispin, const int *ne,
+          const int nfft, const int nfft3d, const int n2ft3d, const double *unitg,
+          const double omega, const double dt, const double fmass, const int *katm,
+          const double *amass, const double *zv, const int nsh, const double *rcell,
+          const double cewald, const double rcut, const int *masker, const double *G,
+          const double *tg, const double *vc, const double *vg, int *lmmax, double *vnlnrm,
+          double *vl, double *vnl, double *r1, double *r2, double *fion, double *c1,
+          double *c2, double *cpsi, double *dn, double *hml, double *lmd, double *cpsdwork,
+          double *E, double *deltae, double *deltac, double *deltar);
+
+/* fftpack3d/fftpack3d.f:22-35 */
+void cr_fft3b_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny,
+               int *nz, double *tmp, int *ntmp);
+/* fftpack3d/fftpack3d.f:107-135 */
+void rc_fft3f_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny,
+               int *nz, double *tmp, int *ntmp);
+
+/* ---- error handling (no reference counterpart) ---------------------- */
+const char *cpsd_last_error(void);
+void cpsd_set_abort_on_error(int on);
+
+/* ---- device-resident API (no reference counterpart) ------------------
+ * Static inputs use the reference's layouts (ccpsdv3/main.c:145-320):
+ * masker[n2ft3d], tg/vc/vg[nfft3d], vl[nkind*nfft3d], vnl[nkind*9*nfft3d],
+ * vnlnrm[nkind*9], lmmax[nkind], katm[nion] (0-based), zv[nkind],
+ * rcell[3*nsh], r[3*nion].  Returns 0 on success, nonzero on error
+ * (message in cpsd_last_error()).                                        */
+typedef struct cpsd_dev cpsd_dev;
+
+int cpsd_dev_create(cpsd_dev **h, int nfft, int ne, int nion, const int *katm, const double *zv,
+                    const int *lmmax, const double *vnlnrm, const double *vl, const double *vnl,
+                    const int *masker, const double *tg, const double *vc, const double *vg,
+                    const double *unitg, double omega, int nsh, const double *rcell,
+                    double cewald, double rcut, const double *r);
+/* orbitals c[ne*n2ft3d] in the reference's ELCIN layout */
+int cpsd_dev_set_orbitals(cpsd_dev *h, const double *c);
+/* inner SD steps + energy; E[0] holds the previous energy on entry
+ * (as in the reference), E[0..9] on exit; deltas as in cpsd().         */
+int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E, double *deltae,
+                 double *deltac, double *deltar);
+/* any pointer may be NULL; layouts as in cpsd() */
+int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn, double *hml,
+                 double *lmd);
+/* last step's Lagrange iteration count and final max|dX| */
+int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff);
+/* sizes: [0] ng (masked G vectors), [1] ngp, [2] nproj */
+int cpsd_dev_sizes(cpsd_dev *h, int *sizes);
+/* per-phase GPU time (ms) accumulated with HIP events on the library's
+ * stream since the last reset; names separated by ';' */
+int cpsd_dev_profile(cpsd_dev *h, int enable);
+int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, double *ms, int *calls,
+                          int max_phases);
+void cpsd_dev_destroy(cpsd_dev *h);
+
+/* ---- kernel-level test hooks (no reference counterpart) -------------
+ * Batched versions of the FFT entry points and the LDA kernel, used by
+ * the parity tests.                                                     */
+int cpsd_fft3d_batch(int dir, int nfft, int nbatch, double *A);  /* dir +1: cr, -1: rc */
+int cpsd_lda_device(int n, const double *dn, double *xcp, double *xce);
+int cpsd_erfc_device(int n, const double *w, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

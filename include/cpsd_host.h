@@ -1,0 +1,57 @@
+/*
+ * cpsd_host.h -- host-side inputs of the SD step (libcpsd_host.so, plain C).
+ *
+ * These build what the reference's driver and preprocessors hand to cpsd():
+ *   cpsdh_setup        ccpsdv3/main.c:145-198, 287-313 (lattice, G vectors,
+ *                      mask, tg/vc/vg, Ewald constant, 27 image cells)
+ *   cpsdh_read_psp     the radial table read by ckbppv3/dkbppv3.c:89-114,
+ *                      accepting the optional 4th token on line 2 that the
+ *                      reference reader misparses
+ *   cpsdh_kbpp         ckbppv3/dkbppv3.c:335-444 (KB transform to the grid)
+ *   cpsdh_makepsi      cmakepsi/dmakepsi.c:86-133 (plane-wave start)
+ * and, with no counterpart in the reference, a seeded Gaussian starting
+ * state orthonormalised under the reference's G-space inner product
+ * (ccpsdv3/gcdotc.c), used for the large workloads.
+ */
+#ifndef CPSD_HOST_H
+#define CPSD_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fills omega, unita[9], unitg[9], G[3*nfft3d], masker[n2ft3d], tg, vc,
+ * vg[nfft3d], rcell[3*27], cewald, rcut.  katm 0-based per ion. */
+int cpsdh_setup(int icube, double unit, int nfft, int nion, const int *katm, const double *zv,
+                double *omega, double *unita, double *unitg, double *G, int *masker, double *tg,
+                double *vc, double *vg, double *rcell, double *cewald, double *rcut);
+
+/* Parses a PSP .dat table.  On entry *nrho_cap is the capacity of rho,
+ * vp, wp (vp and wp hold (lmax+1)*nrho values, column-major as in the
+ * reference).  Returns 0 on success. */
+int cpsdh_read_psp(const char *path, char *atom, double *zv, double *amass, int *lmax, double *rc,
+                   int *nrho, double *drho, double *rho, double *vp, double *wp, int nrho_cap);
+
+/* KB transform; lmax may be lowered below the table's (channel lmax becomes
+ * local).  vp is not modified.  vl[nfft3d], vnl[9*nfft3d], vnlnrm[9]. */
+int cpsdh_kbpp(double zv, int lmax, int nrho, double drho, const double *rho, const double *vp,
+               const double *wp, int nfft3d, const double *G, double *vl, double *vnl,
+               double *vnlnrm);
+
+/* plane-wave orbitals from PSI0.DAT-style rows (i, j, k, l) */
+int cpsdh_makepsi(int nfft, int ne, const int *spec, double *psi);
+
+/* Seeded Gaussian orbitals on the masked G vectors (splitmix64 +
+ * Box-Muller), kx = 0 plane made Hermitian, Im(G=0) = 0, then
+ * orthonormalised (modified Gram-Schmidt under gcdotc).  decay >= 0 damps
+ * coefficient magnitudes by exp(-decay*|G|^2/2) (0: plain Gaussian). */
+int cpsdh_random_orbitals(int nfft, int ne, const int *masker, const double *G, unsigned long long seed,
+                          double decay, double *psi);
+
+/* G-space inner product of ccpsdv3/gcdotc.c */
+double cpsdh_gcdotc(int nfft, const double *a, const double *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

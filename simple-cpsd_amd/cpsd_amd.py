@@ -1,0 +1,398 @@
+"""ctypes front end of the MI355X SD step (libcpsd_hip.so) and of its host
+input builder (libcpsd_host.so).
+
+The compute path is the HIP library; this module only moves numpy arrays
+across the C-ABI declared in include/cpsd_abi.h and include/cpsd_host.h.
+Loading fails loudly when a library is missing: there is no CPU fallback.
+
+Inputs mirror the reference driver (Cpsd-mpi-bamboo/ccpsdv3/main.c:116-320):
+the half-complex grid layouts, masker/tg/vc/vg tables, per-kind vl/vnl/vnlnrm,
+0-based katm and the 27 Ewald image cells.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIP_SO = os.path.join(HERE, "libcpsd_hip.so")
+HOST_SO = os.path.join(HERE, "libcpsd_host.so")
+DATA = os.path.join(HERE, "data")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_DPTR = C.POINTER(C.c_double)
+
+_hip = None
+_host = None
+
+SUPPORTED_GRIDS = (8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128)
+
+# every symbol include/cpsd_abi.h declares
+ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_abort_on_error",
+               "cpsd_dev_create", "cpsd_dev_set_orbitals", "cpsd_dev_run", "cpsd_dev_get",
+               "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
+               "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_fft3d_batch",
+               "cpsd_lda_device", "cpsd_erfc_device")
+HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
+                "cpsdh_random_orbitals", "cpsdh_gcdotc")
+
+
+class CpsdError(RuntimeError):
+    pass
+
+
+def build(quiet=True):
+    """Compile both libraries in place (hipcc for gfx950, gcc for the host part)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def hip_lib():
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_SO):
+            raise CpsdError("libcpsd_hip.so is not built (%s); run __graft_entry__.build()" % HIP_SO)
+        L = C.CDLL(HIP_SO)
+        L.cpsd_last_error.restype = C.c_char_p
+        L.cpsd_set_abort_on_error.argtypes = [C.c_int]
+        L.cpsd_set_abort_on_error(0)
+        L.cpsd_dev_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, _ip, _dp, _ip,
+                                      _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
+                                      _dp, C.c_double, C.c_double, _dp]
+        L.cpsd_dev_set_orbitals.argtypes = [C.c_void_p, _dp]
+        L.cpsd_dev_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, _DPTR, _DPTR, _DPTR]
+        L.cpsd_dev_get.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.cpsd_dev_lagrange_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DPTR]
+        L.cpsd_dev_sizes.argtypes = [C.c_void_p, _ip]
+        L.cpsd_dev_profile.argtypes = [C.c_void_p, C.c_int]
+        L.cpsd_dev_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _ip, C.c_int]
+        L.cpsd_dev_destroy.argtypes = [C.c_void_p]
+        L.cpsd_fft3d_batch.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.cpsd_lda_device.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.cpsd_erfc_device.argtypes = [C.c_int, _dp, _dp]
+        L.cpsd.restype = None
+        L.cpsd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.c_int, _dp,
+                           C.c_double, C.c_double, C.c_double, _ip, _dp, _dp, C.c_int, _dp,
+                           C.c_double, C.c_double, _ip, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp,
+                           _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _DPTR, _DPTR, _DPTR]
+        for name in ("cr_fft3b_", "rc_fft3f_"):
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = [_dp] + [C.POINTER(C.c_int)] * 7 + [_dp, C.POINTER(C.c_int)]
+        _hip = L
+    return _hip
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_SO):
+            raise CpsdError("libcpsd_host.so is not built (%s)" % HOST_SO)
+        L = C.CDLL(HOST_SO)
+        L.cpsdh_setup.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, _ip, _dp, _DPTR, _dp, _dp,
+                                  _dp, _ip, _dp, _dp, _dp, _dp, _DPTR, _DPTR]
+        L.cpsdh_read_psp.argtypes = [C.c_char_p, C.c_char_p, _DPTR, _DPTR, C.POINTER(C.c_int), _dp,
+                                     C.POINTER(C.c_int), _DPTR, _dp, _dp, _dp, C.c_int]
+        L.cpsdh_kbpp.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_int, _dp,
+                                 _dp, _dp, _dp]
+        L.cpsdh_makepsi.argtypes = [C.c_int, C.c_int, _ip, _dp]
+        L.cpsdh_random_orbitals.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_ulonglong, C.c_double, _dp]
+        L.cpsdh_gcdotc.argtypes = [C.c_int, _dp, _dp]
+        L.cpsdh_gcdotc.restype = C.c_double
+        _host = L
+    return _host
+
+
+def _err():
+    return hip_lib().cpsd_last_error().decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CpsdError("%s failed: %s" % (what, _err()))
+
+
+# ------------------------------------------------------------------ inputs
+
+def read_psp(path):
+    L = host_lib()
+    cap = 4096
+    atom = C.create_string_buffer(8)
+    zv, amass, drho = C.c_double(), C.c_double(), C.c_double()
+    lmax, nrho = C.c_int(), C.c_int()
+    rc = np.zeros(8)
+    rho = np.zeros(cap)
+    vp = np.zeros(4 * cap)
+    wp = np.zeros(4 * cap)
+    rc_ = L.cpsdh_read_psp(path.encode(), atom, C.byref(zv), C.byref(amass), C.byref(lmax), rc,
+                           C.byref(nrho), C.byref(drho), rho, vp, wp, cap)
+    if rc_ != 0:
+        raise CpsdError("cannot parse %s (code %d)" % (path, rc_))
+    n, lm = nrho.value, lmax.value
+    return dict(atom=atom.value.decode(), zv=zv.value, amass=amass.value, lmax=lm,
+                rc=rc[: lm + 1].copy(), nrho=n, drho=drho.value, rho=rho[:n].copy(),
+                vp=vp[: (lm + 1) * n].copy(), wp=wp[: (lm + 1) * n].copy())
+
+
+class Cell:
+    """One periodic cell: setup tables (ccpsdv3/main.c:145-320) and the KB
+    pseudopotential of every kind on its grid (ckbppv3/dkbppv3.c)."""
+
+    def __init__(self, icube, unit, nfft, kinds, katm, positions):
+        """kinds: list of (psp path, lmax); katm: 0-based kind per ion;
+        positions: (nion, 3) bohr."""
+        L = host_lib()
+        self.icube, self.unit, self.nfft = int(icube), float(unit), int(nfft)
+        n = self.nfft
+        self.nfft3d = (n // 2 + 1) * n * n
+        self.n2ft3d = (n + 2) * n * n
+        self.katm = np.ascontiguousarray(katm, dtype=np.int32)
+        self.nion = len(self.katm)
+        self.r = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1))
+        self.psp = [read_psp(p) for p, _ in kinds]
+        self.lmax = [min(l, p["lmax"]) for (_, l), p in zip(kinds, self.psp)]
+        self.nkind = len(kinds)
+        self.zv = np.array([p["zv"] for p in self.psp])
+        self.amass = np.array([p["amass"] * 1822.89 for p in self.psp])
+        self.lmmax = np.array([l * l for l in self.lmax], dtype=np.int32)
+        om, cew, rcut = C.c_double(), C.c_double(), C.c_double()
+        self.unita = np.zeros(9)
+        self.unitg = np.zeros(9)
+        self.G = np.zeros(3 * self.nfft3d)
+        self.masker = np.zeros(self.n2ft3d, dtype=np.int32)
+        self.tg = np.zeros(self.nfft3d)
+        self.vc = np.zeros(self.nfft3d)
+        self.vg = np.zeros(self.nfft3d)
+        self.rcell = np.zeros(81)
+        rc = L.cpsdh_setup(self.icube, self.unit, n, self.nion, self.katm, self.zv, C.byref(om),
+                           self.unita, self.unitg, self.G, self.masker, self.tg, self.vc, self.vg,
+                           self.rcell, C.byref(cew), C.byref(rcut))
+        if rc != 0:
+            raise CpsdError("cpsdh_setup failed (%d)" % rc)
+        self.omega, self.cewald, self.rcut = om.value, cew.value, rcut.value
+        self.vl = np.zeros(self.nkind * self.nfft3d)
+        self.vnl = np.zeros(self.nkind * 9 * self.nfft3d)
+        self.vnlnrm = np.zeros(self.nkind * 9)
+        for k, (p, lm) in enumerate(zip(self.psp, self.lmax)):
+            vl = np.zeros(self.nfft3d)
+            vnl = np.zeros(9 * self.nfft3d)
+            nrm = np.zeros(9)
+            rc = L.cpsdh_kbpp(p["zv"], lm, p["nrho"], p["drho"], p["rho"], p["vp"], p["wp"],
+                              self.nfft3d, self.G, vl, vnl, nrm)
+            if rc != 0:
+                raise CpsdError("cpsdh_kbpp failed")
+            self.vl[k * self.nfft3d:(k + 1) * self.nfft3d] = vl
+            self.vnl[k * 9 * self.nfft3d:(k + 1) * 9 * self.nfft3d] = vnl
+            self.vnlnrm[9 * k:9 * k + 9] = nrm
+        self.ng = int(self.masker[0::2].sum())
+        self.nproj = int(sum(self.lmmax[k] for k in self.katm))
+
+    def plane_wave_orbitals(self, ne, psi0=None):
+        spec = read_psi0(psi0 or os.path.join(DATA, "PSI0.DAT"), ne)
+        psi = np.zeros(ne * self.n2ft3d)
+        host_lib().cpsdh_makepsi(self.nfft, ne, spec, psi)
+        return psi
+
+    def random_orbitals(self, ne, seed=1, decay=0.0):
+        psi = np.zeros(ne * self.n2ft3d)
+        rc = host_lib().cpsdh_random_orbitals(self.nfft, ne, self.masker, self.G, seed, decay, psi)
+        if rc != 0:
+            raise CpsdError("cpsdh_random_orbitals failed")
+        return psi
+
+
+def read_psi0(path, ne):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 4:
+                rows.append([int(x) for x in t[:4]])
+            if len(rows) == ne:
+                break
+    if len(rows) < ne:
+        raise CpsdError("%s holds only %d orbitals" % (path, len(rows)))
+    return np.ascontiguousarray(np.array(rows, dtype=np.int32).reshape(-1))
+
+
+# water geometry used by the workloads: O-H 1.81 bohr, H-O-H 104.5 degrees
+OH_BOHR = 1.81
+HOH_DEG = 104.5
+
+
+def water_molecule(center=(0.0, 0.0, 0.0), rot=None):
+    half = np.deg2rad(HOH_DEG) / 2
+    h1 = np.array([OH_BOHR * np.sin(half), OH_BOHR * np.cos(half), 0.0])
+    h2 = np.array([-OH_BOHR * np.sin(half), OH_BOHR * np.cos(half), 0.0])
+    pts = np.array([np.zeros(3), h1, h2])
+    if rot is not None:
+        pts = pts @ rot.T
+    return pts + np.asarray(center)
+
+
+def water_box(nmol, L, seed=0, jitter=0.3):
+    """nmol waters on a near-cubic lattice filling an L bohr cube, randomly
+    oriented, with +-jitter bohr uniform displacement of each molecule
+    (seeded numpy Generator).  Returns katm (O=0, H=1) and positions."""
+    rng = np.random.default_rng(seed)
+    nx = int(np.ceil(nmol ** (1.0 / 3.0)))
+    dims = [nx, nx, nx]
+    while (dims[0] * dims[1] * (dims[2] - 1)) >= nmol:
+        dims[2] -= 1
+    sites = [(i, j, k) for k in range(dims[2]) for j in range(dims[1]) for i in range(dims[0])][:nmol]
+    katm, pos = [], []
+    for (i, j, k) in sites:
+        c = (np.array([i, j, k]) + 0.5) * (L / np.array(dims)) - L / 2
+        c = c + rng.uniform(-jitter, jitter, 3)
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        m = water_molecule(c, q)
+        katm += [0, 1, 1]
+        pos.append(m)
+    return np.array(katm, dtype=np.int32), np.concatenate(pos)
+
+
+def water_cell(nmol, L, nfft, seed=0, jitter=0.3, psp_dir=None):
+    d = psp_dir or os.path.join(DATA, "psp")
+    katm, pos = water_box(nmol, L, seed, jitter)
+    return Cell(1, L, nfft, [(os.path.join(d, "O00.dat"), 2), (os.path.join(d, "H00.dat"), 1)],
+                katm, pos)
+
+
+# --------------------------------------------------------- device SD step
+
+class DeviceSD:
+    """Orbitals resident in HBM across calls (cpsd_dev_* of include/cpsd_abi.h)."""
+
+    def __init__(self, cell, ne):
+        self.cell, self.ne = cell, int(ne)
+        h = C.c_void_p()
+        c = cell
+        _check(hip_lib().cpsd_dev_create(C.byref(h), c.nfft, self.ne, c.nion, c.katm, c.zv, c.lmmax,
+                                         c.vnlnrm, c.vl, c.vnl, c.masker, c.tg, c.vc, c.vg, c.unitg,
+                                         c.omega, 27, c.rcell, c.cewald, c.rcut, c.r),
+               "cpsd_dev_create")
+        self.h = h
+        self.E = np.zeros(10)
+
+    def set_orbitals(self, psi):
+        _check(hip_lib().cpsd_dev_set_orbitals(self.h, np.ascontiguousarray(psi, dtype=np.float64)),
+               "cpsd_dev_set_orbitals")
+
+    def run(self, inner, dt=5.8, fmass=400000.0):
+        de, dc, dr = C.c_double(), C.c_double(), C.c_double()
+        _check(hip_lib().cpsd_dev_run(self.h, inner, dt, fmass, self.E, C.byref(de), C.byref(dc),
+                                      C.byref(dr)), "cpsd_dev_run")
+        return self.E.copy(), de.value, dc.value
+
+    def get(self, c1=False, c2=False, cpsi=False, dn=False, hml=False, lmd=False):
+        c = self.cell
+        out = {}
+        arrs = []
+        for name, want, size in (("c1", c1, self.ne * c.n2ft3d), ("c2", c2, self.ne * c.n2ft3d),
+                                 ("cpsi", cpsi, self.ne * c.n2ft3d), ("dn", dn, c.n2ft3d),
+                                 ("hml", hml, self.ne * self.ne), ("lmd", lmd, self.ne * self.ne)):
+            if want:
+                out[name] = np.zeros(size)
+                arrs.append(out[name].ctypes.data_as(C.c_void_p))
+            else:
+                arrs.append(None)
+        _check(hip_lib().cpsd_dev_get(self.h, *arrs), "cpsd_dev_get")
+        return out
+
+    def lagrange_info(self):
+        it, ad = C.c_int(), C.c_double()
+        hip_lib().cpsd_dev_lagrange_info(self.h, C.byref(it), C.byref(ad))
+        return it.value, ad.value
+
+    def sizes(self):
+        s = np.zeros(3, dtype=np.int32)
+        hip_lib().cpsd_dev_sizes(self.h, s)
+        return dict(ng=int(s[0]), ngp=int(s[1]), nproj=int(s[2]))
+
+    def profile(self, enable=True):
+        hip_lib().cpsd_dev_profile(self.h, 1 if enable else 0)
+
+    def profile_read(self):
+        names = C.create_string_buffer(1024)
+        ms = np.zeros(32)
+        calls = np.zeros(32, dtype=np.int32)
+        k = hip_lib().cpsd_dev_profile_read(self.h, names, 1024, ms, calls, 32)
+        nm = names.value.decode().split(";")
+        return {nm[i]: (float(ms[i]), int(calls[i])) for i in range(k)}
+
+    def close(self):
+        if self.h:
+            hip_lib().cpsd_dev_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0):
+    """Call the drop-in cpsd() symbol exactly as ccpsdv3/main.c:422-426 does.
+    Returns dict of the arrays cpsd() writes."""
+    c = cell
+    L = hip_lib()
+    n2 = c.n2ft3d
+    nea = np.array([ne, 0], dtype=np.int32)
+    r1 = np.zeros(3 * c.nion)
+    r2 = c.r.copy()
+    fion = np.zeros(3 * c.nion)
+    c1 = np.zeros(ne * n2)
+    c2 = np.ascontiguousarray(c2, dtype=np.float64).copy()
+    cpsi = np.zeros(ne * n2)
+    dn = np.zeros(n2)
+    hml = np.zeros(ne * ne)
+    lmd = np.zeros(ne * ne)
+    work = np.zeros(7 * n2 + c.nion * (5 * c.nfft + 3) + 7 * ne * ne)
+    E = np.ascontiguousarray(E, dtype=np.float64).copy()
+    de, dc, dr = C.c_double(), C.c_double(), C.c_double()
+    L.cpsd(inner, 0, c.nion, 1, nea, c.nfft, c.nfft3d, n2, c.unitg, c.omega, dt, fmass, c.katm,
+           c.amass, c.zv, 27, c.rcell, c.cewald, c.rcut, c.masker, c.G, c.tg, c.vc, c.vg,
+           c.lmmax.copy(), c.vnlnrm.copy(), c.vl, c.vnl, r1, r2, fion, c1, c2, cpsi, dn, hml, lmd,
+           work, E, C.byref(de), C.byref(dc), C.byref(dr))
+    return dict(c1=c1, c2=c2, cpsi=cpsi, dn=dn, hml=hml, lmd=lmd, E=E, deltae=de.value,
+                deltac=dc.value, deltar=dr.value, r1=r1)
+
+
+def fft3d(direction, nfft, a):
+    """Batched in-place FFT of reference-layout arrays: direction +1 is
+    cr_fft3b, -1 is rc_fft3f.  a: (nbatch, (N+2) N N)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    nb = a.size // ((nfft + 2) * nfft * nfft)
+    _check(hip_lib().cpsd_fft3d_batch(direction, nfft, nb, a.reshape(-1)), "cpsd_fft3d_batch")
+    return a
+
+
+def fortran_fft(direction, nfft, a):
+    """One transform through the Fortran-ABI symbols cr_fft3b_/rc_fft3f_."""
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    n = nfft
+    ints = [C.c_int(v) for v in (n // 2 + 1, (n // 2 + 1) * n, n + 2, (n + 2) * n, n, n, n)]
+    tmp = np.zeros(2 * (n * n + 10) + 40 * n)
+    ntmp = C.c_int(tmp.size)
+    f = hip_lib().cr_fft3b_ if direction > 0 else hip_lib().rc_fft3f_
+    f(a, *[C.byref(v) for v in ints], tmp, C.byref(ntmp))
+    return a
+
+
+def lda_device(dn):
+    dn = np.ascontiguousarray(dn, dtype=np.float64)
+    p = np.zeros_like(dn)
+    e = np.zeros_like(dn)
+    _check(hip_lib().cpsd_lda_device(dn.size, dn, p, e), "cpsd_lda_device")
+    return p, e
+
+
+def erfc_device(w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros_like(w)
+    _check(hip_lib().cpsd_erfc_device(w.size, w, out), "cpsd_erfc_device")
+    return out

@@ -1,0 +1,845 @@
+/*
+ * cpsd_api.hip -- host orchestration of the MI355X SD step and the C-ABI
+ * entry points declared in include/cpsd_abi.h.
+ *
+ * One SD step (ccpsdv3/cpsd.c:169-491) on the device, FP64, one HIP stream:
+ *   1  C1 <- C2 (buffer role swap, no copy)                    :173
+ *   2  z pass G->r of every orbital into T                      :177-181
+ *   3  y+x passes per z plane, psi^2 summed per orbital block   :185-194
+ *   4  dn = scal2 sum psi^2, LDA xcp/xce at every point         :185, :370-371
+ *   5  dng = mask(FFT(2 dn / N^3))                              :198-205
+ *   6  V(r) = FFT^-1(vall/Omega + dng vc) + xcp                 :373-382, :388
+ *   7  per plane: psi -> V psi -> forward x,y (in place in T)   :384-392
+ *   8  z pass r->G, CP = -scal1 mask(V psi) - tg C1             :212-221, :394-395
+ *   9  P = beta^T W C1 (split-K MFMA), KB coefficients          :287-289
+ *  10  CP += beta X ; C2 = C1 + dte CP (tall MFMA, fused)       :290, :399-400
+ *  11  Gram of [C1|C2] (split-K MFMA, symmetric tiles)          :415-437
+ *  12  Lagrange fixed point (persistent kernel)                 :439-478
+ *  13  C2 += dte C1 lmd (tall MFMA)                             :482-488
+ * then the energy tail (:494-672).  Structure factors and projectors are
+ * rebuilt from the positions at the start of every call (ions fixed).
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <string>
+
+#include "cpsd_internal.h"
+#include "sd_kernels.h"
+#include "../../include/cpsd_abi.h"
+
+/* ---------------------------------------------------------- errors */
+
+static char g_err[1024];
+static int g_abort_on_error = 1;
+
+void cpsd_set_error(const char *fmt, ...)
+{
+   va_list ap;
+   va_start(ap, fmt);
+   vsnprintf(g_err, sizeof g_err, fmt, ap);
+   va_end(ap);
+}
+
+extern "C" const char *cpsd_last_error(void) { return g_err; }
+extern "C" void cpsd_set_abort_on_error(int on) { g_abort_on_error = on; }
+
+static void fail_loudly(const char *where)
+{
+   fprintf(stderr, "libcpsd_hip: %s: %s\n", where, g_err);
+   fflush(stderr);
+   if (g_abort_on_error) abort();
+}
+
+#define TRY(x) do { if ((x) != 0) return -1; } while (0)
+
+/* ---------------------------------------------------------- context */
+
+enum { PH_ZG2R, PH_DENSITY, PH_DNG, PH_VTOT, PH_VPSI, PH_ZR2G, PH_KBPROJ, PH_KBAPPLY,
+       PH_GRAM, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_COUNT };
+static const char *ph_names[PH_COUNT] = {
+   "zpass_g2r", "plane_density", "dng_r2c", "vtot_c2r", "plane_vpsi", "zpass_r2g_hpsi",
+   "kb_projection", "kb_apply_sd", "gram", "lagrange", "constraint", "energy_tail"};
+
+template<typename T>
+static int dalloc(T **p, size_t n)
+{
+   *p = nullptr;
+   if (n == 0) n = 1;
+   CPSD_CHECK(hipMalloc((void **)p, n * sizeof(T)));
+   return 0;
+}
+
+template<typename T>
+static int h2d(T *d, const T *h, size_t n, hipStream_t s)
+{
+   CPSD_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+   return 0;
+}
+
+struct cpsd_dev {
+   int n, nh, ne, nion, nkind, nproj, ng, ngp, nfft3d, n2ft3d, nhc, nsh;
+   size_t n3;
+   hipStream_t stream;
+   /* packed-G tables */
+   int *rstart, *rlen, *roff, *frstart, *frlen, *froff, *gidx, *pidx, *kx, *ky, *kz;
+   double2 *tw;
+   double *w, *tg, *vc, *vg, *vl, *vnl, *pnrm, *zv, *unitg, *r1, *rcell;
+   int *katm, *lmmax;
+   double2 *ex1, *ex2, *ex3;
+   /* state */
+   double2 *Cbase, *CP;   /* Cbase: 2 orbital blocks */
+   int cur;               /* C2 lives in block cur */
+   double2 *T, *Tv;
+   double *dnpart, *dn, *xcp, *xce, *V;
+   int ndnblk, nob;
+   double2 *dng, *s, *vall, *B;
+   double *partial; size_t partial_cap;
+   double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
+   unsigned *bar; int *err; int *flag;
+   double *gsum_part, *xc_part, *orb_part, *ewald_part, *E;
+   int ngsum_parts, nxc_parts, norb_parts;
+   double2 *fullbuf;      /* ne * nfft3d staging of the reference layout */
+   double *realbuf;       /* n2ft3d staging */
+   double omega, cewald, rcut;
+   int have_run;
+   /* profiling */
+   int prof;
+   hipEvent_t ev0[PH_COUNT], ev1[PH_COUNT];
+   double prof_ms[PH_COUNT];
+   int prof_calls[PH_COUNT];
+   int last_lag_iters; double last_lag_adiff;
+};
+
+static int ph_begin(cpsd_dev *c, int ph)
+{
+   if (c->prof) CPSD_CHECK(hipEventRecord(c->ev0[ph], c->stream));
+   return 0;
+}
+
+static int ph_end(cpsd_dev *c, int ph)
+{
+   if (c->prof) {
+      float ms = 0.f;
+      CPSD_CHECK(hipEventRecord(c->ev1[ph], c->stream));
+      CPSD_CHECK(hipEventSynchronize(c->ev1[ph]));
+      CPSD_CHECK(hipEventElapsedTime(&ms, c->ev0[ph], c->ev1[ph]));
+      c->prof_ms[ph] += ms;
+      c->prof_calls[ph] += 1;
+   }
+   return 0;
+}
+
+static void ctx_free(cpsd_dev *c)
+{
+   void *ptrs[] = {c->rstart, c->rlen, c->roff, c->frstart, c->frlen, c->froff, c->gidx, c->pidx,
+                   c->kx, c->ky, c->kz, c->tw, c->w, c->tg, c->vc, c->vg, c->vl, c->vnl, c->pnrm,
+                   c->zv, c->unitg, c->r1, c->rcell, c->katm, c->lmmax, c->ex1, c->ex2, c->ex3,
+                   c->Cbase, c->CP, c->T, c->Tv, c->dnpart, c->dn, c->xcp, c->xce, c->V, c->dng,
+                   c->s, c->vall, c->B, c->partial, c->Gm, c->P, c->Xkb, c->s11, c->s12, c->s21,
+                   c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
+                   c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
+                   c->fullbuf, c->realbuf};
+   for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+   if (c->stream) {
+      for (int i = 0; i < PH_COUNT; ++i) {
+         (void)hipEventDestroy(c->ev0[i]);
+         (void)hipEventDestroy(c->ev1[i]);
+      }
+      (void)hipStreamDestroy(c->stream);
+   }
+   memset(c, 0, sizeof(*c));
+}
+
+static int splits_for(int M, int N, size_t K, int symm)
+{
+   const int tm = (M + 63) / 64, tn = (N + 63) / 64;
+   int tiles = symm ? tm * (tm + 1) / 2 : tm * tn;
+   int sp = 2048 / (tiles > 0 ? tiles : 1);
+   const int maxsp = (int)(K / (16 * 8));
+   if (sp > maxsp) sp = maxsp;
+   if (sp > 256) sp = 256;
+   if (sp < 1) sp = 1;
+   return sp;
+}
+
+static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, const double *zv,
+                     const int *lmmax, const double *vnlnrm, const double *vl, const double *vnl,
+                     const int *masker, const double *tg, const double *vc, const double *vg,
+                     const double *unitg, double omega, int nsh, const double *rcell,
+                     double cewald, double rcut, const double *r)
+{
+   if (!fft_supported(nfft)) { cpsd_set_error("FFT grid %d is not supported", nfft); return -1; }
+   if (ne < 1 || nion < 1) { cpsd_set_error("need ne >= 1 and nion >= 1 (ne=%d nion=%d)", ne, nion); return -1; }
+   if (ne > 512) { cpsd_set_error("ne=%d above 512 is not supported", ne); return -1; }
+   const int n = nfft, nh = n / 2 + 1;
+   c->n = n; c->nh = nh; c->ne = ne; c->nion = nion; c->nsh = nsh;
+   c->nfft3d = nh * n * n; c->n2ft3d = (n + 2) * n * n; c->n3 = (size_t)n * n * n;
+   c->omega = omega; c->cewald = cewald; c->rcut = rcut;
+   int nkind = 0;
+   for (int i = 0; i < nion; ++i) {
+      if (katm[i] < 0) { cpsd_set_error("katm[%d]=%d is negative", i, katm[i]); return -1; }
+      if (katm[i] + 1 > nkind) nkind = katm[i] + 1;
+   }
+   c->nkind = nkind;
+   int nproj = 0;
+   for (int i = 0; i < nion; ++i) {
+      const int lm = lmmax[katm[i]];
+      if (lm < 0 || lm > 9) { cpsd_set_error("lmmax=%d out of range", lm); return -1; }
+      nproj += lm;
+   }
+   c->nproj = nproj;
+
+   /* masked rows: each (ky,kz) row must hold one contiguous kx interval */
+   std::vector<int> rs(n * n), rl(n * n), ro(n * n), pidx(c->nfft3d, -1);
+   std::vector<int> gidx, kxv, kyv, kzv;
+   int ng = 0, nhc = 0;
+   for (int kz = 0; kz < n; ++kz)
+      for (int ky = 0; ky < n; ++ky) {
+         const int r = ky + n * kz;
+         int first = -1, last = -1;
+         for (int kx = 0; kx < nh; ++kx) {
+            const int g = kx + nh * r;
+            const int m0 = masker[2 * g], m1 = masker[2 * g + 1];
+            if (m0 != m1 || (m0 != 0 && m0 != 1)) {
+               cpsd_set_error("masker must be 0/1 and equal on re/im (entry %d)", g);
+               return -1;
+            }
+            if (m0) {
+               if (first < 0) first = kx;
+               else if (last != kx - 1) { cpsd_set_error("mask row %d is not an interval", r); return -1; }
+               last = kx;
+            }
+         }
+         rs[r] = first < 0 ? 0 : first;
+         rl[r] = first < 0 ? 0 : last - first + 1;
+         ro[r] = ng;
+         for (int kx = rs[r]; kx < rs[r] + rl[r]; ++kx) {
+            pidx[kx + nh * r] = ng++;
+            gidx.push_back(kx + nh * r);
+            kxv.push_back(kx); kyv.push_back(ky); kzv.push_back(kz);
+         }
+         if (rs[r] + rl[r] > nhc) nhc = rs[r] + rl[r];
+      }
+   if (ng == 0) { cpsd_set_error("empty mask"); return -1; }
+   c->ng = ng;
+   c->ngp = (ng + 63) / 64 * 64;
+   c->nhc = nhc;
+   const int ngp = c->ngp;
+   for (int p = ng; p < ngp; ++p) { gidx.push_back(0); kxv.push_back(0); kyv.push_back(0); kzv.push_back(0); }
+
+   std::vector<int> frs(n * n, 0), frl(n * n, nh), fro(n * n);
+   for (int r = 0; r < n * n; ++r) fro[r] = r * nh;
+
+   std::vector<double2> tw(n);
+   for (int m = 0; m < n; ++m) {
+      const double a = 2.0 * M_PI * (double)m / (double)n;
+      tw[m] = make_double2(cos(a), sin(a));
+   }
+   std::vector<double> w(2 * (size_t)ngp, 0.0), tgp(ngp, 0.0), vcp(ngp, 0.0), vgp(ngp, 0.0);
+   std::vector<double> vlp((size_t)nkind * ngp, 0.0), vnlp((size_t)nkind * 9 * ngp, 0.0);
+   for (int p = 0; p < ng; ++p) {
+      const int g = gidx[p];
+      const double wp = (kxv[p] == 0) ? 1.0 : 2.0;
+      w[2 * p] = w[2 * p + 1] = wp;
+      tgp[p] = tg[g]; vcp[p] = vc[g]; vgp[p] = vg[g];
+      for (int k = 0; k < nkind; ++k) {
+         vlp[(size_t)k * ngp + p] = vl[(size_t)k * c->nfft3d + g];
+         for (int l = 0; l < lmmax[k]; ++l)
+            vnlp[((size_t)k * 9 + l) * ngp + p] = vnl[((size_t)k * 9 + l) * c->nfft3d + g];
+      }
+   }
+   std::vector<double> pnrm(nproj > 0 ? nproj : 1);
+   {
+      int m = 0;
+      for (int i = 0; i < nion; ++i)
+         for (int l = 0; l < lmmax[katm[i]]; ++l) pnrm[m++] = vnlnrm[katm[i] * 9 + l];
+   }
+
+   CPSD_CHECK(hipStreamCreate(&c->stream));
+   for (int i = 0; i < PH_COUNT; ++i) {
+      CPSD_CHECK(hipEventCreate(&c->ev0[i]));
+      CPSD_CHECK(hipEventCreate(&c->ev1[i]));
+   }
+   hipStream_t st = c->stream;
+   const size_t nn2 = (size_t)n * n;
+   TRY(dalloc(&c->rstart, nn2)); TRY(dalloc(&c->rlen, nn2)); TRY(dalloc(&c->roff, nn2));
+   TRY(dalloc(&c->frstart, nn2)); TRY(dalloc(&c->frlen, nn2)); TRY(dalloc(&c->froff, nn2));
+   TRY(dalloc(&c->gidx, ngp)); TRY(dalloc(&c->pidx, c->nfft3d));
+   TRY(dalloc(&c->kx, ngp)); TRY(dalloc(&c->ky, ngp)); TRY(dalloc(&c->kz, ngp));
+   TRY(dalloc(&c->tw, n)); TRY(dalloc(&c->w, 2 * (size_t)ngp));
+   TRY(dalloc(&c->tg, ngp)); TRY(dalloc(&c->vc, ngp)); TRY(dalloc(&c->vg, ngp));
+   TRY(dalloc(&c->vl, (size_t)nkind * ngp)); TRY(dalloc(&c->vnl, (size_t)nkind * 9 * ngp));
+   TRY(dalloc(&c->pnrm, pnrm.size())); TRY(dalloc(&c->zv, nkind)); TRY(dalloc(&c->unitg, 9));
+   TRY(dalloc(&c->r1, 3 * (size_t)nion)); TRY(dalloc(&c->rcell, 3 * (size_t)nsh));
+   TRY(dalloc(&c->katm, nion)); TRY(dalloc(&c->lmmax, nkind));
+   TRY(dalloc(&c->ex1, (size_t)nion * nh)); TRY(dalloc(&c->ex2, (size_t)nion * n));
+   TRY(dalloc(&c->ex3, (size_t)nion * n));
+   TRY(h2d(c->rstart, rs.data(), nn2, st)); TRY(h2d(c->rlen, rl.data(), nn2, st));
+   TRY(h2d(c->roff, ro.data(), nn2, st));
+   TRY(h2d(c->frstart, frs.data(), nn2, st)); TRY(h2d(c->frlen, frl.data(), nn2, st));
+   TRY(h2d(c->froff, fro.data(), nn2, st));
+   TRY(h2d(c->gidx, gidx.data(), ngp, st)); TRY(h2d(c->pidx, pidx.data(), c->nfft3d, st));
+   TRY(h2d(c->kx, kxv.data(), ngp, st)); TRY(h2d(c->ky, kyv.data(), ngp, st));
+   TRY(h2d(c->kz, kzv.data(), ngp, st));
+   TRY(h2d(c->tw, tw.data(), n, st)); TRY(h2d(c->w, w.data(), w.size(), st));
+   TRY(h2d(c->tg, tgp.data(), ngp, st)); TRY(h2d(c->vc, vcp.data(), ngp, st));
+   TRY(h2d(c->vg, vgp.data(), ngp, st));
+   TRY(h2d(c->vl, vlp.data(), vlp.size(), st)); TRY(h2d(c->vnl, vnlp.data(), vnlp.size(), st));
+   TRY(h2d(c->pnrm, pnrm.data(), pnrm.size(), st)); TRY(h2d(c->zv, zv, nkind, st));
+   TRY(h2d(c->unitg, unitg, 9, st)); TRY(h2d(c->r1, r, 3 * (size_t)nion, st));
+   TRY(h2d(c->rcell, rcell, 3 * (size_t)nsh, st));
+   TRY(h2d(c->katm, katm, nion, st)); TRY(h2d(c->lmmax, lmmax, nkind, st));
+
+   /* state and scratch */
+   const size_t orb = (size_t)ne * ngp;
+   TRY(dalloc(&c->Cbase, 2 * orb)); TRY(dalloc(&c->CP, orb));
+   CPSD_CHECK(hipMemsetAsync(c->Cbase, 0, 2 * orb * sizeof(double2), st));
+   CPSD_CHECK(hipMemsetAsync(c->CP, 0, orb * sizeof(double2), st));
+   const size_t plane = (size_t)n * n * nh;
+   TRY(dalloc(&c->T, (size_t)ne * plane)); TRY(dalloc(&c->Tv, plane));
+   c->ndnblk = ne < 8 ? ne : 8;
+   c->nob = (ne + c->ndnblk - 1) / c->ndnblk;
+   c->ndnblk = (ne + c->nob - 1) / c->nob;
+   TRY(dalloc(&c->dnpart, (size_t)c->ndnblk * c->n3));
+   TRY(dalloc(&c->dn, c->n3)); TRY(dalloc(&c->xcp, c->n3)); TRY(dalloc(&c->xce, c->n3));
+   TRY(dalloc(&c->V, c->n3));
+   TRY(dalloc(&c->dng, ngp)); TRY(dalloc(&c->s, ngp)); TRY(dalloc(&c->vall, ngp));
+   TRY(dalloc(&c->B, (size_t)(nproj > 0 ? nproj : 1) * ngp));
+   {
+      const size_t K = 2 * (size_t)ngp;
+      size_t cap = 0;
+      int sp = splits_for(2 * ne, 2 * ne, K, 1);
+      cap = (size_t)sp * 4 * ne * ne;
+      sp = splits_for(nproj, ne, K, 0);
+      if ((size_t)sp * nproj * ne > cap) cap = (size_t)sp * nproj * ne;
+      sp = splits_for(ne, ne, K, 0);
+      if ((size_t)sp * ne * ne > cap) cap = (size_t)sp * ne * ne;
+      c->partial_cap = cap;
+      TRY(dalloc(&c->partial, cap));
+   }
+   const size_t ee = (size_t)ne * ne;
+   TRY(dalloc(&c->Gm, 4 * ee)); TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
+   TRY(dalloc(&c->Xkb, (size_t)(nproj > 0 ? nproj : 1) * ne));
+   TRY(dalloc(&c->s11, ee)); TRY(dalloc(&c->s12, ee)); TRY(dalloc(&c->s21, ee)); TRY(dalloc(&c->s22, ee));
+   TRY(dalloc(&c->LX, 2 * ee)); TRY(dalloc(&c->st1, ee)); TRY(dalloc(&c->lmd, ee));
+   TRY(dalloc(&c->wgmax, 1024)); TRY(dalloc(&c->info, 4)); TRY(dalloc(&c->hml, ee)); TRY(dalloc(&c->Mh, ee));
+   TRY(dalloc(&c->bar, 4)); TRY(dalloc(&c->err, 4)); TRY(dalloc(&c->flag, 4));
+   c->ngsum_parts = 256; c->nxc_parts = 256; c->norb_parts = 32;
+   TRY(dalloc(&c->gsum_part, 3 * (size_t)c->ngsum_parts));
+   TRY(dalloc(&c->xc_part, 2 * (size_t)c->nxc_parts));
+   TRY(dalloc(&c->orb_part, 2 * (size_t)c->norb_parts * ne));
+   TRY(dalloc(&c->ewald_part, nion)); TRY(dalloc(&c->E, 16));
+   TRY(dalloc(&c->fullbuf, (size_t)ne * c->nfft3d));
+   TRY(dalloc(&c->realbuf, c->n2ft3d));
+   CPSD_CHECK(hipMemsetAsync(c->err, 0, 4 * sizeof(int), st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   c->cur = 0;
+   return 0;
+}
+
+/* ---------------------------------------------------------- one step */
+
+static int zargs_base(cpsd_dev *c, ZArgs &z, int full)
+{
+   memset(&z, 0, sizeof z);
+   if (full) {
+      z.rows.rstart = c->frstart; z.rows.rlen = c->frlen; z.rows.roff = c->froff; z.rows.nhc = c->nh;
+   } else {
+      z.rows.rstart = c->rstart; z.rows.rlen = c->rlen; z.rows.roff = c->roff; z.rows.nhc = c->nhc;
+   }
+   z.tw = c->tw;
+   return 0;
+}
+
+static int run_structure(cpsd_dev *c)
+{
+   hipStream_t st = c->stream;
+   TRY(sd_phafac(c->n, c->unitg, c->nion, c->r1, c->ex1, c->ex2, c->ex3, st));
+   StructArgs a;
+   a.n = c->n; a.ng = c->ng; a.ngp = c->ngp; a.nion = c->nion; a.nproj = c->nproj;
+   a.kx = c->kx; a.ky = c->ky; a.kz = c->kz;
+   a.ex1 = c->ex1; a.ex2 = c->ex2; a.ex3 = c->ex3;
+   a.katm = c->katm; a.lmmax = c->lmmax; a.zv = c->zv; a.vl = c->vl; a.vnl = c->vnl;
+   a.s = c->s; a.vall = c->vall; a.B = c->B;
+   TRY(sd_structure(a, st));
+   return 0;
+}
+
+static int sd_step(cpsd_dev *c, double dte, int last)
+{
+   hipStream_t st = c->stream;
+   const int n = c->n, ne = c->ne, ngp = c->ngp;
+   const size_t orb = (size_t)ne * ngp;
+   const size_t K = 2 * (size_t)ngp;
+   const double scal1 = 1.0 / ((double)n * n * n), scal2 = 1.0 / c->omega;
+   double2 *C1 = c->Cbase + (size_t)c->cur * orb;          /* previous C2 */
+   double2 *C2 = c->Cbase + (size_t)(c->cur ^ 1) * orb;
+
+   ZArgs z;
+   zargs_base(c, z, 0);
+   /* 2 */
+   TRY(ph_begin(c, PH_ZG2R));
+   z.src = C1; z.sstride = ngp; z.T = c->T;
+   TRY(launch_zpass_g2r(n, SRC_PACKED, z, ne, st));
+   TRY(ph_end(c, PH_ZG2R));
+   /* 3, 4 */
+   TRY(ph_begin(c, PH_DENSITY));
+   PlaneArgs pa;
+   memset(&pa, 0, sizeof pa);
+   pa.T = c->T; pa.nhload = c->nhc; pa.tw = c->tw; pa.b0 = 0; pa.nb = c->nob; pa.nbatch = ne;
+   pa.dnpart = c->dnpart;
+   TRY(launch_plane(n, PM_DENSITY, pa, c->ndnblk, st));
+   TRY(sd_dn_reduce_xc(c->dnpart, c->ndnblk, c->n3, scal2, c->dn, c->xcp, c->xce, st));
+   TRY(ph_end(c, PH_DENSITY));
+   /* 5 */
+   TRY(ph_begin(c, PH_DNG));
+   memset(&pa, 0, sizeof pa);
+   pa.T = c->Tv; pa.tw = c->tw; pa.dn = c->dn; pa.scal1 = scal1; pa.nbatch = 1;
+   TRY(launch_plane(n, PM_R2C_DN, pa, 1, st));
+   zargs_base(c, z, 0);
+   z.T = c->Tv; z.out = c->dng; z.ostride = ngp;
+   TRY(launch_zpass_r2g(n, DST_DNG, z, 1, st));
+   TRY(ph_end(c, PH_DNG));
+   /* 6 */
+   TRY(ph_begin(c, PH_VTOT));
+   zargs_base(c, z, 0);
+   z.T = c->Tv; z.vall = c->vall; z.dng = c->dng; z.vc = c->vc; z.scal2 = scal2;
+   TRY(launch_zpass_g2r(n, SRC_VG, z, 1, st));
+   memset(&pa, 0, sizeof pa);
+   pa.T = c->Tv; pa.nhload = c->nhc; pa.tw = c->tw; pa.Vout = c->V; pa.xcp = c->xcp; pa.nbatch = 1;
+   TRY(launch_plane(n, PM_VTOT, pa, 1, st));
+   TRY(ph_end(c, PH_VTOT));
+   /* 7 */
+   TRY(ph_begin(c, PH_VPSI));
+   memset(&pa, 0, sizeof pa);
+   pa.T = c->T; pa.nhload = c->nhc; pa.tw = c->tw; pa.V = c->V; pa.nbatch = ne;
+   TRY(launch_plane(n, PM_VPSI, pa, ne, st));
+   TRY(ph_end(c, PH_VPSI));
+   /* 8 */
+   TRY(ph_begin(c, PH_ZR2G));
+   zargs_base(c, z, 0);
+   z.T = c->T; z.out = c->CP; z.ostride = ngp; z.c1 = C1; z.tg = c->tg; z.scal1 = scal1;
+   TRY(launch_zpass_r2g(n, DST_HPSI, z, ne, st));
+   TRY(ph_end(c, PH_ZR2G));
+   /* 9 */
+   const double *A2 = reinterpret_cast<const double *>(C1);
+   if (c->nproj > 0) {
+      TRY(ph_begin(c, PH_KBPROJ));
+      const int sp = splits_for(c->nproj, ne, K, 0);
+      TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K,
+                                sp, 0, c->partial, st));
+      TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
+      TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
+      TRY(ph_end(c, PH_KBPROJ));
+   }
+   /* 10 */
+   TRY(ph_begin(c, PH_KBAPPLY));
+   {
+      TallArgs t;
+      t.A = reinterpret_cast<const double *>(c->B); t.lda = K;
+      t.X = c->Xkb; t.ldx = c->nproj; t.Mr = c->nproj; t.Nc = ne; t.K = K;
+      t.C2 = reinterpret_cast<double *>(C2); t.C1 = reinterpret_cast<const double *>(C1);
+      t.CP = reinterpret_cast<double *>(c->CP); t.ldc = K; t.dte = dte; t.write_cp = last;
+      TRY(launch_gemm_tall(EPI_KBAPPLY, t, st));
+   }
+   TRY(ph_end(c, PH_KBAPPLY));
+   /* 11 */
+   TRY(ph_begin(c, PH_GRAM));
+   {
+      const int sp = splits_for(2 * ne, 2 * ne, K, 1);
+      const double *Cb = reinterpret_cast<const double *>(c->Cbase);
+      TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
+      TRY(launch_reduce_partials(c->partial, sp, 2 * ne, 2 * ne, c->Gm, 1, st));
+      TRY(sd_smat(c->Gm, ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, st));
+   }
+   TRY(ph_end(c, PH_GRAM));
+   /* 12 */
+   TRY(ph_begin(c, PH_LAGRANGE));
+   {
+      LagArgs l;
+      l.ne = ne; l.s11 = c->s11; l.s12 = c->s12; l.s21 = c->s21; l.s22 = c->s22;
+      l.X = c->LX; l.st1 = c->st1; l.lmd = c->lmd; l.wgmax = c->wgmax; l.info = c->info;
+      l.bar = c->bar; l.err = c->err;
+      TRY(sd_lagrange(l, st));
+   }
+   TRY(ph_end(c, PH_LAGRANGE));
+   /* 13 */
+   TRY(ph_begin(c, PH_CONSTRAINT));
+   {
+      TallArgs t;
+      t.A = A2; t.lda = K; t.X = c->lmd; t.ldx = ne; t.Mr = ne; t.Nc = ne; t.K = K;
+      t.C2 = reinterpret_cast<double *>(C2); t.C1 = nullptr; t.CP = nullptr; t.ldc = K;
+      t.dte = dte; t.write_cp = 0;
+      TRY(launch_gemm_tall(EPI_CONSTRAINT, t, st));
+   }
+   TRY(ph_end(c, PH_CONSTRAINT));
+   c->cur ^= 1;
+   return 0;
+}
+
+static int energy_tail(cpsd_dev *c, int inner, double dt, double dte)
+{
+   hipStream_t st = c->stream;
+   const int ne = c->ne, ngp = c->ngp;
+   const size_t orb = (size_t)ne * ngp, K = 2 * (size_t)ngp;
+   const double2 *C2 = c->Cbase + (size_t)c->cur * orb;
+   const double2 *C1 = c->Cbase + (size_t)(c->cur ^ 1) * orb;
+   TRY(ph_begin(c, PH_ENERGY));
+   {
+      const int sp = splits_for(ne, ne, K, 0);
+      TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(C1), K, ne,
+                                reinterpret_cast<const double *>(c->CP), K, ne, c->w, K, sp, 0,
+                                c->partial, st));
+      TRY(launch_reduce_partials(c->partial, sp, ne, ne, c->Mh, 0, st));
+      TRY(sd_hml(c->Mh, ne, c->hml, st));
+   }
+   TRY(sd_gsums(c->s, c->dng, c->vall, c->vg, c->vc, c->w, ngp, c->gsum_part, c->ngsum_parts, st));
+   TRY(sd_xcsums(c->dn, c->xce, c->xcp, c->n3, c->xc_part, c->nxc_parts, st));
+   TRY(sd_ewald_real(c->r1, c->katm, c->zv, c->nion, c->rcell, c->nsh, c->rcut, c->ewald_part, st));
+   TRY(sd_orbsums(C1, C2, c->CP, c->tg, c->w, ngp, ne, c->orb_part, c->norb_parts, st));
+   EnergyArgs e;
+   e.ne = ne; e.nion = c->nion; e.nproj = c->nproj; e.inner = inner;
+   e.hml = c->hml; e.P = c->P; e.pnrm = c->pnrm;
+   e.gsum_part = c->gsum_part; e.ngsum_parts = c->ngsum_parts;
+   e.xc_part = c->xc_part; e.nxc_parts = c->nxc_parts;
+   e.orb_part = c->orb_part; e.norb_parts = c->norb_parts;
+   e.ewald_part = c->ewald_part;
+   e.scal2 = 1.0 / c->omega; e.omega = c->omega;
+   e.dv = c->omega / ((double)c->n * c->n * c->n); e.cewald = c->cewald; e.dt = dt; e.dte = dte;
+   e.E = c->E;
+   TRY(sd_energy(e, st));
+   TRY(ph_end(c, PH_ENERGY));
+   return 0;
+}
+
+static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, double *deltae,
+                   double *deltac, double *deltar)
+{
+   if (inner < 1) { cpsd_set_error("inner=%d must be >= 1", inner); return -1; }
+   const double dte = dt / sqrt(fmass);
+   hipStream_t st = c->stream;
+   TRY(h2d(c->E, E, 1, st));
+   TRY(run_structure(c));
+   for (int it = 0; it < inner; ++it) TRY(sd_step(c, dte, it == inner - 1));
+   TRY(energy_tail(c, inner, dt, dte));
+   double out[12], info[2];
+   int err = 0;
+   CPSD_CHECK(hipMemcpyAsync(out, c->E, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipMemcpyAsync(info, c->info, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   if (err) { cpsd_set_error("Lagrange grid barrier timed out"); return -1; }
+   c->last_lag_iters = (int)info[0];
+   c->last_lag_adiff = info[1];
+   if (c->last_lag_iters >= 30 && !(info[1] < 1e-15)) printf("ierr=10 adiff=%f\n", info[1]);
+   for (int i = 0; i < 10; ++i) E[i] = out[i];
+   *deltae = out[10];
+   *deltac = out[11];
+   *deltar = 0.0;
+   c->have_run = 1;
+   return 0;
+}
+
+static int ctx_set_orbitals(cpsd_dev *c, const double *cin)
+{
+   hipStream_t st = c->stream;
+   const size_t fstride = c->nfft3d;
+   CPSD_CHECK(hipMemcpyAsync(c->fullbuf, cin, (size_t)c->ne * c->n2ft3d * sizeof(double),
+                             hipMemcpyHostToDevice, st));
+   CPSD_CHECK(hipMemsetAsync(c->flag, 0, sizeof(int), st));
+   TRY(sd_check_unmasked(c->fullbuf, fstride, c->pidx, c->nfft3d, c->ne, c->flag, st));
+   c->cur = 0;
+   TRY(sd_pack(c->fullbuf, fstride, c->gidx, c->ng, c->ngp, c->Cbase, c->ngp, c->ne, st));
+   int flag = 0;
+   CPSD_CHECK(hipMemcpyAsync(&flag, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   if (flag) {
+      cpsd_set_error("orbital coefficients outside the cutoff mask are nonzero; the GPU path "
+                     "stores masked coefficients only");
+      return -1;
+   }
+   return 0;
+}
+
+static int ctx_get_orb(cpsd_dev *c, const double2 *packed, double *host)
+{
+   hipStream_t st = c->stream;
+   TRY(sd_unpack(packed, c->ngp, c->pidx, c->nfft3d, c->fullbuf, c->nfft3d, c->ne, st));
+   CPSD_CHECK(hipMemcpyAsync(host, c->fullbuf, (size_t)c->ne * c->n2ft3d * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+static int ctx_get(cpsd_dev *c, double *c1, double *c2, double *cpsi, double *dn, double *hml,
+                   double *lmd)
+{
+   const size_t orb = (size_t)c->ne * c->ngp;
+   hipStream_t st = c->stream;
+   if (c1) TRY(ctx_get_orb(c, c->Cbase + (size_t)(c->cur ^ 1) * orb, c1));
+   if (c2) TRY(ctx_get_orb(c, c->Cbase + (size_t)c->cur * orb, c2));
+   if (cpsi) TRY(ctx_get_orb(c, c->CP, cpsi));
+   if (dn) {
+      TRY(sd_pad_real(c->dn, c->n, c->realbuf, st));
+      CPSD_CHECK(hipMemcpyAsync(dn, c->realbuf, (size_t)c->n2ft3d * sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+   }
+   const size_t ee = (size_t)c->ne * c->ne;
+   if (hml) CPSD_CHECK(hipMemcpyAsync(hml, c->hml, ee * sizeof(double), hipMemcpyDeviceToHost, st));
+   if (lmd) CPSD_CHECK(hipMemcpyAsync(lmd, c->lmd, ee * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+/* ---------------------------------------------------- public C-ABI */
+
+extern "C" int cpsd_dev_create(cpsd_dev **h, int nfft, int ne, int nion, const int *katm,
+                               const double *zv, const int *lmmax, const double *vnlnrm,
+                               const double *vl, const double *vnl, const int *masker,
+                               const double *tg, const double *vc, const double *vg,
+                               const double *unitg, double omega, int nsh, const double *rcell,
+                               double cewald, double rcut, const double *r)
+{
+   cpsd_dev *c = (cpsd_dev *)calloc(1, sizeof(cpsd_dev));
+   if (!c) { cpsd_set_error("out of host memory"); return -1; }
+   if (ctx_build(c, nfft, ne, nion, katm, zv, lmmax, vnlnrm, vl, vnl, masker, tg, vc, vg, unitg,
+                 omega, nsh, rcell, cewald, rcut, r) != 0) {
+      ctx_free(c);
+      free(c);
+      *h = nullptr;
+      return -1;
+   }
+   *h = c;
+   return 0;
+}
+
+extern "C" int cpsd_dev_set_orbitals(cpsd_dev *h, const double *c) { return ctx_set_orbitals(h, c); }
+
+extern "C" int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E,
+                            double *deltae, double *deltac, double *deltar)
+{
+   return ctx_run(h, inner, dt, fmass, E, deltae, deltac, deltar);
+}
+
+extern "C" int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn,
+                            double *hml, double *lmd)
+{
+   return ctx_get(h, c1, c2, cpsi, dn, hml, lmd);
+}
+
+extern "C" int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff)
+{
+   *iterations = h->last_lag_iters;
+   *adiff = h->last_lag_adiff;
+   return 0;
+}
+
+extern "C" int cpsd_dev_sizes(cpsd_dev *h, int *sizes)
+{
+   sizes[0] = h->ng; sizes[1] = h->ngp; sizes[2] = h->nproj;
+   return 0;
+}
+
+extern "C" int cpsd_dev_profile(cpsd_dev *h, int enable)
+{
+   h->prof = enable;
+   for (int i = 0; i < PH_COUNT; ++i) { h->prof_ms[i] = 0.0; h->prof_calls[i] = 0; }
+   return 0;
+}
+
+extern "C" int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, double *ms, int *calls,
+                                     int max_phases)
+{
+   std::string s;
+   int k = 0;
+   for (int i = 0; i < PH_COUNT && i < max_phases; ++i, ++k) {
+      if (i) s += ";";
+      s += ph_names[i];
+      ms[i] = h->prof_ms[i];
+      calls[i] = h->prof_calls[i];
+   }
+   if (names && names_len > 0) {
+      strncpy(names, s.c_str(), names_len - 1);
+      names[names_len - 1] = 0;
+   }
+   return k;
+}
+
+extern "C" void cpsd_dev_destroy(cpsd_dev *h)
+{
+   if (!h) return;
+   ctx_free(h);
+   free(h);
+}
+
+/* ------------------------------------------- drop-in cpsd() of cpsd.c */
+
+static cpsd_dev *g_ctx = nullptr;
+
+extern "C" void cpsd(const int inner, const int move, const int nion, const int ispin, const int *ne,
+                     const int nfft, const int nfft3d, const int n2ft3d, const double *unitg,
+                     const double omega, const double dt, const double fmass, const int *katm,
+                     const double *amass, const double *zv, const int nsh, const double *rcell,
+                     const double cewald, const double rcut, const int *masker, const double *G,
+                     const double *tg, const double *vc, const double *vg, int *lmmax,
+                     double *vnlnrm, double *vl, double *vnl, double *r1, double *r2, double *fion,
+                     double *c1, double *c2, double *cpsi, double *dn, double *hml, double *lmd,
+                     double *cpsdwork, double *E, double *deltae, double *deltac, double *deltar)
+{
+   (void)amass; (void)G; (void)fion; (void)cpsdwork;
+   if (ispin != 1 || move != 0 || ne[1] != 0) {
+      cpsd_set_error("only ispin=1, move=0 (ne[1]=0) is supported (got ispin=%d move=%d ne[1]=%d)",
+                     ispin, move, ne[1]);
+      fail_loudly("cpsd");
+      return;
+   }
+   if (nfft3d != (nfft / 2 + 1) * nfft * nfft || n2ft3d != (nfft + 2) * nfft * nfft) {
+      cpsd_set_error("inconsistent grid sizes nfft=%d nfft3d=%d n2ft3d=%d", nfft, nfft3d, n2ft3d);
+      fail_loudly("cpsd");
+      return;
+   }
+   /* r1 <- r2 before anything reads the positions (ccpsdv3/cpsd.c:174) */
+   memcpy(r1, r2, 3 * (size_t)nion * sizeof(double));
+   if (g_ctx) { cpsd_dev_destroy(g_ctx); g_ctx = nullptr; }
+   if (cpsd_dev_create(&g_ctx, nfft, ne[0], nion, katm, zv, lmmax, vnlnrm, vl, vnl, masker, tg, vc,
+                       vg, unitg, omega, nsh, rcell, cewald, rcut, r1) != 0) {
+      fail_loudly("cpsd");
+      return;
+   }
+   if (ctx_set_orbitals(g_ctx, c2) != 0 ||
+       ctx_run(g_ctx, inner, dt, fmass, E, deltae, deltac, deltar) != 0 ||
+       ctx_get(g_ctx, c1, c2, cpsi, dn, hml, lmd) != 0) {
+      fail_loudly("cpsd");
+      return;
+   }
+}
+
+/* ---------------------------------------- FFT entry points (Fortran) */
+
+struct FftCtx {
+   int n, nbatch;
+   hipStream_t stream;
+   int *frs, *frl, *fro;
+   double2 *tw, *T, *A;
+};
+static FftCtx g_fft = {0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+static int fft_ctx(int n, int nbatch)
+{
+   if (!fft_supported(n)) { cpsd_set_error("FFT grid %d is not supported", n); return -1; }
+   if (g_fft.n == n && g_fft.nbatch >= nbatch) return 0;
+   if (g_fft.stream) {
+      (void)hipFree(g_fft.frs); (void)hipFree(g_fft.frl); (void)hipFree(g_fft.fro);
+      (void)hipFree(g_fft.tw); (void)hipFree(g_fft.T); (void)hipFree(g_fft.A);
+   } else {
+      CPSD_CHECK(hipStreamCreate(&g_fft.stream));
+   }
+   const int nh = n / 2 + 1;
+   std::vector<int> frs(n * n, 0), frl(n * n, nh), fro(n * n);
+   for (int r = 0; r < n * n; ++r) fro[r] = r * nh;
+   std::vector<double2> tw(n);
+   for (int m = 0; m < n; ++m) {
+      const double a = 2.0 * M_PI * (double)m / (double)n;
+      tw[m] = make_double2(cos(a), sin(a));
+   }
+   g_fft.n = n; g_fft.nbatch = nbatch;
+   TRY(dalloc(&g_fft.frs, n * n)); TRY(dalloc(&g_fft.frl, n * n)); TRY(dalloc(&g_fft.fro, n * n));
+   TRY(dalloc(&g_fft.tw, n));
+   TRY(dalloc(&g_fft.T, (size_t)nbatch * n * n * nh));
+   TRY(dalloc(&g_fft.A, (size_t)nbatch * n * n * nh));
+   hipStream_t st = g_fft.stream;
+   TRY(h2d(g_fft.frs, frs.data(), n * n, st)); TRY(h2d(g_fft.frl, frl.data(), n * n, st));
+   TRY(h2d(g_fft.fro, fro.data(), n * n, st)); TRY(h2d(g_fft.tw, tw.data(), n, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+static int fft_run(int dir, int n, int nbatch, double *Ah)
+{
+   TRY(fft_ctx(n, nbatch));
+   hipStream_t st = g_fft.stream;
+   const int nh = n / 2 + 1;
+   const size_t per = (size_t)n * n * nh;  /* complex = (n+2) n n / 2 */
+   CPSD_CHECK(hipMemcpyAsync(g_fft.A, Ah, (size_t)nbatch * per * sizeof(double2), hipMemcpyHostToDevice, st));
+   ZArgs z;
+   memset(&z, 0, sizeof z);
+   z.rows.rstart = g_fft.frs; z.rows.rlen = g_fft.frl; z.rows.roff = g_fft.fro; z.rows.nhc = nh;
+   z.tw = g_fft.tw;
+   PlaneArgs pa;
+   memset(&pa, 0, sizeof pa);
+   pa.T = g_fft.T; pa.nhload = nh; pa.tw = g_fft.tw; pa.nbatch = nbatch;
+   if (dir > 0) {
+      z.src = g_fft.A; z.sstride = per; z.T = g_fft.T;
+      TRY(launch_zpass_g2r(n, SRC_PACKED, z, nbatch, st));
+      pa.rout = reinterpret_cast<double *>(g_fft.A);
+      TRY(launch_plane(n, PM_C2R_OUT, pa, nbatch, st));
+   } else {
+      pa.rin = reinterpret_cast<const double *>(g_fft.A);
+      TRY(launch_plane(n, PM_R2C_IN, pa, nbatch, st));
+      z.T = g_fft.T; z.out = g_fft.A; z.ostride = per;
+      TRY(launch_zpass_r2g(n, DST_FULL, z, nbatch, st));
+   }
+   CPSD_CHECK(hipMemcpyAsync(Ah, g_fft.A, (size_t)nbatch * per * sizeof(double2), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+static int fft_args_ok(int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny, int *nz)
+{
+   const int n = *nx;
+   if (*ny != n || *nz != n || *inc2c != n / 2 + 1 || *inc3c != (n / 2 + 1) * n || *inc2r != n + 2 ||
+       *inc3r != (n + 2) * n) {
+      cpsd_set_error("only nx=ny=nz with the reference increments is supported");
+      return -1;
+   }
+   return 0;
+}
+
+extern "C" void cr_fft3b_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny,
+                          int *nz, double *tmp, int *ntmp)
+{
+   (void)tmp; (void)ntmp;
+   if (fft_args_ok(inc2c, inc3c, inc2r, inc3r, nx, ny, nz) != 0 || fft_run(+1, *nx, 1, A) != 0)
+      fail_loudly("cr_fft3b_");
+}
+
+extern "C" void rc_fft3f_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny,
+                          int *nz, double *tmp, int *ntmp)
+{
+   (void)tmp; (void)ntmp;
+   if (fft_args_ok(inc2c, inc3c, inc2r, inc3r, nx, ny, nz) != 0 || fft_run(-1, *nx, 1, A) != 0)
+      fail_loudly("rc_fft3f_");
+}
+
+extern "C" int cpsd_fft3d_batch(int dir, int nfft, int nbatch, double *A)
+{
+   return fft_run(dir, nfft, nbatch, A);
+}
+
+extern "C" int cpsd_lda_device(int n, const double *dn, double *xcp, double *xce)
+{
+   double *d = nullptr, *p = nullptr, *e = nullptr;
+   TRY(dalloc(&d, n)); TRY(dalloc(&p, n)); TRY(dalloc(&e, n));
+   CPSD_CHECK(hipMemcpy(d, dn, n * sizeof(double), hipMemcpyHostToDevice));
+   TRY(sd_lda(d, n, p, e, 0));
+   CPSD_CHECK(hipMemcpy(xcp, p, n * sizeof(double), hipMemcpyDeviceToHost));
+   CPSD_CHECK(hipMemcpy(xce, e, n * sizeof(double), hipMemcpyDeviceToHost));
+   (void)hipFree(d); (void)hipFree(p); (void)hipFree(e);
+   return 0;
+}
+
+extern "C" int cpsd_erfc_device(int n, const double *w, double *out)
+{
+   double *d = nullptr, *o = nullptr;
+   TRY(dalloc(&d, n)); TRY(dalloc(&o, n));
+   CPSD_CHECK(hipMemcpy(d, w, n * sizeof(double), hipMemcpyHostToDevice));
+   TRY(sd_erfc_as(d, n, o, 0));
+   CPSD_CHECK(hipMemcpy(out, o, n * sizeof(double), hipMemcpyDeviceToHost));
+   (void)hipFree(d); (void)hipFree(o);
+   return 0;
+}

@@ -1,0 +1,109 @@
+/*
+ * cpsd_internal.h -- device data layout and kernel launchers of the MI355X
+ * SD step (library-internal).
+ *
+ * HBM layout (FP64 path):
+ *   - orbitals C1, C2, CP: [ne][ngp] complex (interleaved re, im), only the
+ *     masked G vectors are stored ("packed"), ordered as the reference's
+ *     half-complex grid (kx fastest, then ky, then kz; ccpsdv3/get_g.c:20-34).
+ *     Each (ky,kz) row of the grid contributes one contiguous kx interval,
+ *     described by rstart/rlen/roff.  ngp = ng rounded up to 64; the tail
+ *     is zero and carries weight 0.
+ *   - T: z-transformed orbitals [ne][z][ky][kx], kx < N/2+1 (the y/x passes
+ *     read one (ky,kx) plane per z).
+ *   - real-space fields (dn, V, xcp, xce): dense [z][y][x] of N^3 doubles.
+ *   - projectors B: [nproj][ngp] complex, beta_{I,lm}(G) = S_I(G) vnl(G)
+ *     (times i for p channels), ions ordered as in the reference loops.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define CPSD_CHECK(call)                                                       \
+   do {                                                                        \
+      hipError_t e_ = (call);                                                  \
+      if (e_ != hipSuccess) {                                                  \
+         cpsd_set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,     \
+                        hipGetErrorString(e_));                                \
+         return -1;                                                            \
+      }                                                                        \
+   } while (0)
+
+void cpsd_set_error(const char *fmt, ...);
+
+/* ---- FFT ------------------------------------------------------------ */
+
+enum { SRC_PACKED = 0, SRC_VG = 1 };
+enum { DST_HPSI = 0, DST_DNG = 1, DST_FULL = 2 };
+enum { PM_DENSITY = 0, PM_VTOT = 1, PM_VPSI = 2, PM_C2R_OUT = 3, PM_R2C_DN = 4, PM_R2C_IN = 5 };
+
+struct RowTable {
+   const int *rstart, *rlen, *roff;  /* N*N rows each */
+   int nhc;                          /* kx columns that can be nonzero */
+};
+
+struct ZArgs {
+   RowTable rows;
+   const double2 *src;   /* packed (or full-grid) input, batch stride sstride */
+   size_t sstride;
+   double2 *T;           /* [batch][z][ky][NH] */
+   const double2 *tw;    /* exp(+2 pi i m/N), m < N */
+   /* SRC_VG */
+   const double2 *vall, *dng;
+   const double *vc;
+   double scal2;
+   /* r->G outputs */
+   double2 *out;         /* packed or full grid, batch stride ostride */
+   size_t ostride;
+   const double2 *c1;    /* DST_HPSI: kinetic term uses C1 */
+   const double *tg;
+   double scal1;
+};
+
+struct PlaneArgs {
+   double2 *T;           /* [batch][z][ky][NH] (in place for PM_VPSI) */
+   int nhload;           /* kx columns loaded (others are zero) */
+   const double2 *tw;
+   int b0, nb;           /* PM_DENSITY: orbitals [b0 + blockIdx.y*nb, ...) */
+   int nbatch;
+   double *dnpart;       /* PM_DENSITY: [blocks][N^3] */
+   const double *V;      /* PM_VPSI */
+   double *Vout;         /* PM_VTOT */
+   const double *xcp;    /* PM_VTOT */
+   const double *dn;     /* PM_R2C_DN: field 2*dn*scal1 */
+   double scal1;
+   const double *rin;    /* PM_R2C_IN: padded real input [z][y][N+2] */
+   double *rout;         /* PM_C2R_OUT: padded real output */
+};
+
+int fft_supported(int n);
+int launch_zpass_g2r(int n, int src, const ZArgs &a, int nbatch, hipStream_t s);
+int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s);
+int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s);
+
+/* ---- GEMM ------------------------------------------------------------- */
+
+/* partial[z][m][n] = sum_{k in chunk z} w[k] A[m][k] B[n][k]
+ * (A: M rows of stride lda, B: N rows of stride ldb; K multiple of 16).
+ * symm: only tiles with tile_m <= tile_n are computed (A == B).           */
+int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, size_t ldb,
+                          int N, const double *w, size_t K, int splits, int symm,
+                          double *partial, hipStream_t s);
+/* out[m*ldo + n] = sum_z partial[z][m][n] */
+int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
+                           int symm, hipStream_t s);
+
+enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1 };
+struct TallArgs {
+   const double *A;  size_t lda;   /* A rows (m < Mr), contiguous along k */
+   const double *X;  int ldx;      /* X[m + n*ldx] */
+   int Mr, Nc;                     /* reduction length, output columns */
+   size_t K;                       /* output rows (multiple of 64) */
+   double *C2; const double *C1; double *CP; size_t ldc;
+   double dte; int write_cp;
+};
+int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);
+
+/* ---- SD kernels --------------------------------------------------------- */
+int sd_kernels_probe(void);

@@ -1,0 +1,415 @@
+/*
+ * fft_kernels.hip -- batched 3D real<->half-complex FFTs of the SD step on
+ * gfx950, FP64.
+ *
+ * The 3D transform of cr_fft3b / rc_fft3f (fftpack3d/fftpack3d.f:22-96,
+ * 107-196) is split into
+ *   z pass   : one workgroup per (kx block, ky, orbital); the kz column
+ *              block is staged in LDS.  G->r reads the packed masked
+ *              coefficients directly (zero fill outside the sphere); r->G
+ *              writes only masked coefficients, with the SD epilogue fused.
+ *   y+x pass : one workgroup per (z plane, orbital [block]); the whole
+ *              (ky, kx) plane of N x (N/2+1) complex lives in LDS (133 KB at
+ *              N = 128), the y transforms and the real x transform run back
+ *              to back, and the real-space work (density, V*psi) is fused
+ *              between the inverse and forward x transforms.
+ * The y and z passes run over kx = 0..N/2 before the x pass, as the
+ * reference does, so the imaginary parts of kx = 0 and kx = N/2 are dropped
+ * at the same point (the complex-to-real x transform); rc_fft3f's zeroing of
+ * Im(kx=0) and Im(kx=N/2) is reproduced after the forward x transform.
+ */
+#include "cpsd_internal.h"
+#include "fft_core.h"
+
+/* ------------------------------------------------------------ z passes */
+
+template<int N, int SRC>
+__global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
+{
+   using C = GridCfg<N>;
+   constexpr int NH = N / 2 + 1, KXB = C::KXB;
+   __shared__ double2 sm[N * KXB];
+   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
+   const int ncol = min(KXB, a.rows.nhc - kx0);
+   const double2 *src = a.src + (size_t)b * a.sstride;
+
+   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+      const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
+      const int r = ky + N * kz;
+      const int rs = a.rows.rstart[r], rl = a.rows.rlen[r];
+      double2 v = make_double2(0.0, 0.0);
+      if (kxl < ncol && kx >= rs && kx < rs + rl) {
+         const int p = a.rows.roff[r] + kx - rs;
+         if (SRC == SRC_PACKED) {
+            v = src[p];
+         } else {
+            const double2 vl = a.vall[p], dg = a.dng[p];
+            const double c = a.vc[p];
+            v = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
+         }
+      }
+      sm[e] = v;
+   }
+   __syncthreads();
+   lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, a.tw, 1);
+   double2 *T = a.T + (size_t)b * N * N * NH;
+   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+      const int s = e / KXB, kxl = e % KXB;
+      if (kxl < ncol) {
+         const int z = tr_index<C::NY1, C::NY2>(s);
+         T[((size_t)z * N + ky) * NH + kx0 + kxl] = sm[e];
+      }
+   }
+}
+
+template<int N, int DST>
+__global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
+{
+   using C = GridCfg<N>;
+   constexpr int NH = N / 2 + 1, KXB = C::KXB;
+   __shared__ double2 sm[N * KXB];
+   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
+   const int ncol = min(KXB, a.rows.nhc - kx0);
+   const double2 *T = a.T + (size_t)b * N * N * NH;
+
+   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+      const int s = e / KXB, kxl = e % KXB;
+      double2 v = make_double2(0.0, 0.0);
+      if (kxl < ncol) {
+         const int z = tr_index<C::NY1, C::NY2>(s);
+         v = T[((size_t)z * N + ky) * NH + kx0 + kxl];
+      }
+      sm[e] = v;
+   }
+   __syncthreads();
+   lds_fft_tr2nat<C::NY1, C::NY2, -1>(sm, ncol, 1, KXB, a.tw, 1);
+   double2 *out = a.out + (size_t)b * a.ostride;
+   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+      const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
+      const int r = ky + N * kz;
+      const int rs = a.rows.rstart[r], rl = a.rows.rlen[r];
+      if (kxl < ncol && kx >= rs && kx < rs + rl) {
+         const int p = a.rows.roff[r] + kx - rs;
+         const double2 v = sm[e];
+         if (DST == DST_HPSI) {
+            const double2 c = a.c1[(size_t)b * a.ostride + p];
+            const double t = a.tg[p];
+            out[p] = make_double2(-a.scal1 * v.x - t * c.x, -a.scal1 * v.y - t * c.y);
+         } else {
+            out[p] = v;
+         }
+      }
+   }
+}
+
+/* ------------------------------------------------------ plane helpers */
+
+/* Z_k = (X_k + conj X_{M-k}) + i W^k (X_k - conj X_{M-k}), W = e^{2 pi i/N};
+ * only Re X_0 and Re X_M are used (drfftb ignores their imaginary parts). */
+template<int N>
+__device__ __forceinline__ void c2r_pre(double2 *sm, const double2 *__restrict__ tw)
+{
+   constexpr int NH = N / 2 + 1, M = N / 2, KP = M / 2 + 1;
+   for (int it = threadIdx.x; it < N * KP; it += blockDim.x) {
+      const int s = it / KP, k = it % KP;
+      double2 *row = sm + s * NH;
+      if (k == 0) {
+         const double x0 = row[0].x, xm = row[M].x;
+         row[0] = make_double2(x0 + xm, x0 - xm);
+      } else if (k < M - k) {
+         const double2 a = row[k], b = row[M - k];
+         const double2 w = tw[k];
+         const double2 wm = make_double2(-w.x, w.y);  /* W^{M-k} = -conj(W^k) */
+         const double2 za = cadd(cadd(a, cconj(b)), cmuli(cmul(w, csub(a, cconj(b)))));
+         const double2 zb = cadd(cadd(b, cconj(a)), cmuli(cmul(wm, csub(b, cconj(a)))));
+         row[k] = za;
+         row[M - k] = zb;
+      } else if (k == M - k) {
+         const double2 a = row[k];
+         const double2 w = tw[k];
+         row[k] = cadd(cadd(a, cconj(a)), cmuli(cmul(w, csub(a, cconj(a)))));
+      }
+   }
+}
+
+/* X_k = (Z_k + conj Z_{M-k})/2 - (i/2) W^{-k} (Z_k - conj Z_{M-k}), k = 0..M,
+ * with Im X_0 = Im X_M = 0 exactly (fftpack3d.f:160-161). */
+template<int N>
+__device__ __forceinline__ void r2c_post(double2 *sm, const double2 *__restrict__ tw)
+{
+   constexpr int NH = N / 2 + 1, M = N / 2, KP = M / 2 + 1;
+   for (int it = threadIdx.x; it < N * KP; it += blockDim.x) {
+      const int s = it / KP, k = it % KP;
+      double2 *row = sm + s * NH;
+      if (k == 0) {
+         const double2 z0 = row[0];
+         row[0] = make_double2(z0.x + z0.y, 0.0);
+         row[M] = make_double2(z0.x - z0.y, 0.0);
+      } else if (k < M - k) {
+         const double2 za = row[k], zb = row[M - k];
+         const double2 w = cconj(tw[k]);                        /* W^{-k} */
+         const double2 wm = make_double2(-tw[k].x, -tw[k].y);    /* W^{-(M-k)} = -W^k */
+         const double2 sa = cadd(za, cconj(zb)), da = csub(za, cconj(zb));
+         const double2 sb = cadd(zb, cconj(za)), db = csub(zb, cconj(za));
+         const double2 ta = cmuli(cmul(w, da)), tb = cmuli(cmul(wm, db));
+         row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
+         row[M - k] = make_double2(0.5 * (sb.x - tb.x), 0.5 * (sb.y - tb.y));
+      } else if (k == M - k) {
+         const double2 z = row[k];
+         const double2 w = cconj(tw[k]);
+         const double2 sa = cadd(z, cconj(z)), ta = cmuli(cmul(w, csub(z, cconj(z))));
+         row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
+      }
+   }
+}
+
+template<int N>
+__device__ __forceinline__ void load_plane(double2 *sm, const double2 *__restrict__ Tp, int nhload)
+{
+   constexpr int NH = N / 2 + 1;
+   for (int e = threadIdx.x; e < N * NH; e += blockDim.x) {
+      const int kx = e % NH;
+      sm[e] = (kx < nhload) ? Tp[e] : make_double2(0.0, 0.0);
+   }
+   __syncthreads();
+}
+
+/* y inverse (natural ky -> transposed y), then x complex-to-real */
+template<int N>
+__device__ __forceinline__ void inverse_yx(double2 *sm, const double2 *__restrict__ tw)
+{
+   using C = GridCfg<N>;
+   constexpr int NH = N / 2 + 1;
+   lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, NH, 1, NH, tw, 1);
+   c2r_pre<N>(sm, tw);
+   __syncthreads();
+   lds_fft_nat2tr<C::NX1, C::NX2, +1>(sm, N, NH, 1, tw, 2);
+}
+
+/* x real-to-complex then y forward (transposed y -> natural ky) */
+template<int N>
+__device__ __forceinline__ void forward_xy(double2 *sm, const double2 *__restrict__ tw)
+{
+   using C = GridCfg<N>;
+   constexpr int NH = N / 2 + 1;
+   lds_fft_tr2nat<C::NX1, C::NX2, -1>(sm, N, NH, 1, tw, 2);
+   r2c_post<N>(sm, tw);
+   __syncthreads();
+   lds_fft_tr2nat<C::NY1, C::NY2, -1>(sm, NH, 1, NH, tw, 1);
+}
+
+/* real-space position of complex slot m of row slot s (x of the pair's first
+ * real); row slots and complex slots are in transposed order after the
+ * inverse passes */
+template<int N>
+__device__ __forceinline__ void slot_xy(int s, int m, int &y, int &x0)
+{
+   using C = GridCfg<N>;
+   y = tr_index<C::NY1, C::NY2>(s);
+   x0 = 2 * tr_index<C::NX1, C::NX2>(m);
+}
+
+/* ------------------------------------------------------- plane kernel */
+
+template<int N, int MODE>
+__global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS) k_plane(PlaneArgs a)
+{
+   using C = GridCfg<N>;
+   constexpr int NH = N / 2 + 1, M = N / 2, NT = C::PLANE_THREADS;
+   extern __shared__ double2 sm[];
+   const int z = blockIdx.x;
+   const size_t plane = (size_t)N * NH;
+
+   if constexpr (MODE == PM_DENSITY) {
+      constexpr int ACC = (N * N + NT - 1) / NT;
+      double acc[ACC];
+#pragma unroll
+      for (int q = 0; q < ACC; ++q) acc[q] = 0.0;
+      const int bb0 = a.b0 + blockIdx.y * a.nb;
+      const int bb1 = min(bb0 + a.nb, a.nbatch);
+      for (int b = bb0; b < bb1; ++b) {
+         load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
+         inverse_yx<N>(sm, a.tw);
+         const double *smr = reinterpret_cast<const double *>(sm);
+#pragma unroll
+         for (int q = 0; q < ACC; ++q) {
+            const int p = threadIdx.x + q * NT;
+            if (p < N * N) {
+               const int s = p / N, j = p % N;
+               const double v = smr[(size_t)s * 2 * NH + j];
+               acc[q] += v * v;
+            }
+         }
+         __syncthreads();
+      }
+      double *dst = a.dnpart + ((size_t)blockIdx.y * N + z) * N * N;
+#pragma unroll
+      for (int q = 0; q < ACC; ++q) {
+         const int p = threadIdx.x + q * NT;
+         if (p < N * N) {
+            const int s = p / N, j = p % N;
+            int y, x0;
+            slot_xy<N>(s, j >> 1, y, x0);
+            dst[y * N + x0 + (j & 1)] = acc[q];
+         }
+      }
+   } else if constexpr (MODE == PM_VTOT || MODE == PM_C2R_OUT) {
+      const int b = blockIdx.y;
+      load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
+      inverse_yx<N>(sm, a.tw);
+      for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
+         const int s = it / M, m = it % M;
+         int y, x0;
+         slot_xy<N>(s, m, y, x0);
+         const double2 v = sm[s * NH + m];
+         if (MODE == PM_VTOT) {
+            const size_t g = ((size_t)z * N + y) * N + x0;
+            a.Vout[g] = v.x + a.xcp[g];
+            a.Vout[g + 1] = v.y + a.xcp[g + 1];
+         } else {
+            double *row = a.rout + (size_t)b * N * N * (N + 2) + ((size_t)z * N + y) * (N + 2);
+            row[x0] = v.x;
+            row[x0 + 1] = v.y;
+            if (m == 0) { row[N] = 0.0; row[N + 1] = 0.0; }
+         }
+      }
+   } else if constexpr (MODE == PM_VPSI) {
+      const int b = blockIdx.y;
+      double2 *Tp = a.T + ((size_t)b * N + z) * plane;
+      load_plane<N>(sm, Tp, a.nhload);
+      inverse_yx<N>(sm, a.tw);
+      for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
+         const int s = it / M, m = it % M;
+         int y, x0;
+         slot_xy<N>(s, m, y, x0);
+         const size_t g = ((size_t)z * N + y) * N + x0;
+         double2 v = sm[s * NH + m];
+         v.x *= a.V[g];
+         v.y *= a.V[g + 1];
+         sm[s * NH + m] = v;
+      }
+      __syncthreads();
+      forward_xy<N>(sm, a.tw);
+      for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
+   } else {  /* PM_R2C_DN, PM_R2C_IN */
+      const int b = blockIdx.y;
+      for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
+         const int s = it / M, m = it % M;
+         int y, x0;
+         slot_xy<N>(s, m, y, x0);
+         double2 v;
+         if (MODE == PM_R2C_DN) {
+            const size_t g = ((size_t)z * N + y) * N + x0;
+            const double d0 = a.dn[g], d1 = a.dn[g + 1];
+            v = make_double2((d0 + d0) * a.scal1, (d1 + d1) * a.scal1);
+         } else {
+            const double *row = a.rin + (size_t)b * N * N * (N + 2) + ((size_t)z * N + y) * (N + 2);
+            v = make_double2(row[x0], row[x0 + 1]);
+         }
+         sm[s * NH + m] = v;
+      }
+      __syncthreads();
+      forward_xy<N>(sm, a.tw);
+      double2 *Tp = a.T + ((size_t)b * N + z) * plane;
+      for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
+   }
+}
+
+/* ------------------------------------------------------------ launchers */
+
+int fft_supported(int n)
+{
+   switch (n) {
+#define CASE(nn) case nn: return 1;
+      CPSD_FOR_EACH_GRID(CASE)
+#undef CASE
+   default: return 0;
+   }
+}
+
+template<int N>
+static int zg2r(int src, const ZArgs &a, int nbatch, hipStream_t s)
+{
+   constexpr int KXB = GridCfg<N>::KXB;
+   dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
+   if (src == SRC_PACKED) hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED>), grid, dim3(256), 0, s, a);
+   else hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG>), grid, dim3(256), 0, s, a);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+template<int N>
+static int zr2g(int dst, const ZArgs &a, int nbatch, hipStream_t s)
+{
+   constexpr int KXB = GridCfg<N>::KXB;
+   dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
+   if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI>), grid, dim3(256), 0, s, a);
+   else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG>), grid, dim3(256), 0, s, a);
+   else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL>), grid, dim3(256), 0, s, a);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+template<int N, int MODE>
+static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
+{
+   const size_t lds = (size_t)N * (N / 2 + 1) * sizeof(double2);
+   static bool attr_set = false;
+   if (!attr_set) {
+      CPSD_CHECK(hipFuncSetAttribute((const void *)k_plane<N, MODE>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr_set = true;
+   }
+   hipLaunchKernelGGL((k_plane<N, MODE>), dim3(N, nblocks), dim3(GridCfg<N>::PLANE_THREADS),
+                      lds, s, a);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+template<int N>
+static int plane_dispatch(int mode, const PlaneArgs &a, int nblocks, hipStream_t s)
+{
+   switch (mode) {
+   case PM_DENSITY: return plane_launch<N, PM_DENSITY>(a, nblocks, s);
+   case PM_VTOT:    return plane_launch<N, PM_VTOT>(a, nblocks, s);
+   case PM_VPSI:    return plane_launch<N, PM_VPSI>(a, nblocks, s);
+   case PM_C2R_OUT: return plane_launch<N, PM_C2R_OUT>(a, nblocks, s);
+   case PM_R2C_DN:  return plane_launch<N, PM_R2C_DN>(a, nblocks, s);
+   case PM_R2C_IN:  return plane_launch<N, PM_R2C_IN>(a, nblocks, s);
+   }
+   cpsd_set_error("plane: bad mode %d", mode);
+   return -1;
+}
+
+int launch_zpass_g2r(int n, int src, const ZArgs &a, int nbatch, hipStream_t s)
+{
+   switch (n) {
+#define CASE(nn) case nn: return zg2r<nn>(src, a, nbatch, s);
+      CPSD_FOR_EACH_GRID(CASE)
+#undef CASE
+   }
+   cpsd_set_error("FFT grid %d not supported", n);
+   return -1;
+}
+
+int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s)
+{
+   switch (n) {
+#define CASE(nn) case nn: return zr2g<nn>(dst, a, nbatch, s);
+      CPSD_FOR_EACH_GRID(CASE)
+#undef CASE
+   }
+   cpsd_set_error("FFT grid %d not supported", n);
+   return -1;
+}
+
+int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s)
+{
+   switch (n) {
+#define CASE(nn) case nn: return plane_dispatch<nn>(mode, a, nblocks, s);
+      CPSD_FOR_EACH_GRID(CASE)
+#undef CASE
+   }
+   cpsd_set_error("FFT grid %d not supported", n);
+   return -1;
+}

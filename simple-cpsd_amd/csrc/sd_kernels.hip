@@ -1,0 +1,700 @@
+/*
+ * sd_kernels.hip -- elementwise, reduction and small dense kernels of the SD
+ * step (FP64): packing of the half-complex grid, structure factors and
+ * projectors, density reduction + LDA, overlap matrices, the Lagrange
+ * multiplier fixed point, and the energy terms.
+ */
+#include "cpsd_internal.h"
+#include "sd_kernels.h"
+
+/* ------------------------------------------------- packing (boundary) */
+
+__global__ void k_pack(const double2 *__restrict__ full, size_t fstride, const int *__restrict__ gidx,
+                       int ng, int ngp, double2 *__restrict__ packed, size_t pstride, int ne)
+{
+   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int i = blockIdx.y;
+   if (p >= (size_t)ngp || i >= ne) return;
+   packed[i * pstride + p] = (p < (size_t)ng) ? full[i * fstride + gidx[p]] : make_double2(0.0, 0.0);
+}
+
+__global__ void k_check_unmasked(const double2 *__restrict__ full, size_t fstride,
+                                 const int *__restrict__ pidx, int nfft3d, int ne,
+                                 int *__restrict__ flag)
+{
+   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int i = blockIdx.y;
+   if (g >= (size_t)nfft3d || i >= ne) return;
+   if (pidx[g] < 0) {
+      const double2 v = full[i * fstride + g];
+      if (v.x != 0.0 || v.y != 0.0) atomicOr(flag, 1);
+   }
+}
+
+__global__ void k_unpack(const double2 *__restrict__ packed, size_t pstride, const int *__restrict__ pidx,
+                         int nfft3d, double2 *__restrict__ full, size_t fstride, int ne)
+{
+   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const int i = blockIdx.y;
+   if (g >= (size_t)nfft3d || i >= ne) return;
+   const int p = pidx[g];
+   full[i * fstride + g] = (p >= 0) ? packed[i * pstride + p] : make_double2(0.0, 0.0);
+}
+
+/* dense N^3 real field -> padded (N+2) rows with zeroed pads */
+__global__ void k_pad_real(const double *__restrict__ dense, int n, double *__restrict__ padded)
+{
+   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const size_t rows = (size_t)n * n;
+   if (idx >= rows * (n + 2)) return;
+   const size_t r = idx / (n + 2);
+   const int x = (int)(idx % (n + 2));
+   padded[idx] = (x < n) ? dense[r * n + x] : 0.0;
+}
+
+/* ------------------------------------- structure factors and projectors */
+
+/* Per packed G: exi_I = ex1[kx] * (ex2[ky] * ex3[kz]) (ccpsdv3/strfac.c:93-123),
+ * s = sum_I zv exi_I, vall = sum_I vl exi_I (ccpsdv3/cpsd.c:232-245), and the
+ * projector rows beta_{I,l} = exi_I vnl_l (times i for l = 1..3,
+ * ccpsdv3/cpsd.c:260-286), ions in input order. */
+__global__ void k_structure(StructArgs a)
+{
+   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+   if (p >= a.ngp) return;
+   double2 s = make_double2(0.0, 0.0), vall = make_double2(0.0, 0.0);
+   if (p >= a.ng) {
+      a.s[p] = s; a.vall[p] = vall;
+      for (int m = 0; m < a.nproj; ++m) a.B[(size_t)m * a.ngp + p] = s;
+      return;
+   }
+   const int kx = a.kx[p], ky = a.ky[p], kz = a.kz[p];
+   int m = 0;
+   for (int ii = 0; ii < a.nion; ++ii) {
+      const int ia = a.katm[ii];
+      const double2 e1 = a.ex1[ii * (a.n + 2) / 2 + kx];
+      const double2 e2 = a.ex2[ii * a.n + ky];
+      const double2 e3 = a.ex3[ii * a.n + kz];
+      const double2 t = make_double2(e2.x * e3.x - e2.y * e3.y, e2.x * e3.y + e2.y * e3.x);
+      const double2 e = make_double2(e1.x * t.x - e1.y * t.y, e1.y * t.x + e1.x * t.y);
+      const double z = a.zv[ia];
+      s.x += z * e.x; s.y += z * e.y;
+      const double vl = a.vl[(size_t)ia * a.ngp + p];
+      vall.x += e.x * vl; vall.y += e.y * vl;
+      for (int l = 0; l < a.lmmax[ia]; ++l) {
+         const double v = a.vnl[((size_t)ia * 9 + l) * a.ngp + p];
+         double2 b;
+         if (l == 0 || l >= 4) b = make_double2(e.x * v, e.y * v);
+         else b = make_double2(-e.y * v, e.x * v);
+         a.B[(size_t)m * a.ngp + p] = b;
+         ++m;
+      }
+   }
+   a.s[p] = s;
+   a.vall[p] = vall;
+}
+
+/* Phase factors by repeated complex multiplication (ccpsdv3/strfac.c:10-77):
+ * ex1[k] = e^{-i(b1.R+pi)k}, k = 0..N/2; ex2/ex3 over k = 0..N-1 with
+ * ex[N-k] = conj(ex[k]); the Nyquist entries are zero.  One thread per
+ * (ion, direction); pi = 4 atan(1) as the reference passes it. */
+__global__ void k_phafac(int n, const double *__restrict__ unitg, int nion, const double *__restrict__ r1,
+                         double2 *__restrict__ ex1, double2 *__restrict__ ex2, double2 *__restrict__ ex3)
+{
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= 3 * nion) return;
+   const int ii = t / 3, d = t % 3;
+   const double pi = 4.0 * atan(1.0);
+   const double sw = unitg[3 * d] * r1[3 * ii] + unitg[3 * d + 1] * r1[3 * ii + 1]
+                   + unitg[3 * d + 2] * r1[3 * ii + 2] + pi;
+   const double cr = cos(sw), ci = -sin(sw);
+   const int nh = n / 2;
+   double2 *x = (d == 0) ? ex1 + (size_t)ii * (nh + 1) : ((d == 1) ? ex2 : ex3) + (size_t)ii * n;
+   double pr = 1.0, pim = 0.0;
+   x[0] = make_double2(1.0, 0.0);
+   for (int k = 1; k <= nh; ++k) {
+      const double nr = pr * cr - pim * ci;
+      const double ni = pr * ci + pim * cr;
+      pr = nr; pim = ni;
+      x[k] = make_double2(nr, ni);
+      if (d > 0) x[n - k] = make_double2(nr, -ni);
+   }
+   x[nh] = make_double2(0.0, 0.0);
+}
+
+int sd_phafac(int n, const double *unitg, int nion, const double *r1, double2 *ex1, double2 *ex2,
+              double2 *ex3, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_phafac, dim3((3 * nion + 63) / 64), dim3(64), 0, s, n, unitg, nion, r1, ex1, ex2, ex3);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* ------------------------------------------------------ density + LDA */
+
+__device__ __forceinline__ void lda_point(double d, double &xcp, double &xce)
+{
+   /* Slater exchange, ccpsdv3/dirac_exchange.c:84-91 */
+   const double alpha = 0.6666666666666666667;
+   const double onethird_x = 0.33333333333333333333;
+   const double pi_x = 3.14159265358979311600;
+   const double nn = d + d;
+   const double n13 = pow(3.0 * nn / pi_x, onethird_x);
+   const double ex = -(9.0 / 8.0) * alpha * n13;
+   const double vx = -(3.0 / 2.0) * alpha * n13;
+   /* VWN correlation, ccpsdv3/vosko.c:99-167 with nup = ndown (xi = 0) */
+   const double bp = 3.727440e+00, cp = 1.293520e+01;
+   const double cp1 = 3.109070e-02, cp2 = 9.690228e-04, cp3 = 1.049800e-01;
+   const double cp4 = 3.878329e-02, cp5 = 3.075995e+00, cp6 = 1.863720e+00;
+   const double dp1 = 6.218140e-02, dp2 = 1.938045e-03;
+   const double dp4 = -3.205972e-02, dp5 = -1.192972e-01, dp6 = 1.863720e+00;
+   const double dp7 = 9.461748e+00;
+   const double onesixth = 0.166666666666667, onethird = 0.333333333333333;
+   const double rs_scale = 0.620350490899400;
+   const double nup = d + 1.0e-30;
+   const double nt = nup + nup;
+   const double rs = rs_scale / pow(nt, onethird);
+   const double x = sqrt(rs);
+   const double xxp = rs + bp * x + cp;
+   const double dxxp = 2.0 * x + bp;
+   const double ecp = cp1 * log(rs / xxp) + cp2 * log((x + cp3) * (x + cp3) / xxp)
+                    + cp4 * atan(cp5 / (x + cp6));
+   const double ucp = ecp - onesixth * x * (dp1 / x + dp2 / (x + cp3) + dp4 * dxxp / xxp
+                    + dp5 / ((x + dp6) * (x + dp6) + dp7));
+   xce = ex + ecp;
+   xcp = vx + ucp;
+}
+
+__global__ void k_dn_reduce_xc(const double *__restrict__ dnpart, int nparts, size_t n3, double scal2,
+                               double *__restrict__ dn, double *__restrict__ xcp, double *__restrict__ xce)
+{
+   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (g >= n3) return;
+   double s = 0.0;
+   for (int b = 0; b < nparts; ++b) s += dnpart[b * n3 + g];
+   const double d = s * scal2;
+   dn[g] = d;
+   double p, e;
+   lda_point(d, p, e);
+   xcp[g] = p;
+   xce[g] = e;
+}
+
+/* standalone LDA on a padded n2ft3d array (test hook of the XC kernel) */
+__global__ void k_lda(const double *__restrict__ dn, size_t n, double *__restrict__ xcp, double *__restrict__ xce)
+{
+   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (g >= n) return;
+   double p, e;
+   lda_point(dn[g], p, e);
+   xcp[g] = p;
+   xce[g] = e;
+}
+
+/* --------------------------------------------------------- reductions */
+
+template<int NT>
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+   red[threadIdx.x] = v;
+   __syncthreads();
+#pragma unroll
+   for (int s = NT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+   }
+   const double r = red[0];
+   __syncthreads();
+   return r;
+}
+
+/* per block: sums over packed G of w|s|^2 vg, w|dng|^2 vc, w Re(vall* dng) */
+__global__ void __launch_bounds__(256)
+k_gsums(const double2 *__restrict__ s, const double2 *__restrict__ dng, const double2 *__restrict__ vall,
+        const double *__restrict__ vg, const double *__restrict__ vc, const double *__restrict__ w,
+        int ngp, double *__restrict__ part)
+{
+   __shared__ double red[256];
+   double a = 0.0, b = 0.0, c = 0.0;
+   for (int p = blockIdx.x * 256 + threadIdx.x; p < ngp; p += gridDim.x * 256) {
+      const double wp = w[2 * p];
+      const double2 sp = s[p], dp = dng[p], vp = vall[p];
+      a += wp * ((sp.x * sp.x + sp.y * sp.y) * vg[p]);
+      b += wp * ((dp.x * dp.x + dp.y * dp.y) * vc[p]);
+      c += wp * (vp.x * dp.x + vp.y * dp.y);
+   }
+   a = block_sum<256>(a, red);
+   b = block_sum<256>(b, red);
+   c = block_sum<256>(c, red);
+   if (threadIdx.x == 0) { part[3 * blockIdx.x] = a; part[3 * blockIdx.x + 1] = b; part[3 * blockIdx.x + 2] = c; }
+}
+
+/* per block: sum dn*xce, dn*xcp over the dense grid */
+__global__ void __launch_bounds__(256)
+k_xcsums(const double *__restrict__ dn, const double *__restrict__ xce, const double *__restrict__ xcp,
+         size_t n3, double *__restrict__ part)
+{
+   __shared__ double red[256];
+   double a = 0.0, b = 0.0;
+   for (size_t g = (size_t)blockIdx.x * 256 + threadIdx.x; g < n3; g += (size_t)gridDim.x * 256) {
+      a += dn[g] * xce[g];
+      b += dn[g] * xcp[g];
+   }
+   a = block_sum<256>(a, red);
+   b = block_sum<256>(b, red);
+   if (threadIdx.x == 0) { part[2 * blockIdx.x] = a; part[2 * blockIdx.x + 1] = b; }
+}
+
+/* per orbital: eke_i = sum w tg |c1|^2 ; cp = c2 - c1 ; dc_i = sum w |cp|^2.
+ * grid (blocks, ne): partials [ne][blocks][2] */
+__global__ void __launch_bounds__(256)
+k_orbsums(const double2 *__restrict__ c1, const double2 *__restrict__ c2, double2 *__restrict__ cp,
+          const double *__restrict__ tg, const double *__restrict__ w, int ngp, double *__restrict__ part)
+{
+   __shared__ double red[256];
+   const int i = blockIdx.y;
+   const size_t off = (size_t)i * ngp;
+   double a = 0.0, b = 0.0;
+   for (int p = blockIdx.x * 256 + threadIdx.x; p < ngp; p += gridDim.x * 256) {
+      const double2 x = c1[off + p], y = c2[off + p];
+      const double2 d = make_double2(y.x - x.x, y.y - x.y);
+      cp[off + p] = d;
+      const double wp = w[2 * p];
+      a += wp * (x.x * (x.x * tg[p]) + x.y * (x.y * tg[p]));
+      b += wp * (d.x * d.x + d.y * d.y);
+   }
+   a = block_sum<256>(a, red);
+   b = block_sum<256>(b, red);
+   if (threadIdx.x == 0) {
+      part[((size_t)i * gridDim.x + blockIdx.x) * 2] = a;
+      part[((size_t)i * gridDim.x + blockIdx.x) * 2 + 1] = b;
+   }
+}
+
+/* erfc of ccpsdv3/cpsd.c:11-17, used as (1/aa^4)^4 at :541-543 */
+__device__ __forceinline__ double erfc_as(double w)
+{
+   const double B1 = 0.0705230784, B2 = 0.0422820123, B3 = 0.0092705272;
+   const double B4 = 0.0001520143, B5 = 0.0002765672, B6 = 0.0000430638;
+   const double aa = 1.0 + w * (B1 + w * (B2 + w * (B3 + w * (B4 + w * (B5 + w * B6)))));
+   const double e = 1.0 / (aa * aa * aa * aa);
+   return e * e * e * e;
+}
+
+/* real-space Ewald pairs (ccpsdv3/cpsd.c:527-545): one thread per ion
+ * ii, summing its pairs jj > ii in order; per-ion partials */
+__global__ void __launch_bounds__(256)
+k_ewald_real(const double *__restrict__ r1, const int *__restrict__ katm, const double *__restrict__ zv,
+             int nion, const double *__restrict__ rcell, int nsh, double rcut, double *__restrict__ part)
+{
+   const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+   if (ii >= nion) return;
+   double e = 0.0;
+   for (int jj = ii + 1; jj < nion; ++jj) {
+      const double dx = r1[3 * ii] - r1[3 * jj];
+      const double dy = r1[3 * ii + 1] - r1[3 * jj + 1];
+      const double dz = r1[3 * ii + 2] - r1[3 * jj + 2];
+      const double zz = zv[katm[ii]] * zv[katm[jj]];
+      for (int l = 0; l < nsh; ++l) {
+         const double x = rcell[l] + dx, y = rcell[l + nsh] + dy, z = rcell[l + 2 * nsh] + dz;
+         const double r = sqrt(x * x + y * y + z * z);
+         e += zz * erfc_as(r / rcut) / r;
+      }
+   }
+   part[ii] = e;
+}
+
+__global__ void k_erfc_as(const double *__restrict__ w, int n, double *__restrict__ out)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) out[i] = erfc_as(w[i]);
+}
+
+/* ------------------------------------------- overlaps -> s matrices */
+
+/* Gram matrix Gm (2ne x 2ne, row-major) of the two orbital blocks (C1 first
+ * unless c1_second) -> the four ne x ne
+ * matrices of ccpsdv3/cpsd.c:415-437 (column-major).  s21 and s12 are each
+ * filled from one triangle and mirrored, as the reference does. */
+__global__ void k_smat(const double *__restrict__ Gm, int ne, int c1_second, double dte, double *__restrict__ s11,
+                       double *__restrict__ s12, double *__restrict__ s21, double *__restrict__ s22)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= ne * ne) return;
+   const int r = idx % ne, c = idx / ne;   /* element (r, c) at r + c*ne */
+   const int i = min(r, c), j = max(r, c);
+   const int n2 = 2 * ne;
+   const int o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;   /* row offsets of C1, C2 */
+   const double g11 = Gm[(size_t)(o1 + i) * n2 + o1 + j];
+   const double g22 = Gm[(size_t)(o2 + i) * n2 + o2 + j];
+   const double g21 = Gm[(size_t)(o2 + i) * n2 + o1 + j];      /* <c2_i|c1_j> */
+   const double g12 = Gm[(size_t)(o1 + i) * n2 + o2 + j];      /* <c1_i|c2_j> */
+   if (i == j) {
+      s22[idx] = (1.0 - g22) * 0.5 / dte;
+      s21[idx] = (1.0 - g21) * 0.5;
+      s12[idx] = s21[idx];
+      s11[idx] = -g11 * 0.5 * dte;
+   } else {
+      s22[idx] = -g22 * 0.5 / dte;
+      s21[idx] = -g21 * 0.5;
+      s12[idx] = -g12 * 0.5;
+      s11[idx] = -g11 * 0.5 * dte;
+   }
+}
+
+/* P (nproj x ne, row-major, P[m][n] = gcdotc(beta_m, c1_n)) -> the apply
+ * coefficients X[m + n*nproj] = -P scal2 / vnlnrm_m (ccpsdv3/cpsd.c:289) */
+__global__ void k_kbcoef(const double *__restrict__ P, int nproj, int ne, const double *__restrict__ pnrm,
+                         double scal2, double *__restrict__ X)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= nproj * ne) return;
+   const int m = idx % nproj, n = idx / nproj;
+   X[idx] = -P[(size_t)m * ne + n] * scal2 / pnrm[m];
+}
+
+/* ---------------------------------------- Lagrange multiplier fixed point */
+
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *err)
+{
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(bar, 1u);
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+         __builtin_amdgcn_s_sleep(4);
+         if (++spins > (1u << 21)) { atomicExch(err, 1); break; }
+      }
+      __threadfence();
+   }
+   __syncthreads();
+}
+
+/* C_tile += A * B over k (column-major ne x ne), TS x TS tile at (ti, tj);
+ * one or more outputs per thread accumulated in acc[]. */
+template<int TS, int PER>
+__device__ __forceinline__ void tile_mm(const double *A, const double *B, int ne, int ti, int tj,
+                                        double (&acc)[PER], double (*sa)[33], double (*sb)[33])
+{
+   /* chunks of 32 along k: sa[r][kk] = A[ti*TS + r, k0+kk], sb[c][kk] = B[k0+kk, tj*TS + c] */
+   for (int k0 = 0; k0 < ne; k0 += 32) {
+      for (int e = threadIdx.x; e < TS * 32; e += blockDim.x) {
+         const int r = e % TS, kk = e / TS;
+         const int row = ti * TS + r, k = k0 + kk;
+         sa[r][kk] = (row < ne && k < ne) ? A[row + (size_t)k * ne] : 0.0;
+      }
+      for (int e = threadIdx.x; e < TS * 32; e += blockDim.x) {
+         const int kk = e % 32, c = e / 32;
+         const int col = tj * TS + c, k = k0 + kk;
+         sb[c][kk] = (col < ne && k < ne) ? B[k + (size_t)col * ne] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+         const int o = threadIdx.x + q * blockDim.x;
+         const int r = o % TS, c = o / TS;
+         double s = 0.0;
+#pragma unroll 8
+         for (int kk = 0; kk < 32; ++kk) s += sa[r][kk] * sb[c][kk];
+         acc[q] += s;
+      }
+      __syncthreads();
+   }
+}
+
+/* X <- s22 + s21 X + X s12 + X (s11 X), from X0 = s22, until max|dX| < 1e-15
+ * or 30 iterations (ccpsdv3/cpsd.c:439-478, ITERLMD/CONVGLMD at :19-20).
+ * Persistent grid of (ne/TS)^2 workgroups (all co-resident) with a bounded
+ * spinning grid barrier; info[0] = iterations, info[1] = last max|dX|. */
+template<int TS>
+__global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
+{
+   constexpr int PER = TS * TS / 256;
+   __shared__ double sa[TS][33];
+   __shared__ double sb[TS][33];
+   __shared__ double red[256];
+   const int nt = (a.ne + TS - 1) / TS;
+   const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+   const int nwg = gridDim.x;
+   const size_t nn = (size_t)a.ne * a.ne;
+   unsigned nbar = 0;
+   int it = 0;
+   double gmax = 0.0;
+   /* X_0 = s22 */
+   for (int q = 0; q < PER; ++q) {
+      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+      if (r < a.ne && c < a.ne) a.X[r + (size_t)c * a.ne] = a.s22[r + (size_t)c * a.ne];
+   }
+   grid_barrier(a.bar, (++nbar) * nwg, a.err);
+   while (it < 30) {
+      const double *X = a.X + (size_t)(it & 1) * nn;
+      double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
+      ++it;
+      double acc[PER];
+      for (int q = 0; q < PER; ++q) acc[q] = 0.0;
+      tile_mm<TS, PER>(a.s11, X, a.ne, ti, tj, acc, sa, sb);
+      for (int q = 0; q < PER; ++q) {
+         const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+         if (r < a.ne && c < a.ne) a.st1[r + (size_t)c * a.ne] = acc[q];
+      }
+      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      for (int q = 0; q < PER; ++q) acc[q] = 0.0;
+      tile_mm<TS, PER>(a.s21, X, a.ne, ti, tj, acc, sa, sb);
+      tile_mm<TS, PER>(X, a.s12, a.ne, ti, tj, acc, sa, sb);
+      tile_mm<TS, PER>(X, a.st1, a.ne, ti, tj, acc, sa, sb);
+      double lmax = 0.0;
+      for (int q = 0; q < PER; ++q) {
+         const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+         if (r < a.ne && c < a.ne) {
+            const size_t e = r + (size_t)c * a.ne;
+            const double v = a.s22[e] + acc[q];
+            Xn[e] = v;
+            lmax = fmax(lmax, fabs(v - X[e]));
+         }
+      }
+      red[threadIdx.x] = lmax;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+         __syncthreads();
+      }
+      if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
+      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      gmax = 0.0;
+      for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (gmax < 1e-15) break;
+      if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+   }
+   /* lmd = the last iterate (sa1 of the final iteration) */
+   const double *Xf = a.X + (size_t)(it & 1) * nn;
+   for (int q = 0; q < PER; ++q) {
+      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+      if (r < a.ne && c < a.ne) a.lmd[r + (size_t)c * a.ne] = Xf[r + (size_t)c * a.ne];
+   }
+   if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.info[0] = (double)it;
+      a.info[1] = gmax;
+   }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = 0u;
+}
+
+/* ------------------------------------------------------ energy assembly */
+
+/* one thread: combine the partial sums into E[0..9], deltae, deltac
+ * (ccpsdv3/cpsd.c:515-660) */
+__global__ void k_energy(EnergyArgs a)
+{
+   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+   double eorbit = 0.0;
+   for (int i = 0; i < a.ne; ++i) eorbit += a.hml[i + (size_t)i * a.ne];
+   eorbit *= 2;
+   double sg = 0.0, sh = 0.0, sl = 0.0;
+   for (int b = 0; b < a.ngsum_parts; ++b) {
+      sg += a.gsum_part[3 * b]; sh += a.gsum_part[3 * b + 1]; sl += a.gsum_part[3 * b + 2];
+   }
+   double eion = sg * 0.5 * a.scal2 + a.cewald;
+   for (int ii = 0; ii < a.nion; ++ii) eion += a.ewald_part[ii];
+   const double ehartr = sh * a.omega * 0.5;
+   double exc = 0.0, pxc = 0.0;
+   for (int b = 0; b < a.nxc_parts; ++b) { exc += a.xc_part[2 * b]; pxc += a.xc_part[2 * b + 1]; }
+   exc += exc; pxc += pxc;
+   exc *= a.dv; pxc *= a.dv;
+   double eke = 0.0, dcmax = 0.0;
+   for (int i = 0; i < a.ne; ++i) {
+      double ek = 0.0, dc = 0.0;
+      for (int b = 0; b < a.norb_parts; ++b) {
+         ek += a.orb_part[((size_t)i * a.norb_parts + b) * 2];
+         dc += a.orb_part[((size_t)i * a.norb_parts + b) * 2 + 1];
+      }
+      eke += ek;
+      if (dc > dcmax) dcmax = dc;
+   }
+   eke *= 2;
+   const double elocal = sl;
+   double enl = 0.0;
+   for (int m = 0; m < a.nproj; ++m)
+      for (int n = 0; n < a.ne; ++n) {
+         const double p = a.P[(size_t)m * a.ne + n];
+         enl += p * p * a.scal2 / a.pnrm[m];
+      }
+   enl *= 2;
+   const double eold = a.E[0];
+   double *E = a.E;
+   E[0] = eorbit + eion + exc - ehartr - pxc;
+   E[1] = eorbit;
+   E[2] = ehartr;
+   E[3] = exc;
+   E[4] = eion;
+   E[5] = eke;
+   E[6] = elocal;
+   E[7] = enl;
+   E[8] = 2.0 * ehartr;
+   E[9] = pxc;
+   E[10] = (E[0] - eold) / (a.dt * a.inner);   /* deltae */
+   E[11] = dcmax / a.dte;                        /* deltac */
+}
+
+/* hml from M[i][j] = gcdotc(c1_i, cp_j) (row-major): upper triangle
+ * -M[i][j], mirrored (ccpsdv3/cpsd.c:504-513) */
+__global__ void k_hml(const double *__restrict__ M, int ne, double *__restrict__ hml)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= ne * ne) return;
+   const int r = idx % ne, c = idx / ne;
+   const int i = min(r, c), j = max(r, c);
+   hml[idx] = -M[(size_t)i * ne + j];
+}
+
+/* ------------------------------------------------------------ launchers */
+
+static inline dim3 g1(size_t n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+int sd_pack(const double2 *full, size_t fstride, const int *gidx, int ng, int ngp, double2 *packed,
+            size_t pstride, int ne, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_pack, dim3((ngp + 255) / 256, ne), dim3(256), 0, s, full, fstride, gidx, ng, ngp,
+                      packed, pstride, ne);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_check_unmasked(const double2 *full, size_t fstride, const int *pidx, int nfft3d, int ne,
+                      int *flag, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_check_unmasked, dim3((nfft3d + 255) / 256, ne), dim3(256), 0, s, full, fstride,
+                      pidx, nfft3d, ne, flag);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_unpack(const double2 *packed, size_t pstride, const int *pidx, int nfft3d, double2 *full,
+              size_t fstride, int ne, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_unpack, dim3((nfft3d + 255) / 256, ne), dim3(256), 0, s, packed, pstride, pidx,
+                      nfft3d, full, fstride, ne);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_pad_real(const double *dense, int n, double *padded, hipStream_t s)
+{
+   const size_t tot = (size_t)n * n * (n + 2);
+   hipLaunchKernelGGL(k_pad_real, g1(tot), dim3(256), 0, s, dense, n, padded);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_structure(const StructArgs &a, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_structure, g1(a.ngp), dim3(256), 0, s, a);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_dn_reduce_xc(const double *dnpart, int nparts, size_t n3, double scal2, double *dn, double *xcp,
+                    double *xce, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_dn_reduce_xc, g1(n3), dim3(256), 0, s, dnpart, nparts, n3, scal2, dn, xcp, xce);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_lda(const double *dn, size_t n, double *xcp, double *xce, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_lda, g1(n), dim3(256), 0, s, dn, n, xcp, xce);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_gsums(const double2 *sg, const double2 *dng, const double2 *vall, const double *vg, const double *vc,
+             const double *w, int ngp, double *part, int nparts, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_gsums, dim3(nparts), dim3(256), 0, s, sg, dng, vall, vg, vc, w, ngp, part);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_xcsums(const double *dn, const double *xce, const double *xcp, size_t n3, double *part, int nparts,
+              hipStream_t s)
+{
+   hipLaunchKernelGGL(k_xcsums, dim3(nparts), dim3(256), 0, s, dn, xce, xcp, n3, part);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_orbsums(const double2 *c1, const double2 *c2, double2 *cp, const double *tg, const double *w, int ngp,
+               int ne, double *part, int nparts, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_orbsums, dim3(nparts, ne), dim3(256), 0, s, c1, c2, cp, tg, w, ngp, part);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_ewald_real(const double *r1, const int *katm, const double *zv, int nion, const double *rcell,
+                  int nsh, double rcut, double *part, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_ewald_real, g1(nion), dim3(256), 0, s, r1, katm, zv, nion, rcell, nsh, rcut, part);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_erfc_as(const double *w, int n, double *out, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_erfc_as, g1(n), dim3(256), 0, s, w, n, out);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_smat(const double *Gm, int ne, int c1_second, double dte, double *s11, double *s12, double *s21,
+            double *s22, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_smat, g1((size_t)ne * ne), dim3(256), 0, s, Gm, ne, c1_second, dte, s11, s12, s21, s22);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double scal2, double *X, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_kbcoef, g1((size_t)nproj * ne), dim3(256), 0, s, P, nproj, ne, pnrm, scal2, X);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_lagrange(const LagArgs &a, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.bar, 1);
+   CPSD_CHECK(hipGetLastError());
+   if (a.ne <= 256) {
+      const int nt = (a.ne + 15) / 16;
+      hipLaunchKernelGGL(k_lagrange<16>, dim3(nt * nt), dim3(256), 0, s, a);
+   } else if (a.ne <= 512) {
+      const int nt = (a.ne + 31) / 32;
+      hipLaunchKernelGGL(k_lagrange<32>, dim3(nt * nt), dim3(256), 0, s, a);
+   } else {
+      cpsd_set_error("lagrange: ne=%d above 512 is not supported", a.ne);
+      return -1;
+   }
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_energy(const EnergyArgs &a, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, s, a);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_hml(const double *M, int ne, double *hml, hipStream_t s)
+{
+   hipLaunchKernelGGL(k_hml, g1((size_t)ne * ne), dim3(256), 0, s, M, ne, hml);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}

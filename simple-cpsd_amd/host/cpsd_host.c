@@ -1,0 +1,382 @@
+/*
+ * cpsd_host.c -- host-side inputs of the SD step (see include/cpsd_host.h).
+ *
+ * Reference counterparts (paths relative to Cpsd-mpi-bamboo/):
+ *   lattice            ccpsdv3/get_cube.c:29-85
+ *   G vectors          ccpsdv3/get_g.c:12-35
+ *   cutoff mask        ccpsdv3/get_masker.c:12-37 (strict |G|^2 < |b1 N/2|^2,
+ *                      kx < N/2, |ky|,|kz| < N/2)
+ *   tables, Ewald      ccpsdv3/main.c:161-198, 287-313
+ *   KB transform       ckbppv3/dkbppv3.c:310-444
+ *   plane-wave start   cmakepsi/dmakepsi.c:86-133
+ * The KB transform is the expensive part (nfft3d x nrho sin/cos); it runs
+ * over G vectors in parallel with OpenMP.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../include/cpsd_host.h"
+
+static int wrapk(int k, int n) { return k < 0 ? k + n : k; }
+
+static void lattice(int icube, double unit, double *omega, double *a, double *b)
+{
+   double base[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+   const double twopi = 8.0 * atan(1.0);
+   double vol;
+   int i;
+   if (icube == 2) {
+      const double f[9] = {0.5, 0.5, 0.0, 0.5, 0.0, 0.5, 0.0, 0.5, 0.5};
+      memcpy(base, f, sizeof f);
+   } else if (icube == 3) {
+      const double f[9] = {0.5, 0.5, 0.5, -0.5, 0.5, 0.5, 0.5, 0.5, -0.5};
+      memcpy(base, f, sizeof f);
+   }
+   for (i = 0; i < 9; ++i) a[i] = unit * base[i];
+   /* reciprocal vectors b_d = 2 pi (a_{d+1} x a_{d+2}) / (a_1 . a_2 x a_3) */
+   for (i = 0; i < 3; ++i) {
+      const double *u = a + 3 * ((i + 1) % 3), *v = a + 3 * ((i + 2) % 3);
+      b[3 * i + 0] = u[1] * v[2] - u[2] * v[1];
+      b[3 * i + 1] = u[2] * v[0] - u[0] * v[2];
+      b[3 * i + 2] = u[0] * v[1] - u[1] * v[0];
+   }
+   vol = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+   for (i = 0; i < 9; ++i) b[i] *= twopi / vol;
+   *omega = fabs(vol);
+}
+
+int cpsdh_setup(int icube, double unit, int nfft, int nion, const int *katm, const double *zv,
+                double *omega, double *unita, double *unitg, double *G, int *masker, double *tg,
+                double *vc, double *vg, double *rcell, double *cewald, double *rcut)
+{
+   const double pi = 4.0 * atan(1.0);
+   const int n = nfft, nh = n / 2, n3d = (nh + 1) * n * n;
+   double madelung, gcut, w, rs, zsum = 0.0, z2sum = 0.0, vgsum_all = 0.0, vgsum_x0 = 0.0;
+   int k1, k2, k3, i, l;
+
+   if (icube < 1 || icube > 3 || n < 2 || (n & 1)) return 1;
+   lattice(icube, unit, omega, unita, unitg);
+   madelung = icube == 1 ? 1.76011888 : (icube == 2 ? 1.791753 : 1.791860);
+   *rcut = unit / pi;
+   gcut = 0.0;
+   for (i = 0; i < 3; ++i) gcut += (unitg[i] * nh) * (unitg[i] * nh);
+   w = 0.25 * (*rcut) * (*rcut);
+
+   for (k3 = -nh + 1; k3 <= nh; ++k3)
+      for (k2 = -nh + 1; k2 <= nh; ++k2)
+         for (k1 = 0; k1 <= nh; ++k1) {
+            const int g = k1 + (nh + 1) * (wrapk(k2, n) + n * wrapk(k3, n));
+            const double gx = k1 * unitg[0] + k2 * unitg[3] + k3 * unitg[6];
+            const double gy = k1 * unitg[1] + k2 * unitg[4] + k3 * unitg[7];
+            const double gz = k1 * unitg[2] + k2 * unitg[5] + k3 * unitg[8];
+            const double gg = gx * gx + gy * gy + gz * gz;
+            const int inside = (k1 < nh && k2 < nh && k3 < nh && gg < gcut);
+            G[g] = gx; G[g + n3d] = gy; G[g + 2 * n3d] = gz;
+            masker[2 * g] = masker[2 * g + 1] = inside;
+            if (g == 0) {
+               tg[0] = vc[0] = vg[0] = 0.0;
+            } else {
+               tg[g] = 0.5 * gg * inside;
+               vc[g] = (4.0 * pi / gg) * inside;
+               vg[g] = (4.0 * pi / gg) * exp(-w * gg) * inside;
+            }
+         }
+   /* gssum(vg): 2 * sum(all) - sum(kx = 0 plane), summed in storage order */
+   for (i = 0; i < n3d; ++i) vgsum_all += vg[i];
+   for (i = 0; i < n * n; ++i) vgsum_x0 += vg[i * (nh + 1)];
+   for (i = 0; i < nion; ++i) {
+      zsum += zv[katm[i]];
+      z2sum += zv[katm[i]] * zv[katm[i]];
+   }
+   rs = pow(3.00 * (*omega) / (4.0 * pi), 1.00 / 3.00);
+   *cewald = -0.50 * z2sum * (madelung / rs + (2.0 * vgsum_all - vgsum_x0) / (*omega))
+             - 0.50 * (zsum * zsum - z2sum) * (*rcut) * (*rcut) * pi / (*omega);
+   l = 0;
+   for (k3 = -1; k3 <= 1; ++k3)
+      for (k2 = -1; k2 <= 1; ++k2)
+         for (k1 = -1; k1 <= 1; ++k1, ++l)
+            for (i = 0; i < 3; ++i)
+               rcell[l + 27 * i] = k1 * unita[i] + k2 * unita[3 + i] + k3 * unita[6 + i];
+   return 0;
+}
+
+/* ------------------------------------------------------------- PSP */
+
+static int next_line(FILE *f, char *buf, int len)
+{
+   return fgets(buf, len, f) != NULL;
+}
+
+int cpsdh_read_psp(const char *path, char *atom, double *zv, double *amass, int *lmax, double *rc,
+                   int *nrho, double *drho, double *rho, double *vp, double *wp, int nrho_cap)
+{
+   char line[4096];
+   FILE *f = fopen(path, "r");
+   int l, i, lm;
+   if (!f) return 1;
+   if (!next_line(f, line, sizeof line) || sscanf(line, "%2s", atom) != 1) { fclose(f); return 2; }
+   /* line 2: zv amass lmax [extra integer, ignored] */
+   if (!next_line(f, line, sizeof line) || sscanf(line, "%lf %lf %d", zv, amass, lmax) != 3) {
+      fclose(f);
+      return 3;
+   }
+   lm = *lmax;
+   if (lm < 0 || lm > 3) { fclose(f); return 4; }
+   for (l = 0; l <= lm; ++l)
+      if (fscanf(f, "%lf", &rc[l]) != 1) { fclose(f); return 5; }
+   if (fscanf(f, "%d %lf", nrho, drho) != 2 || *nrho < 2 || *nrho > nrho_cap) { fclose(f); return 6; }
+   for (i = 0; i < *nrho; ++i) {
+      if (fscanf(f, "%lf", &rho[i]) != 1) { fclose(f); return 7; }
+      for (l = 0; l <= lm; ++l)
+         if (fscanf(f, "%lf", &vp[i + l * (*nrho)]) != 1) { fclose(f); return 7; }
+   }
+   for (i = 0; i < *nrho; ++i) {
+      double r;
+      if (fscanf(f, "%lf", &r) != 1) { fclose(f); return 8; }
+      for (l = 0; l <= lm; ++l)
+         if (fscanf(f, "%lf", &wp[i + l * (*nrho)]) != 1) { fclose(f); return 8; }
+   }
+   fclose(f);
+   return 0;
+}
+
+/* composite Simpson weights as ckbppv3/dkbppv3.c:310-328 applies them:
+ * h/3 * (2 sum(even) + 4 sum(odd) - y_0 - y_{n-1}) */
+static double simpson(int n, const double *y, double h)
+{
+   double even = 0.0, odd = 0.0;
+   int i;
+   for (i = 0; i < n; i += 2) even += y[i];
+   for (i = 1; i < n; i += 2) odd += y[i];
+   return (2.0 * even + 4.0 * odd - y[0] - y[n - 1]) * (h / 3.0);
+}
+
+int cpsdh_kbpp(double zv, int lmax, int nrho, double drho, const double *rho, const double *vp_in,
+               const double *wp, int nfft3d, const double *G, double *vl, double *vnl,
+               double *vnlnrm)
+{
+   const double pi = 4.0 * atan(1.0), fourpi = 4.0 * pi;
+   const double p0 = sqrt(fourpi), p1 = sqrt(3.0 * fourpi), p2 = sqrt(15.0 * fourpi);
+   const int lmmax = lmax * lmax;
+   double *dv = (double *)malloc((size_t)(lmax + 1) * nrho * sizeof(double));
+   double *f = (double *)malloc(nrho * sizeof(double));
+   int l, i, k;
+   if (!dv || !f) { free(dv); free(f); return 1; }
+   /* channel l < lmax carries V_l - V_lmax; channel lmax is the local part */
+   for (l = 0; l <= lmax; ++l)
+      for (i = 0; i < nrho; ++i)
+         dv[i + l * nrho] = (l < lmax) ? vp_in[i + l * nrho] - vp_in[i + lmax * nrho] : vp_in[i + l * nrho];
+   for (l = 0; l < lmax; ++l) {
+      double a;
+      for (i = 0; i < nrho; ++i) f[i] = dv[i + l * nrho] * wp[i + l * nrho] * wp[i + l * nrho];
+      a = simpson(nrho, f, drho);
+      for (i = l * l; i < (l + 1) * (l + 1); ++i) vnlnrm[i] = a;
+   }
+   for (i = lmmax; i < 9; ++i) vnlnrm[i] = 0.0;
+
+#pragma omp parallel
+   {
+      double *fl = (double *)malloc(nrho * sizeof(double));
+      double *cs = (double *)malloc(nrho * sizeof(double));
+      double *sn = (double *)malloc(nrho * sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+      for (k = 1; k < nfft3d; ++k) {
+         double gx = G[k], gy = G[k + nfft3d], gz = G[k + 2 * nfft3d];
+         const double q = sqrt(gx * gx + gy * gy + gz * gz);
+         int j;
+         gx /= q; gy /= q; gz /= q;
+         for (j = 0; j < nrho; ++j) { cs[j] = cos(q * rho[j]); sn[j] = sin(q * rho[j]); }
+         if (lmax > 2) {
+            double d;
+            fl[0] = 0.0;
+            for (j = 1; j < nrho; ++j) {
+               const double t = 3.0 * (sn[j] / (q * rho[j]) - cs[j]) / (q * rho[j]) - sn[j];
+               fl[j] = t * wp[j + 2 * nrho] * dv[j + 2 * nrho];
+            }
+            d = p2 * simpson(nrho, fl, drho) / q;
+            vnl[k + 4 * (size_t)nfft3d] = d * (3.0 * gz * gz - 1.0) / (2.0 * sqrt(3.0));
+            vnl[k + 5 * (size_t)nfft3d] = d * gx * gy;
+            vnl[k + 6 * (size_t)nfft3d] = d * gy * gz;
+            vnl[k + 7 * (size_t)nfft3d] = d * gz * gx;
+            vnl[k + 8 * (size_t)nfft3d] = d * (gx * gx - gy * gy) / 2.0;
+         }
+         if (lmax > 1) {
+            double p;
+            fl[0] = 0.0;
+            for (j = 1; j < nrho; ++j)
+               fl[j] = (sn[j] / (q * rho[j]) - cs[j]) * wp[j + nrho] * dv[j + nrho];
+            p = p1 * simpson(nrho, fl, drho) / q;
+            vnl[k + 1 * (size_t)nfft3d] = p * gx;
+            vnl[k + 2 * (size_t)nfft3d] = p * gy;
+            vnl[k + 3 * (size_t)nfft3d] = p * gz;
+         }
+         if (lmax > 0) {
+            for (j = 0; j < nrho; ++j) fl[j] = sn[j] * wp[j] * dv[j];
+            vnl[k] = p0 * simpson(nrho, fl, drho) / q;
+         }
+         for (j = 0; j < nrho; ++j) fl[j] = rho[j] * dv[j + lmax * nrho] * sn[j];
+         vl[k] = simpson(nrho, fl, drho) * fourpi / q - zv * fourpi * cs[nrho - 1] / (q * q);
+      }
+      free(fl); free(cs); free(sn);
+   }
+   for (i = 0; i < nrho; ++i) f[i] = dv[i + lmax * nrho] * (rho[i] * rho[i]);
+   vl[0] = fourpi * simpson(nrho, f, drho) + 2.0 * pi * zv * rho[nrho - 1] * rho[nrho - 1];
+   for (i = 0; i < nrho; ++i) f[i] = rho[i] * wp[i] * dv[i];
+   vnl[0] = p0 * simpson(nrho, f, drho);
+   for (l = 1; l < lmmax; ++l) vnl[(size_t)l * nfft3d] = 0.0;
+   free(dv);
+   free(f);
+   return 0;
+}
+
+int cpsdh_makepsi(int nfft, int ne, const int *spec, double *psi)
+{
+   const int nh1 = nfft / 2 + 1;
+   const size_t n2 = (size_t)(nfft + 2) * nfft * nfft;
+   const double h = 1.0 / sqrt(2.0);
+   int n;
+   memset(psi, 0, (size_t)ne * n2 * sizeof(double));
+   for (n = 0; n < ne; ++n) {
+      const int i = spec[4 * n], j = (spec[4 * n + 1] + nfft) % nfft, k = (spec[4 * n + 2] + nfft) % nfft;
+      const int l = spec[4 * n + 3];
+      const double re = l < 0 ? 0.0 : (l == 0 ? 1.0 : h), im = l < 0 ? h : 0.0;
+      double *p = psi + (size_t)n * n2;
+      size_t g = (size_t)i + (size_t)nh1 * (j + (size_t)nfft * k);
+      p[2 * g] = re; p[2 * g + 1] = im;
+      if (i == 0) {
+         g = (size_t)nh1 * ((nfft - j) % nfft + (size_t)nfft * ((nfft - k) % nfft));
+         p[2 * g] = re; p[2 * g + 1] = -im;
+      }
+   }
+   return 0;
+}
+
+double cpsdh_gcdotc(int nfft, const double *a, const double *b)
+{
+   const size_t n2 = (size_t)(nfft + 2) * nfft * nfft;
+   double all = 0.0, x0 = 0.0;
+   size_t i;
+   for (i = 0; i < n2; ++i) all += a[i] * b[i];
+   for (i = 0; i < (size_t)nfft * nfft; ++i) {
+      const size_t o = i * (nfft + 2);
+      x0 += a[o] * b[o] + a[o + 1] * b[o + 1];
+   }
+   return 2.0 * all - x0;
+}
+
+/* ------------------------------------------------ random start */
+
+static uint64_t splitmix64(uint64_t *s)
+{
+   uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+   return z ^ (z >> 31);
+}
+
+static double uniform01(uint64_t *s)
+{
+   return ((splitmix64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+int cpsdh_random_orbitals(int nfft, int ne, const int *masker, const double *G, unsigned long long seed,
+                          double decay, double *psi)
+{
+   const int nh1 = nfft / 2 + 1;
+   const size_t n3d = (size_t)nh1 * nfft * nfft, n2 = 2 * n3d;
+   size_t ng = 0, g, p;
+   size_t *gl;
+   double *wt, *cp;
+   uint64_t st = (uint64_t)seed;
+   int i, j;
+   for (g = 0; g < n3d; ++g) ng += masker[2 * g] != 0;
+   gl = (size_t *)malloc(ng * sizeof(size_t));
+   wt = (double *)malloc(ng * sizeof(double));
+   cp = (double *)malloc((size_t)ne * 2 * ng * sizeof(double));
+   if (!gl || !wt || !cp) { free(gl); free(wt); free(cp); return 1; }
+   p = 0;
+   for (g = 0; g < n3d; ++g)
+      if (masker[2 * g]) {
+         gl[p] = g;
+         wt[p] = (g % nh1 == 0) ? 1.0 : 2.0;
+         ++p;
+      }
+   /* Gaussian coefficients, one Box-Muller pair per (orbital, G) */
+   for (i = 0; i < ne; ++i)
+      for (p = 0; p < ng; ++p) {
+         const double u1 = uniform01(&st), u2 = uniform01(&st);
+         const double r = sqrt(-2.0 * log(u1));
+         double damp = 1.0;
+         if (decay > 0.0) {
+            const size_t gg = gl[p];
+            const double gx = G[gg], gy = G[gg + n3d], gz = G[gg + 2 * n3d];
+            damp = exp(-0.5 * decay * (gx * gx + gy * gy + gz * gz));
+         }
+         cp[(size_t)i * 2 * ng + 2 * p] = damp * r * cos(2.0 * M_PI * u2);
+         cp[(size_t)i * 2 * ng + 2 * p + 1] = damp * r * sin(2.0 * M_PI * u2);
+      }
+   /* Hermitian kx = 0 plane: c(-ky,-kz) = conj c(ky,kz); Im c(0) = 0 */
+   {
+      size_t *pos = (size_t *)malloc(n3d * sizeof(size_t));
+      for (g = 0; g < n3d; ++g) pos[g] = (size_t)-1;
+      for (p = 0; p < ng; ++p) pos[gl[p]] = p;
+      for (p = 0; p < ng; ++p) {
+         const size_t gg = gl[p];
+         if (gg % nh1) continue;
+         {
+            const int ky = (int)((gg / nh1) % nfft), kz = (int)(gg / (nh1 * (size_t)nfft));
+            const size_t mate = (size_t)nh1 * ((nfft - ky) % nfft + (size_t)nfft * ((nfft - kz) % nfft));
+            const size_t q = pos[mate];
+            if (q == (size_t)-1) continue;
+            for (i = 0; i < ne; ++i) {
+               double *c = cp + (size_t)i * 2 * ng;
+               if (q == p) c[2 * p + 1] = 0.0;
+               else if (q > p) { c[2 * q] = c[2 * p]; c[2 * q + 1] = -c[2 * p + 1]; }
+            }
+         }
+      }
+      free(pos);
+   }
+   /* modified Gram-Schmidt under sum_p w_p (re re' + im im'); the dot
+    * products are summed over 64 fixed chunks, then the chunks in order, so
+    * the result does not depend on the thread count */
+   {
+      enum { NCH = 64 };
+      double part[NCH];
+      const size_t chunk = (ng + NCH - 1) / NCH;
+      for (i = 0; i < ne; ++i) {
+         double *ci = cp + (size_t)i * 2 * ng;
+         double nrm = 0.0;
+         int b;
+         for (j = 0; j <= i; ++j) {
+            const double *cj = cp + (size_t)j * 2 * ng;
+            double d = 0.0;
+            long q;
+#pragma omp parallel for
+            for (b = 0; b < NCH; ++b) {
+               size_t q0 = (size_t)b * chunk, q1 = q0 + chunk, r;
+               double acc = 0.0;
+               if (q1 > ng) q1 = ng;
+               for (r = q0; r < q1; ++r) acc += wt[r] * (cj[2 * r] * ci[2 * r] + cj[2 * r + 1] * ci[2 * r + 1]);
+               part[b] = acc;
+            }
+            for (b = 0; b < NCH; ++b) d += part[b];
+            if (j == i) { nrm = d; break; }
+#pragma omp parallel for
+            for (q = 0; q < (long)ng; ++q) { ci[2 * q] -= d * cj[2 * q]; ci[2 * q + 1] -= d * cj[2 * q + 1]; }
+         }
+         nrm = 1.0 / sqrt(nrm);
+         for (p = 0; p < 2 * ng; ++p) ci[p] *= nrm;
+      }
+   }
+   memset(psi, 0, (size_t)ne * n2 * sizeof(double));
+   for (i = 0; i < ne; ++i)
+      for (p = 0; p < ng; ++p) {
+         psi[(size_t)i * n2 + 2 * gl[p]] = cp[(size_t)i * 2 * ng + 2 * p];
+         psi[(size_t)i * n2 + 2 * gl[p] + 1] = cp[(size_t)i * 2 * ng + 2 * p + 1];
+      }
+   free(gl); free(wt); free(cp);
+   return 0;
+}

@@ -1,0 +1,213 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: a serial FP64 restatement of the reference SD step
+(ccpsdv3/cpsd.c) and its inputs.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+DATA = os.path.join(ROOT, "simple-cpsd_amd", "data")
+
+_lib = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        L.oracle_cfft1d.argtypes = [C.c_int, C.c_int, dp]
+        L.oracle_cr_fft3b.argtypes = [C.c_int, dp]
+        L.oracle_rc_fft3f.argtypes = [C.c_int, dp]
+        for f in ("oracle_gcdotc", "oracle_gsdot"):
+            getattr(L, f).argtypes = [C.c_int, dp, dp]
+            getattr(L, f).restype = C.c_double
+        L.oracle_gssum.argtypes = [C.c_int, dp]
+        L.oracle_gssum.restype = C.c_double
+        L.oracle_phafac.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, dp]
+        L.oracle_strfac.argtypes = [C.c_int, dp, dp, dp, dp]
+        L.oracle_dirac_exchange.argtypes = [C.c_int, dp, dp, dp]
+        L.oracle_vosko.argtypes = [C.c_int, dp, dp, dp]
+        L.oracle_erfc_as.argtypes = [C.c_double]
+        L.oracle_erfc_as.restype = C.c_double
+        L.oracle_get_cube.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), dp, dp]
+        L.oracle_get_g.argtypes = [C.c_int, dp, dp]
+        L.oracle_get_masker.argtypes = [C.c_int, C.c_double, dp, ip]
+        L.oracle_setup.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, ip, dp,
+                                   C.POINTER(C.c_double), dp, dp, dp, ip, dp, dp, dp, dp,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.oracle_simp.argtypes = [C.c_int, dp, C.c_double]
+        L.oracle_simp.restype = C.c_double
+        L.oracle_kbpp.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, dp, dp, dp,
+                                  C.c_int, dp, dp, dp, dp]
+        L.oracle_makepsi.argtypes = [C.c_int, C.c_int, ip, dp]
+        L.oracle_cpsd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int,
+                                  C.c_int, dp, C.c_double, C.c_double, C.c_double, ip, dp, dp,
+                                  C.c_int, dp, C.c_double, C.c_double, ip, dp, dp, dp, dp, ip,
+                                  dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+        L.oracle_cpsd.restype = C.c_int
+        L.oracle_last_lagrange_iterations.restype = C.c_int
+        L.oracle_syev_sorted.argtypes = [C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- helpers
+
+def cr_fft3b(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    lib().oracle_cr_fft3b(n, a)
+    return a
+
+
+def rc_fft3f(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    lib().oracle_rc_fft3f(n, a)
+    return a
+
+
+def gcdotc(n, a, b):
+    return lib().oracle_gcdotc(n, np.ascontiguousarray(a), np.ascontiguousarray(b))
+
+
+class Setup:
+    """Setup tables of ccpsdv3/main.c:145-320 for one cell."""
+
+    def __init__(self, icube, unit, nfft, katm, zv):
+        self.icube, self.unit, self.nfft = icube, float(unit), nfft
+        self.nfft3d = (nfft // 2 + 1) * nfft * nfft
+        self.n2ft3d = (nfft + 2) * nfft * nfft
+        self.katm = np.ascontiguousarray(katm, dtype=np.int32)
+        self.zv = np.ascontiguousarray(zv, dtype=np.float64)
+        self.nion = len(self.katm)
+        om, cew, rc = C.c_double(), C.c_double(), C.c_double()
+        self.unita = np.zeros(9)
+        self.unitg = np.zeros(9)
+        self.G = np.zeros(3 * self.nfft3d)
+        self.masker = np.zeros(self.n2ft3d, dtype=np.int32)
+        self.tg = np.zeros(self.nfft3d)
+        self.vc = np.zeros(self.nfft3d)
+        self.vg = np.zeros(self.nfft3d)
+        self.rcell = np.zeros(81)
+        lib().oracle_setup(icube, self.unit, nfft, self.nion, self.katm, self.zv,
+                           C.byref(om), self.unita, self.unitg, self.G, self.masker,
+                           self.tg, self.vc, self.vg, self.rcell, C.byref(cew), C.byref(rc))
+        self.omega, self.cewald, self.rcut = om.value, cew.value, rc.value
+
+
+def read_psp(path):
+    """Radial table of a PSP/*.dat file (format read by ckbppv3/dkbppv3.c:89-114).
+
+    Line 2 holds zv, amass, lmax and, in most files of the reference's PSP/
+    directory, a fourth integer that the reference reader misparses; it is
+    read here as an optional trailing token and ignored."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    atom = lines[0].split()[0]
+    t = lines[1].split()
+    zv, amass, lmax = float(t[0]), float(t[1]), int(t[2])
+    rest = " ".join(lines[2:]).split()
+    rc = [float(x) for x in rest[: lmax + 1]]
+    pos = lmax + 1
+    nrho = int(rest[pos]); drho = float(rest[pos + 1]); pos += 2
+    vals = np.array([float(x) for x in rest[pos: pos + 2 * nrho * (lmax + 2)]])
+    a = vals[: nrho * (lmax + 2)].reshape(nrho, lmax + 2)
+    b = vals[nrho * (lmax + 2):].reshape(nrho, lmax + 2)
+    rho = a[:, 0].copy()
+    vp = np.ascontiguousarray(a[:, 1:].T.reshape(-1))  # [l][i] column-major as reference
+    wp = np.ascontiguousarray(b[:, 1:].T.reshape(-1))
+    return dict(atom=atom, zv=zv, amass=amass, lmax=lmax, rc=rc, nrho=nrho,
+                drho=drho, rho=rho, vp=vp, wp=wp)
+
+
+def kbpp(psp, lmax, setup):
+    """vl, vnl (9 slots), vnlnrm (9) for one kind on setup's grid."""
+    lmax = min(lmax, psp["lmax"])
+    n3 = setup.nfft3d
+    vp = psp["vp"][: (psp["lmax"] + 1) * psp["nrho"]].copy()
+    # the reference reduces lmax before kbpp but keeps the original column
+    # layout; the local channel is column lmax of the reduced set
+    vl = np.zeros(n3)
+    vnl = np.zeros(9 * n3)
+    vnlnrm = np.zeros(9)
+    lib().oracle_kbpp(psp["zv"], lmax, psp["nrho"], psp["drho"], psp["rho"], vp,
+                      np.ascontiguousarray(psp["wp"]), n3, setup.G, vl, vnl, vnlnrm)
+    return vl, vnl, vnlnrm
+
+
+def read_psi0(path, ne):
+    spec = []
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 4:
+                spec.append([int(x) for x in t[:4]])
+            if len(spec) == ne:
+                break
+    return np.ascontiguousarray(np.array(spec, dtype=np.int32).reshape(-1))
+
+
+def makepsi(nfft, ne, spec):
+    psi = np.zeros(ne * (nfft + 2) * nfft * nfft)
+    lib().oracle_makepsi(nfft, ne, np.ascontiguousarray(spec, dtype=np.int32), psi)
+    return psi
+
+
+class State:
+    """Arrays main() owns (ccpsdv3/main.c:126-136, 237-262, 317-320)."""
+
+    def __init__(self, setup, ne, lmmax, vl, vnl, vnlnrm, r, c, amass):
+        s = setup
+        self.s = s
+        self.ne = np.array([ne, 0], dtype=np.int32)
+        nn = ne * s.n2ft3d
+        self.lmmax = np.ascontiguousarray(lmmax, dtype=np.int32)
+        self.vl = np.ascontiguousarray(vl)
+        self.vnl = np.ascontiguousarray(vnl)
+        self.vnlnrm = np.ascontiguousarray(vnlnrm)
+        self.amass = np.ascontiguousarray(amass, dtype=np.float64)
+        self.r2 = np.ascontiguousarray(r, dtype=np.float64).copy()
+        self.r1 = self.r2.copy()
+        self.fion = np.zeros(3 * s.nion)
+        self.c2 = np.ascontiguousarray(c, dtype=np.float64).copy()
+        self.c1 = self.c2.copy()
+        self.cpsi = np.zeros(nn)
+        self.dn = np.zeros(s.n2ft3d)
+        self.hml = np.zeros(ne * ne)
+        self.lmd = np.zeros(ne * ne)
+        nfft = s.nfft
+        self.work = np.zeros(7 * s.n2ft3d + s.nion * (5 * nfft + 3) + 7 * ne * ne)
+        self.E = np.zeros(10)
+        self.deltae = C.c_double(0.0)
+        self.deltac = C.c_double(0.0)
+        self.deltar = C.c_double(0.0)
+
+    def step(self, inner, dt=5.8, fmass=400000.0):
+        s = self.s
+        rc = lib().oracle_cpsd(inner, 0, s.nion, 1, self.ne, s.nfft, s.nfft3d, s.n2ft3d,
+                               s.unitg, s.omega, dt, fmass, s.katm, self.amass, s.zv, 27,
+                               s.rcell, s.cewald, s.rcut, s.masker, s.G, s.tg, s.vc, s.vg,
+                               self.lmmax, self.vnlnrm, self.vl, self.vnl, self.r1, self.r2,
+                               self.fion, self.c1, self.c2, self.cpsi, self.dn, self.hml,
+                               self.lmd, self.work, self.E, C.byref(self.deltae),
+                               C.byref(self.deltac), C.byref(self.deltar))
+        if rc != 0:
+            raise RuntimeError("oracle_cpsd failed: %d" % rc)
+        return self.E.copy(), self.deltae.value, self.deltac.value

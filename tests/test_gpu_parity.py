@@ -1,0 +1,215 @@
+"""GPU parity of the HIP SD step against the oracle, through the C-ABI.
+
+Bars: FP64 everywhere; energies and their components within 1e-9 Ha of the
+oracle after a fixed number of steps (the north-star bar is 1e-8), densities
+within 1e-10 max-abs, FFTs within 1e-12 relative to the transform's scale.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cpsd_amd
+import oracle_lib as O
+from example_inputs import example0_expected, example0_oracle_state, water_oracle_state
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128])
+def test_fft_cr_rc_vs_oracle(gpu, n):
+    say("fft parity N=%d" % n)
+    rng = np.random.default_rng(n)
+    nb = 2 if n <= 64 else 1
+    a = rng.standard_normal((nb, (n + 2) * n * n))   # general input incl. kx = N/2
+    got = cpsd_amd.fft3d(+1, n, a)
+    for b in range(nb):
+        ref = O.cr_fft3b(a[b], n)
+        scale = np.sqrt(n ** 3) * np.max(np.abs(a[b]))
+        assert np.max(np.abs(got[b] - ref)) < 1e-13 * scale * np.log2(n) * 4
+        assert np.all(got[b].reshape(n, n, n + 2)[:, :, n:] == 0.0)
+    r = rng.standard_normal((nb, (n + 2) * n * n))
+    got = cpsd_amd.fft3d(-1, n, r)
+    for b in range(nb):
+        ref = O.rc_fft3f(r[b], n)
+        scale = np.sqrt(n ** 3) * np.max(np.abs(r[b]))
+        assert np.max(np.abs(got[b] - ref)) < 1e-13 * scale * np.log2(n) * 4
+        g = got[b].reshape(n, n, n + 2)
+        # Im(kx = 0) and Im(kx = N/2) are zeroed after the x pass, then the
+        # y/z passes mix them; compare layout, not just values
+        assert g.shape == (n, n, n + 2)
+
+
+def test_fortran_abi_entry_points(gpu):
+    n = 32
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((n + 2) * n * n)
+    got = cpsd_amd.fortran_fft(+1, n, a)
+    assert np.max(np.abs(got - O.cr_fft3b(a, n))) < 1e-10
+    got = cpsd_amd.fortran_fft(-1, n, a)
+    assert np.max(np.abs(got - O.rc_fft3f(a, n))) < 1e-10
+
+
+# ------------------------------------------------------------ pointwise
+
+def test_lda_vs_oracle(gpu):
+    rng = np.random.default_rng(1)
+    dn = np.concatenate([np.zeros(7), 10.0 ** rng.uniform(-12, 1.5, 4000)])
+    xcp, xce = cpsd_amd.lda_device(dn)
+    rp = np.zeros_like(dn); re = np.zeros_like(dn)
+    O.lib().oracle_dirac_exchange(dn.size, dn, rp, re)
+    O.lib().oracle_vosko(dn.size, dn, rp, re)
+    assert np.allclose(xcp, rp, rtol=1e-13, atol=1e-15)
+    assert np.allclose(xce, re, rtol=1e-13, atol=1e-15)
+
+
+def test_erfc_vs_oracle(gpu):
+    w = np.linspace(0.0, 8.0, 1001)
+    got = cpsd_amd.erfc_device(w)
+    ref = np.array([O.lib().oracle_erfc_as(x) for x in w])
+    assert np.allclose(got, ref, rtol=1e-14, atol=0)
+
+
+# ------------------------------------------------------------ SD step
+
+def cell_from_state(st):
+    """a cpsd_amd.Cell carrying exactly the oracle state's inputs"""
+    s = st.s
+    c = cpsd_amd.Cell.__new__(cpsd_amd.Cell)
+    c.icube, c.unit, c.nfft = s.icube, s.unit, s.nfft
+    c.nfft3d, c.n2ft3d = s.nfft3d, s.n2ft3d
+    c.katm, c.nion, c.r = s.katm, s.nion, st.r2.copy()
+    c.zv, c.amass, c.lmmax = s.zv, st.amass, st.lmmax
+    c.unita, c.unitg, c.G, c.masker = s.unita, s.unitg, s.G, s.masker
+    c.tg, c.vc, c.vg, c.rcell = s.tg, s.vc, s.vg, s.rcell
+    c.omega, c.cewald, c.rcut = s.omega, s.cewald, s.rcut
+    c.vl, c.vnl, c.vnlnrm = st.vl, st.vnl, st.vnlnrm
+    c.nkind = len(st.lmmax)
+    return c
+
+
+def compare(E_gpu, E_ref, tol=1e-9):
+    d = np.abs(np.asarray(E_gpu) - np.asarray(E_ref))
+    assert np.max(d) < tol, (d, E_gpu, E_ref)
+
+
+def test_dropin_cpsd_example0(gpu):
+    """The drop-in cpsd() symbol against the oracle on the reference's
+    Example0 inputs: 3 calls of inner = 10 steps."""
+    say("dropin example0")
+    ctl, st = example0_oracle_state()
+    cell = cell_from_state(st)
+    c2 = st.c2.copy()
+    E = np.zeros(10)
+    for call in range(3):
+        out = cpsd_amd.cpsd_dropin(cell, 4, ctl["inner"], c2, E, ctl["dt"], ctl["fmass"])
+        Er, der, dcr = st.step(ctl["inner"], ctl["dt"], ctl["fmass"])
+        compare(out["E"], Er)
+        assert abs(out["deltae"] - der) < 1e-9
+        assert abs(out["deltac"] - dcr) < 1e-9 * max(1.0, abs(dcr))
+        assert np.max(np.abs(out["dn"] - st.dn)) < 1e-10
+        assert np.max(np.abs(out["c2"] - st.c2)) < 1e-10
+        assert np.max(np.abs(out["c1"] - st.c1)) < 1e-10
+        assert np.max(np.abs(out["cpsi"] - st.cpsi)) < 1e-10
+        assert np.max(np.abs(out["hml"] - st.hml)) < 1e-9
+        assert np.max(np.abs(out["lmd"] - st.lmd)) < 1e-9
+        assert np.array_equal(out["r1"], st.r1)
+        c2, E = out["c2"], out["E"]
+
+
+def test_device_water_vs_oracle(gpu):
+    say("device water 32^3")
+    st = water_oracle_state()
+    cell = cell_from_state(st)
+    dev = cpsd_amd.DeviceSD(cell, 4)
+    dev.set_orbitals(st.c2)
+    for _ in range(4):
+        Eg, deg, dcg = dev.run(5)
+        Er, der, dcr = st.step(5)
+        compare(Eg, Er)
+    o = dev.get(c1=True, c2=True, cpsi=True, dn=True, hml=True, lmd=True)
+    assert np.max(np.abs(o["dn"] - st.dn)) < 1e-10
+    assert np.max(np.abs(o["c2"] - st.c2)) < 1e-10
+    assert np.max(np.abs(o["hml"] - st.hml)) < 1e-9
+    dev.close()
+
+
+@pytest.mark.parametrize("nfft,L,nmol,ne", [(24, 12.0, 2, 12), (48, 14.0, 4, 16)])
+def test_device_random_start_vs_oracle(gpu, nfft, L, nmol, ne):
+    """multi-water cells from the product's own input builder, Gaussian start"""
+    say("random start N=%d ne=%d" % (nfft, ne))
+    cell = cpsd_amd.water_cell(nmol, L, nfft, seed=7)
+    psi = cell.random_orbitals(ne, seed=3, decay=0.05)
+    st = O.State(O.Setup(1, L, nfft, cell.katm, cell.zv), ne, cell.lmmax, cell.vl, cell.vnl,
+                 cell.vnlnrm, cell.r, psi, cell.amass)
+    dev = cpsd_amd.DeviceSD(cell, ne)
+    dev.set_orbitals(psi)
+    for _ in range(2):
+        Eg, _, _ = dev.run(3)
+        Er, _, _ = st.step(3)
+        compare(Eg, Er, tol=1e-9)
+    o = dev.get(dn=True, c2=True)
+    assert np.max(np.abs(o["dn"] - st.dn)) < 1e-10
+    assert np.max(np.abs(o["c2"] - st.c2)) < 1e-10
+    dev.close()
+
+
+def test_example0_golden_1000_steps_on_gpu(gpu):
+    """The reference's own Example0 run: E after 1000 steps."""
+    say("example0 golden on gpu")
+    exp = example0_expected()
+    ctl, st = example0_oracle_state()
+    dev = cpsd_amd.DeviceSD(cell_from_state(st), 4)
+    dev.set_orbitals(st.c2)
+    for _ in range(ctl["outer"]):
+        E, de, dc = dev.run(ctl["inner"], ctl["dt"], ctl["fmass"])
+    assert abs(E[0] - exp["E0"]) < 5e-9
+    assert de == pytest.approx(exp["deltae"], rel=5e-3)
+    dev.close()
+
+
+def test_invariants_large_cell(gpu):
+    """size-independent properties on a 64^3 cell with 32 orbitals"""
+    say("invariants 64^3")
+    cell = cpsd_amd.water_cell(8, 14.8, 64, seed=2)
+    ne = 32
+    dev = cpsd_amd.DeviceSD(cell, ne)
+    dev.set_orbitals(cell.random_orbitals(ne, seed=5, decay=0.05))
+    E1, _, _ = dev.run(5)
+    E2, _, _ = dev.run(5)
+    assert E2[0] < E1[0]
+    o = dev.get(c2=True, dn=True, hml=True)
+    n2 = cell.n2ft3d
+    c2 = o["c2"].reshape(ne, n2)
+    w = np.full(n2, 2.0)
+    w.reshape(64 * 64, 66)[:, 0:2] = 1.0
+    S = (c2 * w) @ c2.T
+    assert np.max(np.abs(S - np.eye(ne))) < 1e-11
+    H = o["hml"].reshape(ne, ne)
+    assert np.array_equal(H, H.T)
+    dv = cell.omega / 64 ** 3
+    assert 2 * o["dn"].sum() * dv == pytest.approx(2 * ne, rel=1e-10)
+    assert np.all(o["dn"].reshape(64, 64, 66)[:, :, 64:] == 0)
+    assert np.all(o["c2"][np.tile(cell.masker, ne) == 0] == 0)
+    it, ad = dev.lagrange_info()
+    assert 1 <= it < 30
+    dev.close()
+
+
+def test_unmasked_input_is_rejected(gpu):
+    st = water_oracle_state()
+    cell = cell_from_state(st)
+    dev = cpsd_amd.DeviceSD(cell, 4)
+    bad = st.c2.copy()
+    bad[np.where(np.tile(cell.masker, 4) == 0)[0][5]] = 1.0
+    with pytest.raises(cpsd_amd.CpsdError):
+        dev.set_orbitals(bad)
+    dev.close()

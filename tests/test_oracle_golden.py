@@ -1,0 +1,55 @@
+"""End-to-end pin of the oracle against the reference's own run of its
+Example0 (C2 dimer, 32^3, 4 orbitals, 1000 SD steps), plus invariants of
+the SD step that the reference prints or implies."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from example_inputs import example0_expected, example0_oracle_state, water_oracle_state
+
+
+def test_example0_energy_after_1000_steps():
+    exp = example0_expected()
+    ctl, st = example0_oracle_state()
+    assert (ctl["inner"], ctl["outer"]) == (10, 100)
+    E = de = None
+    for _ in range(ctl["outer"]):
+        E, de, dc = st.step(ctl["inner"], ctl["dt"], ctl["fmass"])
+    # the reference printed 10 significant digits; the oracle must agree to
+    # well inside the 1e-8 Ha parity bar
+    assert abs(E[0] - exp["E0"]) < 5e-9
+    assert de == pytest.approx(exp["deltae"], rel=5e-3)
+
+
+def test_sd_invariants_water():
+    st = water_oracle_state()
+    s = st.s
+    n = s.nfft
+    E, de, dc = st.step(5)
+    # orthonormality after the constraint step (main.c:457-464 prints it)
+    ne = 4
+    n2 = s.n2ft3d
+    S = np.array([[O.gcdotc(n, st.c2[i * n2:(i + 1) * n2], st.c2[j * n2:(j + 1) * n2])
+                   for j in range(ne)] for i in range(ne)])
+    assert np.max(np.abs(S - np.eye(ne))) < 1e-12
+    # hml symmetric, E[8] = 2 E[2]
+    H = st.hml.reshape(ne, ne)
+    assert np.array_equal(H, H.T)
+    assert E[8] == 2 * E[2]
+    # integrated density = 2 ne electrons (ispin = 1)
+    dv = s.omega / n ** 3
+    assert st.dn.sum() * dv * 2 == pytest.approx(2 * ne, rel=1e-12)
+    # padding columns of dn are zero, masked-out coefficients stay zero
+    d = st.dn.reshape(n, n, n + 2)
+    assert np.all(d[:, :, n:] == 0)
+    m = np.tile(s.masker, ne)
+    assert np.all(st.c2[m == 0] == 0) and np.all(st.c1[m == 0] == 0)
+    # energy decreases over SD steps from the plane-wave start
+    E2, _, _ = st.step(5)
+    assert E2[0] < E[0]
+
+
+def test_lagrange_converges_and_lmd_symmetric_ish():
+    st = water_oracle_state()
+    st.step(3)
+    assert O.lib().oracle_last_lagrange_iterations() < 30
