@@ -92,6 +92,11 @@ int cpsd_dev_profile(cpsd_dev *h, int enable);
 int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, double *ms, int *calls,
                           int max_phases);
 void cpsd_dev_destroy(cpsd_dev *h);
+/* copies internal buffer `what` (0 dn, 1 V, 2 dng, 3 CP, 4 P, 5 Gram,
+ * 6-9 s11 s12 s21 s22, 10 lmd, 11 both orbital blocks, 12 xcp, 13 vall,
+ * 14 KB coefficients, 15 projectors, 16 Lagrange info) into out[cap];
+ * returns the number of doubles or -1 (test hook) */
+long cpsd_dev_debug_fetch(cpsd_dev *h, int what, double *out, long cap);
 
 /* ---- kernel-level test hooks (no reference counterpart) -------------
  * Batched versions of the FFT entry points and the LDA kernel, used by

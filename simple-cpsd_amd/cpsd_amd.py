@@ -33,7 +33,7 @@ SUPPORTED_GRIDS = (8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128)
 ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_abort_on_error",
                "cpsd_dev_create", "cpsd_dev_set_orbitals", "cpsd_dev_run", "cpsd_dev_get",
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
-               "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_fft3d_batch",
+               "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc")
@@ -70,6 +70,8 @@ def hip_lib():
         L.cpsd_dev_profile.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _ip, C.c_int]
         L.cpsd_dev_destroy.argtypes = [C.c_void_p]
+        L.cpsd_dev_debug_fetch.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long]
+        L.cpsd_dev_debug_fetch.restype = C.c_long
         L.cpsd_fft3d_batch.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         L.cpsd_lda_device.argtypes = [C.c_int, _dp, _dp, _dp]
         L.cpsd_erfc_device.argtypes = [C.c_int, _dp, _dp]
@@ -322,6 +324,14 @@ class DeviceSD:
         k = hip_lib().cpsd_dev_profile_read(self.h, names, 1024, ms, calls, 32)
         nm = names.value.decode().split(";")
         return {nm[i]: (float(ms[i]), int(calls[i])) for i in range(k)}
+
+    def fetch(self, what):
+        cap = 1 << 28
+        buf = np.zeros(min(cap, 4 * self.ne * self.cell.n2ft3d + 64))
+        n = hip_lib().cpsd_dev_debug_fetch(self.h, what, buf, buf.size)
+        if n < 0:
+            raise CpsdError(_err())
+        return buf[:n].copy()
 
     def close(self):
         if self.h:

@@ -309,6 +309,8 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    TRY(dalloc(&c->dn, c->n3)); TRY(dalloc(&c->xcp, c->n3)); TRY(dalloc(&c->xce, c->n3));
    TRY(dalloc(&c->V, c->n3));
    TRY(dalloc(&c->dng, ngp)); TRY(dalloc(&c->s, ngp)); TRY(dalloc(&c->vall, ngp));
+   /* the packed tail p >= ng is never written by the kernels: keep it zero */
+   CPSD_CHECK(hipMemsetAsync(c->dng, 0, ngp * sizeof(double2), st));
    TRY(dalloc(&c->B, (size_t)(nproj > 0 ? nproj : 1) * ngp));
    {
       const size_t K = 2 * (size_t)ngp;
@@ -668,6 +670,41 @@ extern "C" int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, do
       names[names_len - 1] = 0;
    }
    return k;
+}
+
+/* copy an internal device buffer to the host (test hook): returns the number
+ * of doubles written, or -1 */
+extern "C" long cpsd_dev_debug_fetch(cpsd_dev *h, int what, double *out, long cap)
+{
+   const size_t orb = (size_t)h->ne * h->ngp;
+   const void *src = nullptr;
+   size_t nd = 0;
+   switch (what) {
+   case 0: src = h->dn; nd = h->n3; break;
+   case 1: src = h->V; nd = h->n3; break;
+   case 2: src = h->dng; nd = 2 * (size_t)h->ngp; break;
+   case 3: src = h->CP; nd = 2 * orb; break;
+   case 4: src = h->P; nd = (size_t)h->nproj * h->ne; break;
+   case 5: src = h->Gm; nd = 4 * (size_t)h->ne * h->ne; break;
+   case 6: src = h->s11; nd = (size_t)h->ne * h->ne; break;
+   case 7: src = h->s12; nd = (size_t)h->ne * h->ne; break;
+   case 8: src = h->s21; nd = (size_t)h->ne * h->ne; break;
+   case 9: src = h->s22; nd = (size_t)h->ne * h->ne; break;
+   case 10: src = h->lmd; nd = (size_t)h->ne * h->ne; break;
+   case 11: src = h->Cbase; nd = 4 * orb; break;
+   case 12: src = h->xcp; nd = h->n3; break;
+   case 13: src = h->vall; nd = 2 * (size_t)h->ngp; break;
+   case 14: src = h->Xkb; nd = (size_t)h->nproj * h->ne; break;
+   case 15: src = h->B; nd = 2 * (size_t)h->nproj * h->ngp; break;
+   case 16: src = h->info; nd = 2; break;
+   default: cpsd_set_error("debug_fetch: unknown buffer %d", what); return -1;
+   }
+   if ((long)nd > cap) { cpsd_set_error("debug_fetch: capacity %ld < %zu", cap, nd); return -1; }
+   if (hipMemcpy(out, src, nd * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+      cpsd_set_error("debug_fetch: copy failed");
+      return -1;
+   }
+   return (long)nd;
 }
 
 extern "C" void cpsd_dev_destroy(cpsd_dev *h)

@@ -17,7 +17,7 @@
  *                      (projectors or orbitals), SD epilogues fused.
  * MFMA operand layout (wave64): lane l supplies A[l%16][l/16] and
  * B[l/16][l%16] of the 16x4 and 4x16 blocks; the 16x16 result comes back as
- * four doubles per lane, d[r] = C[4*(l/16) + r][l%16].
+ * four doubles per lane, d[r] = C[l/16 + 4*r][l%16] (measured on gfx950).
  */
 #include "cpsd_internal.h"
 
@@ -110,7 +110,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
          for (int r = 0; r < 4; ++r) {
-            const int m = tm * TT + 32 * wm + 16 * bi + 4 * (lane >> 4) + r;
+            const int m = tm * TT + 32 * wm + 16 * bi + (lane >> 4) + 4 * r;
             const int n = tn * TT + 32 * wn + 16 * bj + (lane & 15);
             if (m < M && n < N) partial[((size_t)kz * M + m) * N + n] = acc[bi][bj][r];
          }
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) k_gemm_tall(TallArgs a)
    for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-         const int n = n0 + 16 * j + 4 * (lane >> 4) + r;
+         const int n = n0 + 16 * j + (lane >> 4) + 4 * r;
          if (n < a.Nc) {
             const size_t idx = (size_t)n * a.ldc + k;
             if (EPI == EPI_KBAPPLY) {
