@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""SD-step benchmark on one MI355X.
+
+Workload (default "h2o-32"): 32 water molecules (96 atoms) in an 18.6 bohr
+simple-cubic cell, 96^3 FFT grid, 128 doubly occupied orbitals, O s+p / H s
+Kleinman-Bylander projectors (192), LDA, dt = 5.8, fmass = 400000, energy
+evaluated every inner = 10 steps, FP64 throughout.  Positions: seeded 4x4x2
+lattice with +-0.3 bohr jitter and random orientations; orbitals: seeded
+complex Gaussian coefficients on the masked G vectors, orthonormalised.
+
+A "step" is one electronic SD step (ccpsdv3/cpsd.c:169-491) of all orbitals;
+the timed region runs exactly --steps of them, resident in HBM, in calls of
+cpsd_dev_run(inner <= 10), each ending with the energy tail.  Per-phase GPU
+times come from HIP events recorded on the library's stream inside the timed
+region.  A second run from the same start measures the time until
+|E(k) - E(k-1)| < 1e-7 Ha between energy evaluations (10 steps apart).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "simple-cpsd_amd"))
+
+import numpy as np  # noqa: E402
+
+WORKLOADS = {
+    "h2o-1": dict(nmol=1, L=20.0, nfft=32, ne=4),
+    "h2o-32": dict(nmol=32, L=18.6, nfft=96, ne=128),
+    "h2o-64": dict(nmol=64, L=23.5, nfft=128, ne=256),
+}
+HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def algorithmic(phase, w, ng, nproj, ndnblk):
+    """(kind, amount per SD step) of the algorithmic work of one phase:
+    kind 'flop' for MFMA contractions (flops over the masked G vectors,
+    complex coefficients as 2 reals), 'byte' for FFT / elementwise passes
+    (compulsory traffic of the pass: its inputs read once, outputs written
+    once)."""
+    n, ne = w["nfft"], w["ne"]
+    nh = n // 2 + 1
+    plane = n * n * nh * 16            # one orbital's z-transformed array
+    packed = ng * 16                   # one orbital's masked coefficients
+    n3 = n ** 3 * 8
+    if phase == "kb_projection":
+        return "flop", 2.0 * nproj * ne * 2 * ng
+    if phase == "kb_apply_sd":
+        return "flop", 2.0 * nproj * ne * 2 * ng
+    if phase == "gram":
+        return "flop", 2.0 * (ne * ne + ne * (ne + 1)) * 2 * ng
+    if phase == "constraint":
+        return "flop", 2.0 * ne * ne * 2 * ng
+    if phase == "zpass_g2r":
+        return "byte", ne * (packed + plane)
+    if phase == "plane_density":
+        return "byte", ne * plane + ndnblk * n3 * 2 + 3 * n3
+    if phase == "plane_vpsi":
+        return "byte", ne * 2 * plane + n3
+    if phase == "zpass_r2g_hpsi":
+        return "byte", ne * (plane + 2 * packed)
+    return None, None
+
+
+def run(args):
+    import cpsd_amd
+    w = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    cell = cpsd_amd.water_cell(w["nmol"], w["L"], w["nfft"], seed=args.seed)
+    psi = cell.random_orbitals(w["ne"], seed=args.seed + 1, decay=0.0)
+    t_setup = time.perf_counter() - t0
+
+    dev = cpsd_amd.DeviceSD(cell, w["ne"])
+    dev.set_orbitals(psi)
+    sizes = dev.sizes()
+    inner = 10
+
+    def steps(k):
+        E = de = dc = None
+        while k > 0:
+            m = min(inner, k)
+            E, de, dc = dev.run(m)
+            k -= m
+        return E, de, dc
+
+    steps(args.warmup)
+    dev.profile(True)
+    t0 = time.perf_counter()
+    E, de, dc = steps(args.steps)
+    elapsed = time.perf_counter() - t0
+    prof = dev.profile_read()
+    dev.profile(False)
+    lag_it, lag_ad = dev.lagrange_info()
+
+    # time to |dE| < 1e-7 Ha between energy evaluations, from the same start
+    conv = None
+    if args.converge_steps > 0:
+        dev.set_orbitals(psi)
+        dev.E[:] = 0.0
+        t0 = time.perf_counter()
+        done, nsteps, last = False, 0, None
+        Eprev = None
+        while nsteps < args.converge_steps:
+            Ec, dec, dcc = dev.run(inner)
+            nsteps += inner
+            if Eprev is not None and abs(Ec[0] - Eprev) < 1e-7:
+                done = True
+                break
+            last = None if Eprev is None else Ec[0] - Eprev
+            Eprev = Ec[0]
+        tconv = time.perf_counter() - t0
+        conv = dict(converged=done, steps=nsteps, seconds=round(tconv, 4),
+                    E=float(Ec[0]), last_dE=None if last is None else float(last))
+    dev.close()
+
+    ms = elapsed * 1000.0 / args.steps
+    value = args.steps / elapsed
+    # per-phase GPU time per SD step and the roofline of the dominant phase
+    ndnblk = min(8, w["ne"])
+    phases = {}
+    for name, (tms, calls) in prof.items():
+        if calls == 0:
+            continue
+        per_step = tms / args.steps if name != "energy_tail" else tms / max(1, calls) / inner
+        phases[name] = dict(ms_total=round(tms, 3), launches=calls,
+                            ms_per_step=round(per_step, 4),
+                            launches_per_step=round(calls / args.steps, 3))
+    roofline = None
+    for name in sorted(phases, key=lambda n: -phases[n]["ms_per_step"]):
+        kind, amount = algorithmic(name, w, sizes["ng"], sizes["nproj"], ndnblk)
+        if kind is None:
+            continue
+        sec = phases[name]["ms_per_step"] / 1000.0
+        per_launch = amount / max(1e-9, phases[name]["launches_per_step"])
+        if kind == "flop":
+            ach = amount / sec / 1e12
+            roofline = dict(bound="mfma", kernel=name, achieved=round(ach, 3),
+                            peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
+                            frac=round(ach / FP64_MFMA_PEAK_TFLOPS, 4), traffic=None,
+                            algorithmic_per_launch=per_launch)
+        else:
+            ach = amount / sec / 1e9
+            roofline = dict(bound="hbm", kernel=name, achieved=round(ach, 1), peak=HBM_PEAK_GBS,
+                            unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                            algorithmic_per_launch=per_launch)
+        break
+    for name in phases:
+        kind, amount = algorithmic(name, w, sizes["ng"], sizes["nproj"], ndnblk)
+        if kind is not None:
+            sec = phases[name]["ms_per_step"] / 1000.0
+            phases[name]["achieved"] = round(amount / sec / (1e12 if kind == "flop" else 1e9), 3)
+            phases[name]["unit"] = "TFLOP/s" if kind == "flop" else "GB/s"
+
+    cpu = None
+    if args.cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        t0 = time.perf_counter()
+        est = oracle_lib.step_time_sampled(w["nfft"], w["ne"], cell.nion, cell.nproj,
+                                           lag_iters=lag_it, inner=inner, sample=args.cpu_sample)
+        tcpu = time.perf_counter() - t0
+        cpu = dict(value=round(1.0 / est["total"], 6), unit="SD steps/s", cores=1, kind="port",
+                   sample=("serial FP64 restatement (oracle), one SD step of this workload "
+                           "estimated from %d timed work items per component at full array "
+                           "size (3D FFT pairs, orbital passes, 8x KB pairs, overlaps, "
+                           "constraint columns), one full Lagrange iteration x %d, LDA, "
+                           "energy tail / %d; %.1f s of CPU work" %
+                           (args.cpu_sample, lag_it, inner, tcpu)),
+                   seconds_per_step=round(est["total"], 4),
+                   parts={k: round(v, 5) for k, v in est.items()},
+                   nproc=os.cpu_count())
+
+    out = {
+        "metric": "SD steps/s and time-to-|dE|<1e-7 Ha: mixed+learned vs FP64 GPU, 1 MI355X",
+        "value": round(value, 3),
+        "unit": "SD steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded water positions and Gaussian orbitals)",
+        "config": {"workload": args.workload, "waters": w["nmol"], "atoms": 3 * w["nmol"],
+                   "cell_bohr": w["L"], "fft_grid": w["nfft"], "orbitals": w["ne"],
+                   "projectors": sizes["nproj"], "masked_G": sizes["ng"], "inner": inner,
+                   "precision": "fp64", "start": "gaussian", "setup_s": round(t_setup, 2)},
+        "time_to_converge": conv,
+        "lagrange_iterations_last_step": lag_it,
+        "E_last": float(E[0]),
+        "phases": phases,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=500)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--workload", default="h2o-32", choices=sorted(WORKLOADS))
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--converge-steps", type=int, default=3000)
+    p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--cpu-sample", type=int, default=16)
+    args = p.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the SD step of one "
+              "cell runs on one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

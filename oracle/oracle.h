@@ -102,6 +102,11 @@ int oracle_last_lagrange_iterations(void);
  * does; V column-major n x n.                                               */
 void oracle_syev_sorted(int n, double *a, double *eig);
 
+/* CPU-baseline estimate of one SD step from a bounded sample of each
+ * component at full size (sample_timing.c); parts[10]. */
+double oracle_step_time_sampled(int nfft, int ne, int nion, int nproj, int lag_iters, int inner,
+                                int sample_items, double *parts);
+
 #ifdef __cplusplus
 }
 #endif

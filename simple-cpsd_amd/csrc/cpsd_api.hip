@@ -95,7 +95,7 @@ struct cpsd_dev {
    int cur;               /* C2 lives in block cur */
    double2 *T, *Tv;
    double *dnpart, *dn, *xcp, *xce, *V;
-   int ndnblk, nob;
+   int ndnblk, nob, fb;
    double2 *dng, *s, *vall, *B;
    double *partial; size_t partial_cap;
    double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
@@ -106,29 +106,46 @@ struct cpsd_dev {
    double *realbuf;       /* n2ft3d staging */
    double omega, cewald, rcut;
    int have_run;
-   /* profiling */
+   /* profiling: event pairs recorded on the stream without synchronising;
+    * they are resolved when the pool fills or the profile is read */
    int prof;
-   hipEvent_t ev0[PH_COUNT], ev1[PH_COUNT];
+   hipEvent_t *pool;
+   int *pool_ph;
+   int pool_cap, pool_used;
    double prof_ms[PH_COUNT];
    int prof_calls[PH_COUNT];
    int last_lag_iters; double last_lag_adiff;
 };
 
+static int ph_flush(cpsd_dev *c)
+{
+   if (!c->pool_used) return 0;
+   CPSD_CHECK(hipEventSynchronize(c->pool[2 * c->pool_used - 1]));
+   for (int i = 0; i < c->pool_used; ++i) {
+      float ms = 0.f;
+      CPSD_CHECK(hipEventElapsedTime(&ms, c->pool[2 * i], c->pool[2 * i + 1]));
+      c->prof_ms[c->pool_ph[i]] += ms;
+      c->prof_calls[c->pool_ph[i]] += 1;
+   }
+   c->pool_used = 0;
+   return 0;
+}
+
 static int ph_begin(cpsd_dev *c, int ph)
 {
-   if (c->prof) CPSD_CHECK(hipEventRecord(c->ev0[ph], c->stream));
+   (void)ph;
+   if (c->prof) {
+      if (c->pool_used == c->pool_cap) TRY(ph_flush(c));
+      CPSD_CHECK(hipEventRecord(c->pool[2 * c->pool_used], c->stream));
+   }
    return 0;
 }
 
 static int ph_end(cpsd_dev *c, int ph)
 {
    if (c->prof) {
-      float ms = 0.f;
-      CPSD_CHECK(hipEventRecord(c->ev1[ph], c->stream));
-      CPSD_CHECK(hipEventSynchronize(c->ev1[ph]));
-      CPSD_CHECK(hipEventElapsedTime(&ms, c->ev0[ph], c->ev1[ph]));
-      c->prof_ms[ph] += ms;
-      c->prof_calls[ph] += 1;
+      CPSD_CHECK(hipEventRecord(c->pool[2 * c->pool_used + 1], c->stream));
+      c->pool_ph[c->pool_used++] = ph;
    }
    return 0;
 }
@@ -145,19 +162,21 @@ static void ctx_free(cpsd_dev *c)
                    c->fullbuf, c->realbuf};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
-   if (c->stream) {
-      for (int i = 0; i < PH_COUNT; ++i) {
-         (void)hipEventDestroy(c->ev0[i]);
-         (void)hipEventDestroy(c->ev1[i]);
-      }
-      (void)hipStreamDestroy(c->stream);
+   if (c->pool) {
+      for (int i = 0; i < 2 * c->pool_cap; ++i) (void)hipEventDestroy(c->pool[i]);
+      free(c->pool);
+      free(c->pool_ph);
    }
+   if (c->stream) (void)hipStreamDestroy(c->stream);
    memset(c, 0, sizeof(*c));
 }
 
+int gemm_tn_tile(void);
+
 static int splits_for(int M, int N, size_t K, int symm)
 {
-   const int tm = (M + 63) / 64, tn = (N + 63) / 64;
+   const int t = gemm_tn_tile();
+   const int tm = (M + t - 1) / t, tn = (N + t - 1) / t;
    int tiles = symm ? tm * (tm + 1) / 2 : tm * tn;
    int sp = 2048 / (tiles > 0 ? tiles : 1);
    const int maxsp = (int)(K / (16 * 8));
@@ -261,10 +280,6 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    }
 
    CPSD_CHECK(hipStreamCreate(&c->stream));
-   for (int i = 0; i < PH_COUNT; ++i) {
-      CPSD_CHECK(hipEventCreate(&c->ev0[i]));
-      CPSD_CHECK(hipEventCreate(&c->ev1[i]));
-   }
    hipStream_t st = c->stream;
    const size_t nn2 = (size_t)n * n;
    TRY(dalloc(&c->rstart, nn2)); TRY(dalloc(&c->rlen, nn2)); TRY(dalloc(&c->roff, nn2));
@@ -302,9 +317,20 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    CPSD_CHECK(hipMemsetAsync(c->CP, 0, orb * sizeof(double2), st));
    const size_t plane = (size_t)n * n * nh;
    TRY(dalloc(&c->T, (size_t)ne * plane)); TRY(dalloc(&c->Tv, plane));
-   c->ndnblk = ne < 8 ? ne : 8;
-   c->nob = (ne + c->ndnblk - 1) / c->ndnblk;
-   c->ndnblk = (ne + c->nob - 1) / c->nob;
+   /* FFT batch (orbitals per z-pass/plane-pass launch pair); all orbitals by
+    * default, CPSD_FFT_BATCH sets a smaller batch whose z-transformed planes
+    * stay in the Infinity Cache between the two launches */
+   {
+      long fb = ne;
+      const char *env = getenv("CPSD_FFT_BATCH");
+      if (env && atol(env) > 0) fb = atol(env);
+      if (fb < 1) fb = 1;
+      if (fb > ne) fb = ne;
+      c->fb = (int)fb;
+   }
+   c->ndnblk = c->fb < 8 ? c->fb : 8;
+   c->nob = (c->fb + c->ndnblk - 1) / c->ndnblk;
+   c->ndnblk = (c->fb + c->nob - 1) / c->nob;
    TRY(dalloc(&c->dnpart, (size_t)c->ndnblk * c->n3));
    TRY(dalloc(&c->dn, c->n3)); TRY(dalloc(&c->xcp, c->n3)); TRY(dalloc(&c->xce, c->n3));
    TRY(dalloc(&c->V, c->n3));
@@ -335,7 +361,7 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    TRY(dalloc(&c->gsum_part, 3 * (size_t)c->ngsum_parts));
    TRY(dalloc(&c->xc_part, 2 * (size_t)c->nxc_parts));
    TRY(dalloc(&c->orb_part, 2 * (size_t)c->norb_parts * ne));
-   TRY(dalloc(&c->ewald_part, nion)); TRY(dalloc(&c->E, 16));
+   TRY(dalloc(&c->ewald_part, (size_t)nion * nion)); TRY(dalloc(&c->E, 16));
    TRY(dalloc(&c->fullbuf, (size_t)ne * c->nfft3d));
    TRY(dalloc(&c->realbuf, c->n2ft3d));
    CPSD_CHECK(hipMemsetAsync(c->err, 0, 4 * sizeof(int), st));
@@ -383,19 +409,27 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    double2 *C2 = c->Cbase + (size_t)(c->cur ^ 1) * orb;
 
    ZArgs z;
-   zargs_base(c, z, 0);
-   /* 2 */
-   TRY(ph_begin(c, PH_ZG2R));
-   z.src = C1; z.sstride = ngp; z.T = c->T;
-   TRY(launch_zpass_g2r(n, SRC_PACKED, z, ne, st));
-   TRY(ph_end(c, PH_ZG2R));
-   /* 3, 4 */
-   TRY(ph_begin(c, PH_DENSITY));
    PlaneArgs pa;
-   memset(&pa, 0, sizeof pa);
-   pa.T = c->T; pa.nhload = c->nhc; pa.tw = c->tw; pa.b0 = 0; pa.nb = c->nob; pa.nbatch = ne;
-   pa.dnpart = c->dnpart;
-   TRY(launch_plane(n, PM_DENSITY, pa, c->ndnblk, st));
+   const size_t plane = (size_t)n * n * c->nh;
+   /* 2-3: orbitals in batches of fb, so that each batch's z-transformed
+    * planes are read by the density pass while they sit in the Infinity
+    * Cache; T keeps every orbital for the V*psi pass */
+   for (int b0 = 0; b0 < ne; b0 += c->fb) {
+      const int bs = ne - b0 < c->fb ? ne - b0 : c->fb;
+      TRY(ph_begin(c, PH_ZG2R));
+      zargs_base(c, z, 0);
+      z.src = C1 + (size_t)b0 * ngp; z.sstride = ngp; z.T = c->T + (size_t)b0 * plane;
+      TRY(launch_zpass_g2r(n, SRC_PACKED, z, bs, st));
+      TRY(ph_end(c, PH_ZG2R));
+      TRY(ph_begin(c, PH_DENSITY));
+      memset(&pa, 0, sizeof pa);
+      pa.T = c->T + (size_t)b0 * plane; pa.nhload = c->nhc; pa.tw = c->tw; pa.b0 = 0;
+      pa.nb = c->nob; pa.nbatch = bs; pa.dnpart = c->dnpart; pa.accumulate = b0 > 0;
+      TRY(launch_plane(n, PM_DENSITY, pa, (bs + c->nob - 1) / c->nob, st));
+      TRY(ph_end(c, PH_DENSITY));
+   }
+   /* 4 */
+   TRY(ph_begin(c, PH_DENSITY));
    TRY(sd_dn_reduce_xc(c->dnpart, c->ndnblk, c->n3, scal2, c->dn, c->xcp, c->xce, st));
    TRY(ph_end(c, PH_DENSITY));
    /* 5 */
@@ -416,18 +450,22 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    pa.T = c->Tv; pa.nhload = c->nhc; pa.tw = c->tw; pa.Vout = c->V; pa.xcp = c->xcp; pa.nbatch = 1;
    TRY(launch_plane(n, PM_VTOT, pa, 1, st));
    TRY(ph_end(c, PH_VTOT));
-   /* 7 */
-   TRY(ph_begin(c, PH_VPSI));
-   memset(&pa, 0, sizeof pa);
-   pa.T = c->T; pa.nhload = c->nhc; pa.tw = c->tw; pa.V = c->V; pa.nbatch = ne;
-   TRY(launch_plane(n, PM_VPSI, pa, ne, st));
-   TRY(ph_end(c, PH_VPSI));
-   /* 8 */
-   TRY(ph_begin(c, PH_ZR2G));
-   zargs_base(c, z, 0);
-   z.T = c->T; z.out = c->CP; z.ostride = ngp; z.c1 = C1; z.tg = c->tg; z.scal1 = scal1;
-   TRY(launch_zpass_r2g(n, DST_HPSI, z, ne, st));
-   TRY(ph_end(c, PH_ZR2G));
+   /* 7-8, batched the same way: the z pass reads what the plane pass wrote */
+   for (int b0 = 0; b0 < ne; b0 += c->fb) {
+      const int bs = ne - b0 < c->fb ? ne - b0 : c->fb;
+      TRY(ph_begin(c, PH_VPSI));
+      memset(&pa, 0, sizeof pa);
+      pa.T = c->T + (size_t)b0 * plane; pa.nhload = c->nhc; pa.tw = c->tw; pa.V = c->V;
+      pa.nbatch = bs;
+      TRY(launch_plane(n, PM_VPSI, pa, bs, st));
+      TRY(ph_end(c, PH_VPSI));
+      TRY(ph_begin(c, PH_ZR2G));
+      zargs_base(c, z, 0);
+      z.T = c->T + (size_t)b0 * plane; z.out = c->CP + (size_t)b0 * ngp; z.ostride = ngp;
+      z.c1 = C1 + (size_t)b0 * ngp; z.tg = c->tg; z.scal1 = scal1;
+      TRY(launch_zpass_r2g(n, DST_HPSI, z, bs, st));
+      TRY(ph_end(c, PH_ZR2G));
+   }
    /* 9 */
    const double *A2 = reinterpret_cast<const double *>(C1);
    if (c->nproj > 0) {
@@ -649,6 +687,14 @@ extern "C" int cpsd_dev_sizes(cpsd_dev *h, int *sizes)
 
 extern "C" int cpsd_dev_profile(cpsd_dev *h, int enable)
 {
+   TRY(ph_flush(h));
+   if (enable && !h->pool) {
+      h->pool_cap = 2048;
+      h->pool = (hipEvent_t *)calloc(2 * h->pool_cap, sizeof(hipEvent_t));
+      h->pool_ph = (int *)calloc(h->pool_cap, sizeof(int));
+      if (!h->pool || !h->pool_ph) { cpsd_set_error("out of host memory"); return -1; }
+      for (int i = 0; i < 2 * h->pool_cap; ++i) CPSD_CHECK(hipEventCreate(&h->pool[i]));
+   }
    h->prof = enable;
    for (int i = 0; i < PH_COUNT; ++i) { h->prof_ms[i] = 0.0; h->prof_calls[i] = 0; }
    return 0;
@@ -659,6 +705,7 @@ extern "C" int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, do
 {
    std::string s;
    int k = 0;
+   TRY(ph_flush(h));
    for (int i = 0; i < PH_COUNT && i < max_phases; ++i, ++k) {
       if (i) s += ";";
       s += ph_names[i];

@@ -66,6 +66,7 @@ struct PlaneArgs {
    int nhload;           /* kx columns loaded (others are zero) */
    const double2 *tw;
    int b0, nb;           /* PM_DENSITY: orbitals [b0 + blockIdx.y*nb, ...) */
+   int accumulate;       /* PM_DENSITY: add into dnpart instead of overwriting */
    int nbatch;
    double *dnpart;       /* PM_DENSITY: [blocks][N^3] */
    const double *V;      /* PM_VPSI */

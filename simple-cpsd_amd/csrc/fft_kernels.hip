@@ -23,32 +23,64 @@
 
 /* ------------------------------------------------------------ z passes */
 
+/* row-table entries (start, length, offset) of the N rows (ky, kz = 0..N-1)
+ * a z-pass workgroup touches, staged once in LDS */
+template<int N>
+__device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*rt)[N])
+{
+   for (int kz = threadIdx.x; kz < N; kz += blockDim.x) {
+      const int r = ky + N * kz;
+      rt[0][kz] = rows.rstart[r];
+      rt[1][kz] = rows.rlen[r];
+      rt[2][kz] = rows.roff[r];
+   }
+   __syncthreads();
+}
+
 template<int N, int SRC>
 __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB;
+   constexpr int EPT = (N * KXB + 255) / 256;   /* elements per thread */
    __shared__ double2 sm[N * KXB];
+   __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
    const int ncol = min(KXB, a.rows.nhc - kx0);
    const double2 *src = a.src + (size_t)b * a.sstride;
+   stage_rows<N>(a.rows, ky, rt);
 
-   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+   /* every load of this thread is issued before any is consumed */
+   int pe[EPT];
+#pragma unroll
+   for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + 256 * q;
       const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
-      const int r = ky + N * kz;
-      const int rs = a.rows.rstart[r], rl = a.rows.rlen[r];
-      double2 v = make_double2(0.0, 0.0);
-      if (kxl < ncol && kx >= rs && kx < rs + rl) {
-         const int p = a.rows.roff[r] + kx - rs;
-         if (SRC == SRC_PACKED) {
-            v = src[p];
-         } else {
-            const double2 vl = a.vall[p], dg = a.dng[p];
-            const double c = a.vc[p];
-            v = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
+      pe[q] = -1;
+      if (e < N * KXB && kxl < ncol) {
+         const int rs = rt[0][kz], rl = rt[1][kz];
+         if (kx >= rs && kx < rs + rl) pe[q] = rt[2][kz] + kx - rs;
+      }
+   }
+   double2 v[EPT];
+   if (SRC == SRC_PACKED) {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) v[q] = pe[q] >= 0 ? src[pe[q]] : make_double2(0.0, 0.0);
+   } else {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+         v[q] = make_double2(0.0, 0.0);
+         if (pe[q] >= 0) {
+            const double2 vl = a.vall[pe[q]], dg = a.dng[pe[q]];
+            const double c = a.vc[pe[q]];
+            v[q] = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
          }
       }
-      sm[e] = v;
+   }
+#pragma unroll
+   for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      if (e < N * KXB) sm[e] = v[q];
    }
    __syncthreads();
    lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, a.tw, 1);
@@ -67,38 +99,67 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB;
+   constexpr int EPT = (N * KXB + 255) / 256;
    __shared__ double2 sm[N * KXB];
+   __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
    const int ncol = min(KXB, a.rows.nhc - kx0);
    const double2 *T = a.T + (size_t)b * N * N * NH;
+   stage_rows<N>(a.rows, ky, rt);
 
-   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
-      const int s = e / KXB, kxl = e % KXB;
-      double2 v = make_double2(0.0, 0.0);
-      if (kxl < ncol) {
-         const int z = tr_index<C::NY1, C::NY2>(s);
-         v = T[((size_t)z * N + ky) * NH + kx0 + kxl];
+   {
+      double2 v[EPT];
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+         const int e = threadIdx.x + 256 * q;
+         const int s = e / KXB, kxl = e % KXB;
+         v[q] = make_double2(0.0, 0.0);
+         if (e < N * KXB && kxl < ncol) {
+            const int z = tr_index<C::NY1, C::NY2>(s);
+            v[q] = T[((size_t)z * N + ky) * NH + kx0 + kxl];
+         }
       }
-      sm[e] = v;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+         const int e = threadIdx.x + 256 * q;
+         if (e < N * KXB) sm[e] = v[q];
+      }
    }
    __syncthreads();
    lds_fft_tr2nat<C::NY1, C::NY2, -1>(sm, ncol, 1, KXB, a.tw, 1);
    double2 *out = a.out + (size_t)b * a.ostride;
-   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+   int pe[EPT];
+#pragma unroll
+   for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + 256 * q;
       const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
-      const int r = ky + N * kz;
-      const int rs = a.rows.rstart[r], rl = a.rows.rlen[r];
-      if (kxl < ncol && kx >= rs && kx < rs + rl) {
-         const int p = a.rows.roff[r] + kx - rs;
-         const double2 v = sm[e];
-         if (DST == DST_HPSI) {
-            const double2 c = a.c1[(size_t)b * a.ostride + p];
-            const double t = a.tg[p];
-            out[p] = make_double2(-a.scal1 * v.x - t * c.x, -a.scal1 * v.y - t * c.y);
-         } else {
-            out[p] = v;
+      pe[q] = -1;
+      if (e < N * KXB && kxl < ncol) {
+         const int rs = rt[0][kz], rl = rt[1][kz];
+         if (kx >= rs && kx < rs + rl) pe[q] = rt[2][kz] + kx - rs;
+      }
+   }
+   if (DST == DST_HPSI) {
+      /* kinetic term inputs, all loads issued first */
+      double2 c[EPT];
+      double t[EPT];
+      const double2 *c1 = a.c1 + (size_t)b * a.ostride;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+         c[q] = pe[q] >= 0 ? c1[pe[q]] : make_double2(0.0, 0.0);
+         t[q] = pe[q] >= 0 ? a.tg[pe[q]] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+         if (pe[q] >= 0) {
+            const double2 v = sm[threadIdx.x + 256 * q];
+            out[pe[q]] = make_double2(-a.scal1 * v.x - t[q] * c[q].x, -a.scal1 * v.y - t[q] * c[q].y);
          }
       }
+   } else {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q)
+         if (pe[q] >= 0) out[pe[q]] = sm[threadIdx.x + 256 * q];
    }
 }
 
@@ -166,10 +227,18 @@ __device__ __forceinline__ void r2c_post(double2 *sm, const double2 *__restrict_
 template<int N>
 __device__ __forceinline__ void load_plane(double2 *sm, const double2 *__restrict__ Tp, int nhload)
 {
-   constexpr int NH = N / 2 + 1;
-   for (int e = threadIdx.x; e < N * NH; e += blockDim.x) {
-      const int kx = e % NH;
-      sm[e] = (kx < nhload) ? Tp[e] : make_double2(0.0, 0.0);
+   constexpr int NH = N / 2 + 1, NT = GridCfg<N>::PLANE_THREADS;
+   constexpr int EPT = (N * NH + NT - 1) / NT;
+   double2 v[EPT];
+#pragma unroll
+   for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + NT * q;
+      v[q] = (e < N * NH && e % NH < nhload) ? Tp[e] : make_double2(0.0, 0.0);
+   }
+#pragma unroll
+   for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + NT * q;
+      if (e < N * NH) sm[e] = v[q];
    }
    __syncthreads();
 }
@@ -250,7 +319,8 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS) k_plane(PlaneArgs a
             const int s = p / N, j = p % N;
             int y, x0;
             slot_xy<N>(s, j >> 1, y, x0);
-            dst[y * N + x0 + (j & 1)] = acc[q];
+            double *d = dst + y * N + x0 + (j & 1);
+            *d = a.accumulate ? *d + acc[q] : acc[q];
          }
       }
    } else if constexpr (MODE == PM_VTOT || MODE == PM_C2R_OUT) {
@@ -276,17 +346,31 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS) k_plane(PlaneArgs a
    } else if constexpr (MODE == PM_VPSI) {
       const int b = blockIdx.y;
       double2 *Tp = a.T + ((size_t)b * N + z) * plane;
+      /* this plane's potential, fetched before the transforms need it */
+      constexpr int EPV = (N * M + NT - 1) / NT;
+      double2 vv[EPV];
+#pragma unroll
+      for (int q = 0; q < EPV; ++q) {
+         const int it = threadIdx.x + NT * q;
+         vv[q] = make_double2(1.0, 1.0);
+         if (it < N * M) {
+            int y, x0;
+            slot_xy<N>(it / M, it % M, y, x0);
+            vv[q] = *reinterpret_cast<const double2 *>(a.V + ((size_t)z * N + y) * N + x0);
+         }
+      }
       load_plane<N>(sm, Tp, a.nhload);
       inverse_yx<N>(sm, a.tw);
-      for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
-         const int s = it / M, m = it % M;
-         int y, x0;
-         slot_xy<N>(s, m, y, x0);
-         const size_t g = ((size_t)z * N + y) * N + x0;
-         double2 v = sm[s * NH + m];
-         v.x *= a.V[g];
-         v.y *= a.V[g + 1];
-         sm[s * NH + m] = v;
+#pragma unroll
+      for (int q = 0; q < EPV; ++q) {
+         const int it = threadIdx.x + NT * q;
+         if (it < N * M) {
+            const int s = it / M, m = it % M;
+            double2 v = sm[s * NH + m];
+            v.x *= vv[q].x;
+            v.y *= vv[q].y;
+            sm[s * NH + m] = v;
+         }
       }
       __syncthreads();
       forward_xy<N>(sm, a.tw);

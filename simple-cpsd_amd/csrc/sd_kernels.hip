@@ -281,16 +281,18 @@ __device__ __forceinline__ double erfc_as(double w)
    return e * e * e * e;
 }
 
-/* real-space Ewald pairs (ccpsdv3/cpsd.c:527-545): one thread per ion
- * ii, summing its pairs jj > ii in order; per-ion partials */
+/* real-space Ewald pairs (ccpsdv3/cpsd.c:527-545): one thread per pair
+ * (ii, jj > ii) summing its nsh images in order; part[ii*nion + jj]
+ * (zero for jj <= ii) */
 __global__ void __launch_bounds__(256)
 k_ewald_real(const double *__restrict__ r1, const int *__restrict__ katm, const double *__restrict__ zv,
              int nion, const double *__restrict__ rcell, int nsh, double rcut, double *__restrict__ part)
 {
-   const int ii = blockIdx.x * blockDim.x + threadIdx.x;
-   if (ii >= nion) return;
+   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= (size_t)nion * nion) return;
+   const int ii = (int)(t / nion), jj = (int)(t % nion);
    double e = 0.0;
-   for (int jj = ii + 1; jj < nion; ++jj) {
+   if (jj > ii) {
       const double dx = r1[3 * ii] - r1[3 * jj];
       const double dy = r1[3 * ii + 1] - r1[3 * jj + 1];
       const double dz = r1[3 * ii + 2] - r1[3 * jj + 2];
@@ -301,7 +303,7 @@ k_ewald_real(const double *__restrict__ r1, const int *__restrict__ katm, const 
          e += zz * erfc_as(r / rcut) / r;
       }
    }
-   part[ii] = e;
+   part[t] = e;
 }
 
 __global__ void k_erfc_as(const double *__restrict__ w, int n, double *__restrict__ out)
@@ -487,44 +489,62 @@ __global__ void k_zero_u32(unsigned *p, int n)
 
 /* ------------------------------------------------------ energy assembly */
 
-/* one thread: combine the partial sums into E[0..9], deltae, deltac
- * (ccpsdv3/cpsd.c:515-660) */
-__global__ void k_energy(EnergyArgs a)
+/* one workgroup of 256 threads: combine the partial sums into E[0..9],
+ * deltae, deltac (ccpsdv3/cpsd.c:515-660); every sum is a fixed strided
+ * loop followed by a fixed tree, so results do not depend on timing */
+__global__ void __launch_bounds__(256) k_energy(EnergyArgs a)
 {
-   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-   double eorbit = 0.0;
-   for (int i = 0; i < a.ne; ++i) eorbit += a.hml[i + (size_t)i * a.ne];
-   eorbit *= 2;
+   __shared__ double red[256];
+   const int t = threadIdx.x;
+   double v;
+   v = 0.0;
+   for (int i = t; i < a.ne; i += 256) v += a.hml[i + (size_t)i * a.ne];
+   const double eorbit = 2.0 * block_sum<256>(v, red);
    double sg = 0.0, sh = 0.0, sl = 0.0;
-   for (int b = 0; b < a.ngsum_parts; ++b) {
+   for (int b = t; b < a.ngsum_parts; b += 256) {
       sg += a.gsum_part[3 * b]; sh += a.gsum_part[3 * b + 1]; sl += a.gsum_part[3 * b + 2];
    }
-   double eion = sg * 0.5 * a.scal2 + a.cewald;
-   for (int ii = 0; ii < a.nion; ++ii) eion += a.ewald_part[ii];
+   sg = block_sum<256>(sg, red);
+   sh = block_sum<256>(sh, red);
+   sl = block_sum<256>(sl, red);
+   v = 0.0;
+   for (size_t q = t; q < (size_t)a.nion * a.nion; q += 256) v += a.ewald_part[q];
+   const double ereal = block_sum<256>(v, red);
+   const double eion = sg * 0.5 * a.scal2 + a.cewald + ereal;
    const double ehartr = sh * a.omega * 0.5;
    double exc = 0.0, pxc = 0.0;
-   for (int b = 0; b < a.nxc_parts; ++b) { exc += a.xc_part[2 * b]; pxc += a.xc_part[2 * b + 1]; }
+   for (int b = t; b < a.nxc_parts; b += 256) { exc += a.xc_part[2 * b]; pxc += a.xc_part[2 * b + 1]; }
+   exc = block_sum<256>(exc, red);
+   pxc = block_sum<256>(pxc, red);
    exc += exc; pxc += pxc;
    exc *= a.dv; pxc *= a.dv;
    double eke = 0.0, dcmax = 0.0;
-   for (int i = 0; i < a.ne; ++i) {
+   for (int i = t; i < a.ne; i += 256) {
       double ek = 0.0, dc = 0.0;
       for (int b = 0; b < a.norb_parts; ++b) {
          ek += a.orb_part[((size_t)i * a.norb_parts + b) * 2];
          dc += a.orb_part[((size_t)i * a.norb_parts + b) * 2 + 1];
       }
       eke += ek;
-      if (dc > dcmax) dcmax = dc;
+      dcmax = fmax(dcmax, dc);
    }
-   eke *= 2;
+   eke = 2.0 * block_sum<256>(eke, red);
+   red[t] = dcmax;
+   __syncthreads();
+   for (int s = 128; s > 0; s >>= 1) {
+      if (t < s) red[t] = fmax(red[t], red[t + s]);
+      __syncthreads();
+   }
+   dcmax = red[0];
+   __syncthreads();
    const double elocal = sl;
-   double enl = 0.0;
-   for (int m = 0; m < a.nproj; ++m)
-      for (int n = 0; n < a.ne; ++n) {
-         const double p = a.P[(size_t)m * a.ne + n];
-         enl += p * p * a.scal2 / a.pnrm[m];
-      }
-   enl *= 2;
+   v = 0.0;
+   for (size_t q = t; q < (size_t)a.nproj * a.ne; q += 256) {
+      const double p = a.P[q];
+      v += p * p * a.scal2 / a.pnrm[q / a.ne];
+   }
+   const double enl = 2.0 * block_sum<256>(v, red);
+   if (t != 0) return;
    const double eold = a.E[0];
    double *E = a.E;
    E[0] = eorbit + eion + exc - ehartr - pxc;
@@ -640,7 +660,7 @@ int sd_orbsums(const double2 *c1, const double2 *c2, double2 *cp, const double *
 int sd_ewald_real(const double *r1, const int *katm, const double *zv, int nion, const double *rcell,
                   int nsh, double rcut, double *part, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_ewald_real, g1(nion), dim3(256), 0, s, r1, katm, zv, nion, rcell, nsh, rcut, part);
+   hipLaunchKernelGGL(k_ewald_real, g1((size_t)nion * nion), dim3(256), 0, s, r1, katm, zv, nion, rcell, nsh, rcut, part);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -687,7 +707,7 @@ int sd_lagrange(const LagArgs &a, hipStream_t s)
 
 int sd_energy(const EnergyArgs &a, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_energy, dim3(1), dim3(64), 0, s, a);
+   hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

@@ -211,3 +211,19 @@ class State:
         if rc != 0:
             raise RuntimeError("oracle_cpsd failed: %d" % rc)
         return self.E.copy(), self.deltae.value, self.deltac.value
+
+
+SAMPLE_PARTS = ("fft", "orbital_passes", "kb_pairs", "structure", "overlaps", "lagrange",
+                "constraint", "lda", "energy_tail", "total")
+
+
+def step_time_sampled(nfft, ne, nion, nproj, lag_iters=5, inner=10, sample=4):
+    """Serial restatement's seconds per SD step, from a bounded sample of
+    every component at full array size (oracle/sample_timing.c)."""
+    L = lib()
+    f = L.oracle_step_time_sampled
+    f.argtypes = [C.c_int] * 7 + [dp]
+    f.restype = C.c_double
+    parts = np.zeros(10)
+    f(nfft, ne, nion, nproj, lag_iters, inner, sample, parts)
+    return dict(zip(SAMPLE_PARTS, parts))
