@@ -103,6 +103,7 @@ struct TallArgs {
    size_t K;                       /* output rows (multiple of 64) */
    double *C2; const double *C1; double *CP; size_t ldc;
    double dte; int write_cp;
+   int ntn, per;                   /* set by the launcher */
 };
 int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);
 

@@ -7,69 +7,123 @@
  *   - hml             -gcdotc(c1_i, cpsi_j)         ccpsdv3/cpsd.c:497-513
  *   - KB apply        c2_i += sw * beta (daxpy)     ccpsdv3/cpsd.c:290
  *   - constraint      c2 += dte c1 lmd (dgemm)      ccpsdv3/cpsd.c:482-488
- * Here they are two kernels built on v_mfma_f64_16x16x4_f64:
+ * Here they are two kernels built on v_mfma_f64_4x4x4_4b_f64, which on
+ * gfx950 sustains ~74 TFLOP/s against ~36-49 for the 16x16x4 FP64 form:
  *   k_gemm_tn_splitk : small M x N output, reduction over the huge K = 2*ngp
  *                      real coefficients, split over blockIdx.z; 128 x 128
- *                      output tiles (4 waves of 64 x 64); the G-space weight
+ *                      output tiles (8 waves of 64 x 32); the G-space weight
  *                      (1 on the kx = 0 plane, 2 elsewhere: the
  *                      2*sum - sum_{kx=0} of ccpsdv3/gcdotc.c:25-27) is
- *                      applied while staging A.
+ *                      applied to the A operand.
  *   k_gemm_tall      : tall output [ncol][K] = X^T A with a short reduction
- *                      (projectors or orbitals), 128 x 128 output tiles over
- *                      8 waves, SD epilogues fused.
- * MFMA operand layout (wave64): lane l supplies A[l%16][l/16] and
- * B[l/16][l%16] of the 16x4 and 4x16 blocks; the 16x16 result comes back as
- * four doubles per lane, d[r] = C[l/16 + 4*r][l%16] (measured on gfx950).
+ *                      (projectors or orbitals), 64-column x 128-row output
+ *                      tiles over 8 waves, SD epilogues fused.
+ * Instruction layout (measured on gfx950): with A lane l = A[l%16][l/16] and
+ * B lane l = B[l/16][l%16], one instruction forms the four diagonal 4x4
+ * blocks of the 16x16 product A(16x4) B(4x16); output lane o = 16q + 4a + b
+ * holds C[4a+q][4a+b].  Feeding B with its 4-column groups rotated by s
+ * (column c taken from 4((c/4 + s) % 4) + c%4) gives C[4a+q][4((a+s)%4)+b],
+ * so four rotations cover a 16 x 16 tile.
  */
 #include "cpsd_internal.h"
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c)
+__device__ __forceinline__ double mfma4(double a, double b, double c)
 {
-   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+   return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+/* v taken from the lane L places along this lane's row of 16 (DPP row
+ * rotate, two 32-bit moves); which way the row turns is read once per kernel
+ * by rot_step() so that the epilogue indexing never depends on it */
+template<int L>
+__device__ __forceinline__ double row_ror(double v)
+{
+   const long long u = __double_as_longlong(v);
+   const int lo = __builtin_amdgcn_mov_dpp((int)u, 0x120 + L, 0xf, 0xf, false);
+   const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x120 + L, 0xf, 0xf, false);
+   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+/* 1 if row_ror<4> hands lane c the value of lane c+4, 3 if that of lane c-4;
+ * the rotation fed by row_ror<4r> is then (rot_step() * r) % 4 */
+__device__ __forceinline__ int rot_step(int lane)
+{
+   const int src = __builtin_amdgcn_mov_dpp(lane, 0x124, 0xf, 0xf, false);
+   return ((src - lane) & 15) >> 2;
+}
+
+/* B operands for the four rotations of a 16-column block from the unrotated one */
+__device__ __forceinline__ void rotations(double b0, double b[4])
+{
+   b[0] = b0;
+   b[1] = row_ror<4>(b0);
+   b[2] = row_ror<8>(b0);
+   b[3] = row_ror<12>(b0);
 }
 
 /* ------------------------------------------------------- split-K TN */
 
 #define TT 128  /* output tile edge */
-#define KT 16   /* K step */
+#define KT 32   /* K step */
+#define TP (TT + 2)  /* row pitch of the epilogue staging tile */
 
-/* two waves per SIMD: at most 256 VGPR + AGPR per lane */
-__global__ void __launch_bounds__(256, 2)
+/* 8 waves, each a 64 x 32 block of the 128 x 128 tile.  One workgroup per
+ * (tile, K split); the grid is 1D over the valid pairs only.  Workgroups go
+ * to the 8 XCDs round-robin by block id, so block b is given pair
+ * (b % 8) * per + b / 8: the workgroups one XCD runs together hold
+ * consecutive pairs, i.e. all tiles of the same K range, which then share
+ * that XCD's L2 (per = pairs / 8; 0 falls back to b itself). */
+__global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
                  const double *__restrict__ B, size_t ldb, int N,
-                 const double *__restrict__ w, size_t K, size_t kchunk, int symm,
-                 double *__restrict__ partial)
+                 const double *__restrict__ w, size_t K, size_t kchunk, int symm, int ntm,
+                 int ntile, int per, double *__restrict__ partial)
 {
-   const int tm = blockIdx.x, tn = blockIdx.y, kz = blockIdx.z;
-   if (symm && tm > tn) return;
+   const int bid = blockIdx.x;
+   const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
+   const int t = g % ntile, kz = g / ntile;
+   int tm, tn;
+   if (symm) {   /* t enumerates the upper triangle row by row */
+      tm = 0;
+      int rem = t;
+      while (rem >= ntm - tm) { rem -= ntm - tm; ++tm; }
+      tn = tm + rem;
+   } else {
+      tm = t % ntm;
+      tn = t / ntm;
+   }
    const size_t kb = (size_t)kz * kchunk;
    const size_t ke = min(K, kb + kchunk);
-   __shared__ double As[2][TT][KT + 1];
-   __shared__ double Bs[2][TT][KT + 1];
+   /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
+    * stages the output tile for the epilogue */
+   __shared__ double S[2][2][TT][KT + 1];
    __shared__ double Ws[2][KT];
+   static_assert(sizeof(S) >= sizeof(double) * TT * TP, "staging tile does not fit");
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int wm = wave & 1, wn = wave >> 1;
+   const int s1 = rot_step(lane);
    /* a diagonal tile of a symmetric product reads one operand block */
    const bool same = symm && tm == tn;
-   /* loader: row tid/2 of the tile, 8 consecutive k at 8*(tid%2) */
-   const int lr = tid >> 1, lq = (tid & 1) * 8;
+   const int bsel = same ? 0 : 1;
+   /* loader: row tid/4 of the tile, 8 consecutive k at 8*(tid%4) */
+   const int lr = tid >> 2, lq = (tid & 3) * 8;
    const int am = tm * TT + lr, bn = tn * TT + lr;
    const bool aval = am < M, bval = bn < N && !same;
    const double *Ap = A + (size_t)(am < M ? am : 0) * lda;
    const double *Bp = B + (size_t)(bn < N ? bn : 0) * ldb;
-   double (*Bt)[TT][KT + 1] = same ? As : Bs;
 
-   v4d acc[4][4];
+   double acc[4][2][4];   /* [row block][column block][rotation] */
 #pragma unroll
    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+         for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
 
-   /* raw prefetch registers: nothing consumes them until the next LDS store,
-    * so the loads stay in flight across the MFMA work of the current tile;
-    * the K-tile's 16 weights are fetched by 8 threads only */
+   /* K tile k+1 waits in registers while tile k is multiplied; it is moved
+    * to the other LDS buffer halfway through tile k's MFMA stream and tile
+    * k+2 is requested right after, so loads, LDS stores and MFMAs overlap
+    * with one barrier per tile */
    double2 ra[4], rb[4], rw;
    auto gload = [&](size_t k0) {
 #pragma unroll
@@ -81,50 +135,84 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       }
       if (tid < KT / 2) rw = *reinterpret_cast<const double2 *>(w + k0 + 2 * tid);
    };
-
-   int buf = 0;
-   if (kb < ke) gload(kb);
-   for (size_t k0 = kb; k0 < ke; k0 += KT) {
+   auto lstore = [&](int bf) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-         As[buf][lr][lq + 2 * h] = ra[h].x;
-         As[buf][lr][lq + 2 * h + 1] = ra[h].y;
+         S[bf][0][lr][lq + 2 * h] = ra[h].x;
+         S[bf][0][lr][lq + 2 * h + 1] = ra[h].y;
          if (!same) {
-            Bs[buf][lr][lq + 2 * h] = rb[h].x;
-            Bs[buf][lr][lq + 2 * h + 1] = rb[h].y;
+            S[bf][1][lr][lq + 2 * h] = rb[h].x;
+            S[bf][1][lr][lq + 2 * h + 1] = rb[h].y;
          }
       }
-      if (tid < KT / 2) { Ws[buf][2 * tid] = rw.x; Ws[buf][2 * tid + 1] = rw.y; }
-      __syncthreads();
-      if (k0 + KT < ke) gload(k0 + KT);
+      if (tid < KT / 2) { Ws[bf][2 * tid] = rw.x; Ws[bf][2 * tid + 1] = rw.y; }
+   };
+   auto mma = [&](int bf, int ks) {
+      const int kk = 4 * ks + (lane >> 4);
+      const double wk = Ws[bf][kk];
+      double a[4];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-         const int kk = 4 * ks + (lane >> 4);
-         const double wk = Ws[buf][kk];
-         double a[4], b[4];
+      for (int i = 0; i < 4; ++i) a[i] = S[bf][0][64 * wm + 16 * i + (lane & 15)][kk] * wk;
 #pragma unroll
-         for (int i = 0; i < 4; ++i) {
-            a[i] = As[buf][64 * wm + 16 * i + (lane & 15)][kk] * wk;
-            b[i] = Bt[buf][64 * wn + 16 * i + (lane & 15)][kk];
-         }
+      for (int j = 0; j < 2; ++j) {
+         double b[4];
+         rotations(S[bf][bsel][32 * wn + 16 * j + (lane & 15)][kk], b);
 #pragma unroll
          for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = mfma4(a[i], b[r], acc[i][j][r]);
       }
+   };
+
+   if (kb < ke) {
+      gload(kb);
+      lstore(0);
+   }
+   __syncthreads();
+   if (kb + KT < ke) gload(kb + KT);
+   int buf = 0;
+   for (size_t k0 = kb; k0 < ke; k0 += KT) {
+#pragma unroll
+      for (int ks = 0; ks < KT / 8; ++ks) mma(buf, ks);
+      if (k0 + KT < ke) {
+         lstore(buf ^ 1);
+         if (k0 + 2 * KT < ke) gload(k0 + 2 * KT);
+      }
+#pragma unroll
+      for (int ks = KT / 8; ks < KT / 4; ++ks) mma(buf, ks);
+      __syncthreads();
       buf ^= 1;
    }
 
+   /* stage the tile in LDS, then write whole 1 KB rows of the partial */
+   double (*T)[TP] = reinterpret_cast<double (*)[TP]>(&S[0][0][0][0]);
+   {
+      const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
 #pragma unroll
-   for (int bi = 0; bi < 4; ++bi)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
+         for (int j = 0; j < 2; ++j)
 #pragma unroll
-         for (int r = 0; r < 4; ++r) {
-            const int m = tm * TT + 64 * wm + 16 * bi + (lane >> 4) + 4 * r;
-            const int n = tn * TT + 64 * wn + 16 * bj + (lane & 15);
-            if (m < M && n < N) partial[((size_t)kz * M + m) * N + n] = acc[bi][bj][r];
+            for (int r = 0; r < 4; ++r)
+               T[64 * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
+                  acc[i][j][r];
+   }
+   __syncthreads();
+   const int n = tn * TT + 2 * lane;
+   if (n < N) {
+#pragma unroll 4
+      for (int ml = wave; ml < TT; ml += 8) {
+         const int m = tm * TT + ml;
+         if (m >= M) break;
+         double *dst = partial + ((size_t)kz * M + m) * N + n;
+         if (n + 1 < N && !(N & 1)) {
+            *reinterpret_cast<double2 *>(dst) = make_double2(T[ml][2 * lane], T[ml][2 * lane + 1]);
+         } else {
+            dst[0] = T[ml][2 * lane];
+            if (n + 1 < N) dst[1] = T[ml][2 * lane + 1];
          }
+      }
+   }
 }
 
 int gemm_tn_tile(void) { return TT; }
@@ -134,11 +222,15 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
                           double *partial, hipStream_t s)
 {
    if (K % KT) { cpsd_set_error("gemm_tn: K=%zu not a multiple of %d", K, KT); return -1; }
+   if (symm && M != N) { cpsd_set_error("gemm_tn: symmetric product needs M == N"); return -1; }
    size_t kchunk = (K + splits - 1) / splits;
    kchunk = (kchunk + KT - 1) / KT * KT;
-   dim3 grid((M + TT - 1) / TT, (N + TT - 1) / TT, splits);
-   hipLaunchKernelGGL(k_gemm_tn_splitk, grid, dim3(256), 0, s, A, lda, M, B, ldb, N, w, K,
-                      kchunk, symm, partial);
+   const int ntm = (M + TT - 1) / TT, ntn = (N + TT - 1) / TT;
+   const int ntile = symm ? ntm * (ntm + 1) / 2 : ntm * ntn;
+   const long pairs = (long)ntile * splits;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   hipLaunchKernelGGL(k_gemm_tn_splitk, dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B, ldb,
+                      N, w, K, kchunk, symm, ntm, ntile, per, partial);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -170,113 +262,160 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
 /* ---------------------------------------------------------- tall NN */
 
 #define TK 128  /* output rows (K) per workgroup */
+#define MC 16   /* reduction step */
 
-/* waves arranged WK (along K) x WN (along the output columns) = 8 */
+/* waves arranged WK (along K) x WN (along the output columns) = 8; at most
+ * 128 VGPRs and 68 KB of LDS, so that two workgroups share a CU and one's
+ * epilogue traffic overlaps the other's MFMA stream */
 template<int EPI, int NT>
-__global__ void __launch_bounds__(512) k_gemm_tall(TallArgs a)
+__global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
 {
    constexpr int WN = NT / 64, WK = 8 / WN;
    constexpr int BI = TK / WK / 16;          /* 16-row blocks of K per wave */
    constexpr int BJ = 4;                     /* 16-column blocks per wave */
-   __shared__ double As[2][16][TK + 4];
-   __shared__ double Xs[2][NT][17];
+   constexpr int AS = 2 * MC * (TK + 4), XS = 2 * NT * (MC + 1), TS = 64 * (TK + 2);
+   constexpr int LD = AS + XS > TS ? AS + XS : TS;
+   /* main loop: As[2][MC][TK+4] and Xs[2][NT][MC+1]; epilogue: T[64][TK+2],
+    * one 64-column half of the output tile at a time */
+   __shared__ double lds[LD];
+   double (*As)[MC][TK + 4] = reinterpret_cast<double (*)[MC][TK + 4]>(lds);
+   double (*Xs)[NT][MC + 1] = reinterpret_cast<double (*)[NT][MC + 1]>(lds + AS);
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
    const int wk = wave % WK, wn = wave / WK;
-   const size_t k0 = (size_t)blockIdx.x * TK;
-   const int n0 = blockIdx.y * NT;
-   v4d acc[BI][BJ];
+   const int s1 = rot_step(lane);
+   /* block -> (K block, column tile) with the column tile fastest, remapped
+    * so that the workgroups one XCD runs together (block id mod 8) hold
+    * consecutive pairs: the column tiles of one K block then read its A
+    * rows through the same L2 */
+   const int bid = blockIdx.x;
+   const int g = a.per > 0 ? (bid & 7) * a.per + (bid >> 3) : bid;
+   const size_t k0 = (size_t)(g / a.ntn) * TK;
+   const int n0 = (g % a.ntn) * NT;
+   double acc[BI][BJ][4];   /* [K block][column block][rotation] */
 #pragma unroll
    for (int i = 0; i < BI; ++i)
 #pragma unroll
-      for (int j = 0; j < BJ; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < BJ; ++j)
+#pragma unroll
+         for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
 
-   /* loaders: A chunk 16 x TK (4 doubles per thread), X chunk NT x 16 */
-   constexpr int XQ = NT * 16 / 512;
+   /* loaders: A chunk MC x TK (4 doubles per thread), X chunk NT x MC */
+   constexpr int XQ = NT * MC / 512;
    const int ar = tid / (TK / 4), ac = (tid % (TK / 4)) * 4;
-   double ra[4], rx[XQ];
+   double2 ra[2];
+   double rx[XQ];
    auto gload = [&](int m0) {
       const int m = m0 + ar;
-      if (m < a.Mr) {
-         const double *p = a.A + (size_t)m * a.lda + k0 + ac;
-         const double2 x = *reinterpret_cast<const double2 *>(p);
-         const double2 y = *reinterpret_cast<const double2 *>(p + 2);
-         ra[0] = x.x; ra[1] = x.y; ra[2] = y.x; ra[3] = y.y;
-      } else {
-         ra[0] = ra[1] = ra[2] = ra[3] = 0.0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+         ra[h] = make_double2(0.0, 0.0);
+         if (m < a.Mr) ra[h] = *reinterpret_cast<const double2 *>(a.A + (size_t)m * a.lda + k0 + ac + 2 * h);
       }
 #pragma unroll
-      for (int q = 0; q < XQ; ++q) {
-         const int e = tid + 512 * q;
-         const int n = e >> 4, mm = e & 15;
+      for (int u = 0; u < XQ; ++u) {
+         const int e = tid + 512 * u;
+         const int n = e / MC, mm = e % MC;
          const int mg = m0 + mm, ng = n0 + n;
-         rx[q] = (mg < a.Mr && ng < a.Nc) ? a.X[mg + (size_t)ng * a.ldx] : 0.0;
+         rx[u] = (mg < a.Mr && ng < a.Nc) ? a.X[mg + (size_t)ng * a.ldx] : 0.0;
+      }
+   };
+   auto lstore = [&](int bf) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+         As[bf][ar][ac + 2 * h] = ra[h].x;
+         As[bf][ar][ac + 2 * h + 1] = ra[h].y;
+      }
+#pragma unroll
+      for (int u = 0; u < XQ; ++u) {
+         const int e = tid + 512 * u;
+         Xs[bf][e / MC][e % MC] = rx[u];
+      }
+   };
+   auto mma = [&](int bf, int ks) {
+      const int mm = 4 * ks + (lane >> 4);
+      double x[BJ];
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) x[j] = Xs[bf][64 * wn + 16 * j + (lane & 15)][mm];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) {
+         double b[4];
+         rotations(As[bf][mm][(TK / WK) * wk + 16 * i + (lane & 15)], b);
+#pragma unroll
+         for (int j = 0; j < BJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = mfma4(x[j], b[r], acc[i][j][r]);
       }
    };
 
-   int buf = 0;
+   /* chunk c+1 waits in registers while chunk c is multiplied and is moved
+    * to the other LDS buffer halfway through, chunk c+2 requested right
+    * after: one barrier per chunk */
    gload(0);
-   for (int m0 = 0; m0 < a.Mr; m0 += 16) {
+   lstore(0);
+   __syncthreads();
+   if (MC < a.Mr) gload(MC);
+   int buf = 0;
+   for (int m0 = 0; m0 < a.Mr; m0 += MC) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) As[buf][ar][ac + q] = ra[q];
-#pragma unroll
-      for (int q = 0; q < XQ; ++q) {
-         const int e = tid + 512 * q;
-         Xs[buf][e >> 4][e & 15] = rx[q];
+      for (int ks = 0; ks < MC / 8; ++ks) mma(buf, ks);
+      if (m0 + MC < a.Mr) {
+         lstore(buf ^ 1);
+         if (m0 + 2 * MC < a.Mr) gload(m0 + 2 * MC);
       }
+#pragma unroll
+      for (int ks = MC / 8; ks < MC / 4; ++ks) mma(buf, ks);
       __syncthreads();
-      if (m0 + 16 < a.Mr) gload(m0 + 16);
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-         const int mm = 4 * ks + (lane >> 4);
-         double b[BI], x[BJ];
-#pragma unroll
-         for (int i = 0; i < BI; ++i) b[i] = As[buf][mm][(TK / WK) * wk + 16 * i + (lane & 15)];
-#pragma unroll
-         for (int j = 0; j < BJ; ++j) x[j] = Xs[buf][64 * wn + 16 * j + (lane & 15)][mm];
-#pragma unroll
-         for (int i = 0; i < BI; ++i)
-#pragma unroll
-            for (int j = 0; j < BJ; ++j) acc[i][j] = mfma16x16x4(x[j], b[i], acc[i][j]);
-      }
       buf ^= 1;
    }
 
+   /* epilogue per 64-column half: the half's waves stage their tile in LDS,
+    * then all waves apply the update along whole 1 KB rows */
+   double (*T)[TK + 2] = reinterpret_cast<double (*)[TK + 2]>(lds);
 #pragma unroll
-   for (int i = 0; i < BI; ++i) {
-      const size_t k = k0 + (TK / WK) * wk + 16 * i + (lane & 15);
+   for (int half = 0; half < WN; ++half) {
+      if (half > 0) __syncthreads();
+      if (wn == half) {
 #pragma unroll
-      for (int j = 0; j < BJ; ++j)
+         for (int i = 0; i < BI; ++i)
 #pragma unroll
-         for (int r = 0; r < 4; ++r) {
-            const int n = n0 + 64 * wn + 16 * j + (lane >> 4) + 4 * r;
-            if (n < a.Nc) {
-               const size_t idx = (size_t)n * a.ldc + k;
-               if (EPI == EPI_KBAPPLY) {
-                  const double cp = a.CP[idx] + acc[i][j][r];
-                  a.C2[idx] = a.C1[idx] + a.dte * cp;
-                  if (a.write_cp) a.CP[idx] = cp;
-               } else {
-                  a.C2[idx] += a.dte * acc[i][j][r];
-               }
-            }
+            for (int j = 0; j < BJ; ++j)
+#pragma unroll
+               for (int r = 0; r < 4; ++r)
+                  T[16 * j + 4 * ab + q][(TK / WK) * wk + 16 * i + 4 * ((ab + s1 * r) & 3) + bb] =
+                     acc[i][j][r];
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (int nl = wave; nl < 64; nl += 8) {
+         const int n = n0 + 64 * half + nl;
+         if (n >= a.Nc) break;
+         const size_t idx = (size_t)n * a.ldc + k0 + 2 * lane;
+         const double2 v = make_double2(T[nl][2 * lane], T[nl][2 * lane + 1]);
+         if (EPI == EPI_KBAPPLY) {
+            double2 cp = *reinterpret_cast<const double2 *>(a.CP + idx);
+            const double2 c1 = *reinterpret_cast<const double2 *>(a.C1 + idx);
+            cp.x += v.x; cp.y += v.y;
+            *reinterpret_cast<double2 *>(a.C2 + idx) = make_double2(c1.x + a.dte * cp.x, c1.y + a.dte * cp.y);
+            if (a.write_cp) *reinterpret_cast<double2 *>(a.CP + idx) = cp;
+         } else {
+            double2 c2 = *reinterpret_cast<const double2 *>(a.C2 + idx);
+            c2.x += a.dte * v.x; c2.y += a.dte * v.y;
+            *reinterpret_cast<double2 *>(a.C2 + idx) = c2;
          }
+      }
    }
 }
 
-int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s)
+int launch_gemm_tall(int epi, const TallArgs &args, hipStream_t s)
 {
-   if (a.K % TK) { cpsd_set_error("gemm_tall: K=%zu not a multiple of %d", a.K, TK); return -1; }
-   /* one output-column tile covering up to 128 orbitals, so the A rows
-    * (projectors or C1) stream from HBM once */
-   const int NT = a.Nc > 64 ? 128 : 64;
-   dim3 grid(a.K / TK, (a.Nc + NT - 1) / NT);
-   if (NT == 128) {
-      if (epi == EPI_KBAPPLY) hipLaunchKernelGGL((k_gemm_tall<EPI_KBAPPLY, 128>), grid, dim3(512), 0, s, a);
-      else hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 128>), grid, dim3(512), 0, s, a);
-   } else {
-      if (epi == EPI_KBAPPLY) hipLaunchKernelGGL((k_gemm_tall<EPI_KBAPPLY, 64>), grid, dim3(512), 0, s, a);
-      else hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 64>), grid, dim3(512), 0, s, a);
-   }
+   if (args.K % TK) { cpsd_set_error("gemm_tall: K=%zu not a multiple of %d", args.K, TK); return -1; }
+   TallArgs a = args;
+   a.ntn = (a.Nc + 63) / 64;
+   const long blocks = (long)(a.K / TK) * a.ntn;
+   a.per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
+   if (epi == EPI_KBAPPLY) hipLaunchKernelGGL((k_gemm_tall<EPI_KBAPPLY, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
+   else hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
