@@ -234,7 +234,7 @@ GRID_CFG(40,  8,  5, 5, 4, 256, 16)
 GRID_CFG(48,  8,  6, 6, 4, 256, 16)
 GRID_CFG(64,  8,  8, 8, 4, 256, 32)
 GRID_CFG(80,  10, 8, 8, 5, 512, 32)
-GRID_CFG(96,  12, 8, 8, 6, 512, 32)
+GRID_CFG(96,  8, 12, 8, 6, 512, 32)
 GRID_CFG(128, 16, 8, 8, 8, 512, 32)
 #undef GRID_CFG
 

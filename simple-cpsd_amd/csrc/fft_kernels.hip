@@ -281,7 +281,7 @@ __device__ __forceinline__ void slot_xy(int s, int m, int &y, int &x0)
 /* ------------------------------------------------------- plane kernel */
 
 template<int N, int MODE>
-__global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS) k_plane(PlaneArgs a)
+__global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE == PM_DENSITY) ? 4 : 1) k_plane(PlaneArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, M = N / 2, NT = C::PLANE_THREADS;

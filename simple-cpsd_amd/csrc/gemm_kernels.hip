@@ -236,24 +236,41 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
 }
 
 /* out[m][n] = sum_z partial[z][m][n]; with symm only the tiles with
- * tile(m) <= tile(n) exist and the rest is mirrored */
-__global__ void k_reduce_partials(const double *__restrict__ partial, int splits, int M, int N,
-                                  int symm, double *__restrict__ out)
+ * tile(m) <= tile(n) exist and the rest is mirrored.  A block covers 64
+ * consecutive outputs with four thread groups: group g sums the splits
+ * z = g mod 4 in two interleaved chains, and the four group sums are added
+ * in g order, so the result does not depend on scheduling. */
+__global__ void __launch_bounds__(256) k_reduce_partials(const double *__restrict__ partial, int splits,
+                                                         int M, int N, int symm, double *__restrict__ out)
 {
-   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (idx >= (size_t)M * N) return;
-   int m = (int)(idx / N), n = (int)(idx % N);
-   if (symm && (m / TT) > (n / TT)) { const int t = m; m = n; n = t; }
-   double s = 0.0;
-   for (int z = 0; z < splits; ++z) s += partial[((size_t)z * M + m) * N + n];
-   out[idx] = s;
+   __shared__ double red[4][64];
+   const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
+   const size_t idx = (size_t)blockIdx.x * 64 + l;
+   const bool valid = idx < (size_t)M * N;
+   double s0 = 0.0, s1 = 0.0;
+   if (valid) {
+      int m = (int)(idx / N), n = (int)(idx % N);
+      if (symm && (m / TT) > (n / TT)) { const int t = m; m = n; n = t; }
+      const size_t stride = (size_t)M * N;
+      const double *p = partial + (size_t)m * N + n;
+      int z = g;
+#pragma unroll 4
+      for (; z + 4 < splits; z += 8) {
+         s0 += p[(size_t)z * stride];
+         s1 += p[(size_t)(z + 4) * stride];
+      }
+      if (z < splits) s0 += p[(size_t)z * stride];
+   }
+   red[g][l] = s0 + s1;
+   __syncthreads();
+   if (g == 0 && valid) out[idx] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
 }
 
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s)
 {
    const size_t tot = (size_t)M * N;
-   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 255) / 256), dim3(256), 0, s, partial,
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
                       splits, M, N, symm, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
