@@ -9,7 +9,8 @@
  *                the orbitals followed by one energy evaluation.  All
  *                arrays are the caller's host arrays with the reference's
  *                layouts; on return c1, c2, cpsi (= c2 - c1), dn, hml, lmd,
- *                E[0..9], *deltae, *deltac, *deltar, r1 hold what the
+ *                E[0..9], *deltae, *deltac, *deltar, r1 (and with move = 1
+ *                r2, fion: ccpsdv3/cpsd.c:246-364, 401-411) hold what the
  *                reference leaves there.  cpsdwork is not written (the
  *                scratch lives on the GPU).
  *   cr_fft3b_()  replaces fftpack3d/fftpack3d.f:22-96 (Fortran ABI, all
@@ -19,7 +20,7 @@
  *   rc_fft3f_()  replaces fftpack3d/fftpack3d.f:107-196: the forward
  *                transform, Im(kx=0) = Im(kx=N/2) = 0.
  *
- * Supported inputs: ispin = 1, move = 0, ne[1] = 0, nx = ny = nz = nfft
+ * Supported inputs: ispin = 1, move = 0 or 1, ne[1] = 0, nx = ny = nz = nfft
  * with nfft in {8,12,16,20,24,32,40,48,64,80,96,128}, the reference's
  * increments, orbitals that vanish outside the spherical cutoff mask (the
  * reference keeps them there).  Anything else is reported on stderr and,
@@ -75,6 +76,13 @@ int cpsd_dev_create(cpsd_dev **h, int nfft, int ne, int nion, const int *katm, c
                     double cewald, double rcut, const double *r);
 /* orbitals c[ne*n2ft3d] in the reference's ELCIN layout */
 int cpsd_dev_set_orbitals(cpsd_dev *h, const double *c);
+/* ions for the following runs: positions r2[3*nion] (the r2 that cpsd()
+ * takes); move = 1 also moves them every step with the forces of
+ * ccpsdv3/cpsd.c:246-364 and masses amass[nkind] (as passed to cpsd()) */
+int cpsd_dev_set_ions(cpsd_dev *h, int move, const double *amass, const double *r2);
+/* positions at the start (r1) and end (r2) of the last step and, after a
+ * moving-ion run, its forces fion[3*nion]; any pointer may be NULL */
+int cpsd_dev_get_ions(cpsd_dev *h, double *r1, double *r2, double *fion);
 /* inner SD steps + energy; E[0] holds the previous energy on entry
  * (as in the reference), E[0..9] on exit; deltas as in cpsd().         */
 int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E, double *deltae,

@@ -1,8 +1,8 @@
 /*
  * cpsd_oracle.c -- test infrastructure only (see oracle.h).
  *
- * Serial FP64 restatement of the reference SD step ccpsdv3/cpsd.c:33-698 and
- * the helpers it calls.  Sums are sequential loops; the reference's BLAS
+ * Serial FP64 restatement of the reference SD step ccpsdv3/cpsd.c:33-698
+ * (ispin = 1, fixed or moving ions) and the helpers it calls.  Sums are sequential loops; the reference's BLAS
  * calls (ddot/daxpy/dcopy/dgemm/idamax) are written out as plain loops.
  */
 #include <math.h>
@@ -218,9 +218,9 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
    size_t jr = 0, k;
    int itime, i, j, ii, jj, l;
 
-   (void)amass; (void)G; (void)fion; (void)r2;
-   if (ispin != 1 || move != 0 || ne[1] != 0) {
-      fprintf(stderr, "oracle_cpsd: only ispin=1, move=0 is restated\n");
+   double *dti = NULL;
+   if (ispin != 1 || ne[1] != 0) {
+      fprintf(stderr, "oracle_cpsd: only ispin=1 is restated\n");
       return 1;
    }
    if (nfft3d != (nfft / 2 + 1) * nfft * nfft || n2ft3d != (nfft + 2) * nfft * nfft) {
@@ -228,7 +228,11 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
       return 2;
    }
 
-   /* scratch carved from cpsdwork as in ccpsdv3/cpsd.c:147-165 */
+   /* ionic time steps first, then the scratch, as in ccpsdv3/cpsd.c:116-165 */
+   if (move) {
+      dti = cpsdwork + jr; jr += (size_t)nion;
+      for (ii = 0; ii < nion; ++ii) dti[ii] = dt / sqrt(amass[katm[ii]]);
+   }
    dng  = cpsdwork + jr; jr += n2;
    vall = cpsdwork + jr; jr += n2;
    vtmp = cpsdwork + jr; jr += n2;
@@ -294,6 +298,14 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
             vall[2*k]   += vtmp[2*k];
             vall[2*k+1] += vtmp[2*k+1];
          }
+         if (move) {
+            /* local force, :246-257 */
+            for (k = 0; k < (size_t)nfft3d; ++k)
+               xce[k] = dng[2*k+1] * vtmp[2*k] - dng[2*k] * vtmp[2*k+1];
+            fion[3*ii]   = oracle_gsdot(nfft, G, xce);
+            fion[3*ii+1] = oracle_gsdot(nfft, G + nfft3d, xce);
+            fion[3*ii+2] = oracle_gsdot(nfft, G + 2*(size_t)nfft3d, xce);
+         }
          for (l = 0; l < lmmax[ia]; ++l) {
             rex = vnl + (size_t)ia * 9 * nfft3d + (size_t)l * nfft3d;
             if (l == 0 || l >= 4) {
@@ -312,6 +324,58 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
                double sw1 = -oracle_gcdotc(nfft, vtmp, c1 + i*n2) * scal2
                             / vnlnrm[ia*9 + l];
                axpy(n2, sw1, vtmp, c2 + i*n2);
+               if (move) {
+                  /* nonlocal force, :291-304 */
+                  const double *rex1 = c1 + i * n2;
+                  for (k = 0; k < (size_t)nfft3d; ++k)
+                     xce[k] = rex1[2*k+1] * vtmp[2*k] - rex1[2*k] * vtmp[2*k+1];
+                  sw1 *= 2;   /* ispin == 1 */
+                  fion[3*ii]   -= 2 * sw1 * oracle_gsdot(nfft, G, xce);
+                  fion[3*ii+1] -= 2 * sw1 * oracle_gsdot(nfft, G + nfft3d, xce);
+                  fion[3*ii+2] -= 2 * sw1 * oracle_gsdot(nfft, G + 2*(size_t)nfft3d, xce);
+               }
+            }
+         }
+      }
+
+      /* Ewald forces, :310-364 */
+      if (move) {
+         const double CERFC = 1.128379167;
+         const double B1 = 0.0705230784, B2 = 0.0422820123, B3 = 0.0092705272;
+         const double B4 = 0.0001520143, B5 = 0.0002765672, B6 = 0.0000430638;
+         for (ii = 0; ii < nion; ++ii) {
+            int ia = katm[ii];
+            oracle_strfac(nfft, ex1 + (size_t)ii*(nfft+2), ex2 + (size_t)ii*2*nfft,
+                          ex3 + (size_t)ii*2*nfft, exi);
+            for (k = 0; k < n2; ++k) exi[k] *= masker[k];
+            /* reciprocal space */
+            for (k = 0; k < (size_t)nfft3d; ++k)
+               vtmp[k] = (exi[2*k] * s[2*k+1] - exi[2*k+1] * s[2*k]) * vg[k];
+            fion[3*ii]   += oracle_gsdot(nfft, G, vtmp) * zv[ia] * scal2;
+            fion[3*ii+1] += oracle_gsdot(nfft, G + nfft3d, vtmp) * zv[ia] * scal2;
+            fion[3*ii+2] += oracle_gsdot(nfft, G + 2*(size_t)nfft3d, vtmp) * zv[ia] * scal2;
+            /* real space, pairs over the nsh images */
+            for (jj = ii + 1; jj < nion; ++jj) {
+               double dx = r1[3*ii] - r1[3*jj];
+               double dy = r1[3*ii+1] - r1[3*jj+1];
+               double dz = r1[3*ii+2] - r1[3*jj+2];
+               double zz = zv[katm[ii]] * zv[katm[jj]];
+               double sw1 = 0.0, sw2 = 0.0, sw3 = 0.0;
+               for (l = 0; l < nsh; ++l) {
+                  double x = rcell[l] + dx, y = rcell[l+nsh] + dy, z = rcell[l+2*nsh] + dz;
+                  double r = sqrt(x*x + y*y + z*z);
+                  double w = r / rcut;
+                  double aa = 1.0 + w * (B1 + w * (B2 + w * (B3 + w * (B4 + w * (B5 + w * B6)))));
+                  double erfc, adiff;
+                  aa *= aa * aa * aa;
+                  erfc = 1.0 / (aa * aa * aa * aa);
+                  adiff = zz * (erfc + CERFC * w * exp(-w * w)) / (r * r * r);
+                  sw1 += x * adiff;
+                  sw2 += y * adiff;
+                  sw3 += z * adiff;
+               }
+               fion[3*ii]   += sw1; fion[3*ii+1] += sw2; fion[3*ii+2] += sw3;
+               fion[3*jj]   -= sw1; fion[3*jj+1] -= sw2; fion[3*jj+2] -= sw3;
             }
          }
       }
@@ -336,9 +400,12 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
             p[k] = -scal1 * p[k] * masker[k] + c2[i*n2 + k];
       }
 
-      /* steepest descent, :399-400 */
+      /* steepest descent, :399-411 */
       memcpy(c2, c1, nn * sizeof(double));
       axpy(nn, dte, cpsi, c2);
+      if (move)
+         for (ii = 0; ii < nion; ++ii)
+            for (j = 0; j < 3; ++j) r2[3*ii+j] = r1[3*ii+j] + dti[ii] * fion[3*ii+j];
 
       /* overlap matrices, :415-437 (note: s21 and s12 are filled from one
        * triangle each and mirrored, exactly as the reference does) */
@@ -492,6 +559,12 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
    }
    *deltac /= dte;
    *deltar = 0.0;
+   if (move)
+      for (ii = 0; ii < nion; ++ii) {
+         double a = sqrt(fion[3*ii] * fion[3*ii] + fion[3*ii+1] * fion[3*ii+1]
+                         + fion[3*ii+2] * fion[3*ii+2]);
+         if (a > *deltar) *deltar = a;
+      }
    return 0;
 }
 

@@ -11,9 +11,8 @@
  * sequential sums) and cites the reference file:line it follows.  Paths are
  * relative to the reference's Cpsd-mpi-bamboo/ directory.
  *
- * Scope: ispin = 1 (spin-restricted) and move = 0 (fixed ions), which is what
- * every workload of this project uses; other values are rejected with an
- * error message and return code.
+ * Scope: ispin = 1 (spin-restricted), fixed (move = 0) or moving (move = 1)
+ * ions; ispin = 2 is rejected with an error message and return code.
  */
 #ifndef CPSD_ORACLE_H
 #define CPSD_ORACLE_H

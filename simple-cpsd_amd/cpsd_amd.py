@@ -31,7 +31,8 @@ SUPPORTED_GRIDS = (8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128)
 
 # every symbol include/cpsd_abi.h declares
 ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_abort_on_error",
-               "cpsd_dev_create", "cpsd_dev_set_orbitals", "cpsd_dev_run", "cpsd_dev_get",
+               "cpsd_dev_create", "cpsd_dev_set_orbitals", "cpsd_dev_set_ions", "cpsd_dev_get_ions",
+               "cpsd_dev_run", "cpsd_dev_get",
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device")
@@ -63,6 +64,8 @@ def hip_lib():
                                       _dp, _dp, _dp, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_int,
                                       _dp, C.c_double, C.c_double, _dp]
         L.cpsd_dev_set_orbitals.argtypes = [C.c_void_p, _dp]
+        L.cpsd_dev_set_ions.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.cpsd_dev_get_ions.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cpsd_dev_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, _DPTR, _DPTR, _DPTR]
         L.cpsd_dev_get.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.cpsd_dev_lagrange_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DPTR]
@@ -283,10 +286,27 @@ class DeviceSD:
         _check(hip_lib().cpsd_dev_set_orbitals(self.h, np.ascontiguousarray(psi, dtype=np.float64)),
                "cpsd_dev_set_orbitals")
 
+    def set_ions(self, move, r2=None, amass=None):
+        """Positions (default: the cell's) and, with move=1, ion motion with
+        the cell's masses."""
+        c = self.cell
+        r2 = np.ascontiguousarray(c.r if r2 is None else r2, dtype=np.float64)
+        am = np.ascontiguousarray(c.amass if amass is None else amass, dtype=np.float64)
+        _check(hip_lib().cpsd_dev_set_ions(self.h, int(move), am, r2), "cpsd_dev_set_ions")
+
+    def get_ions(self):
+        n = 3 * self.cell.nion
+        r1, r2, f = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(hip_lib().cpsd_dev_get_ions(self.h, r1.ctypes.data_as(C.c_void_p),
+                                           r2.ctypes.data_as(C.c_void_p),
+                                           f.ctypes.data_as(C.c_void_p)), "cpsd_dev_get_ions")
+        return dict(r1=r1, r2=r2, fion=f)
+
     def run(self, inner, dt=5.8, fmass=400000.0):
         de, dc, dr = C.c_double(), C.c_double(), C.c_double()
         _check(hip_lib().cpsd_dev_run(self.h, inner, dt, fmass, self.E, C.byref(de), C.byref(dc),
                                       C.byref(dr)), "cpsd_dev_run")
+        self.deltar = dr.value
         return self.E.copy(), de.value, dc.value
 
     def get(self, c1=False, c2=False, cpsi=False, dn=False, hml=False, lmd=False):
@@ -345,7 +365,7 @@ class DeviceSD:
             pass
 
 
-def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0):
+def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0, move=0, r2=None):
     """Call the drop-in cpsd() symbol exactly as ccpsdv3/main.c:422-426 does.
     Returns dict of the arrays cpsd() writes."""
     c = cell
@@ -353,7 +373,7 @@ def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0):
     n2 = c.n2ft3d
     nea = np.array([ne, 0], dtype=np.int32)
     r1 = np.zeros(3 * c.nion)
-    r2 = c.r.copy()
+    r2 = (c.r if r2 is None else np.asarray(r2, dtype=np.float64)).copy()
     fion = np.zeros(3 * c.nion)
     c1 = np.zeros(ne * n2)
     c2 = np.ascontiguousarray(c2, dtype=np.float64).copy()
@@ -364,12 +384,12 @@ def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0):
     work = np.zeros(7 * n2 + c.nion * (5 * c.nfft + 3) + 7 * ne * ne)
     E = np.ascontiguousarray(E, dtype=np.float64).copy()
     de, dc, dr = C.c_double(), C.c_double(), C.c_double()
-    L.cpsd(inner, 0, c.nion, 1, nea, c.nfft, c.nfft3d, n2, c.unitg, c.omega, dt, fmass, c.katm,
+    L.cpsd(inner, move, c.nion, 1, nea, c.nfft, c.nfft3d, n2, c.unitg, c.omega, dt, fmass, c.katm,
            c.amass, c.zv, 27, c.rcell, c.cewald, c.rcut, c.masker, c.G, c.tg, c.vc, c.vg,
            c.lmmax.copy(), c.vnlnrm.copy(), c.vl, c.vnl, r1, r2, fion, c1, c2, cpsi, dn, hml, lmd,
            work, E, C.byref(de), C.byref(dc), C.byref(dr))
     return dict(c1=c1, c2=c2, cpsi=cpsi, dn=dn, hml=hml, lmd=lmd, E=E, deltae=de.value,
-                deltac=dc.value, deltar=dr.value, r1=r1)
+                deltac=dc.value, deltar=dr.value, r1=r1, r2=r2, fion=fion)
 
 
 def fft3d(direction, nfft, a):

@@ -59,10 +59,11 @@ static void fail_loudly(const char *where)
 /* ---------------------------------------------------------- context */
 
 enum { PH_ZG2R, PH_DENSITY, PH_DNG, PH_VTOT, PH_VPSI, PH_ZR2G, PH_KBPROJ, PH_KBAPPLY,
-       PH_GRAM, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_COUNT };
+       PH_GRAM, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_FORCE, PH_COUNT };
 static const char *ph_names[PH_COUNT] = {
    "zpass_g2r", "plane_density", "dng_r2c", "vtot_c2r", "plane_vpsi", "zpass_r2g_hpsi",
-   "kb_projection", "kb_apply_sd", "gram", "lagrange", "constraint", "energy_tail"};
+   "kb_projection", "kb_apply_sd", "gram", "lagrange", "constraint", "energy_tail",
+   "ion_forces"};
 
 template<typename T>
 static int dalloc(T **p, size_t n)
@@ -106,6 +107,14 @@ struct cpsd_dev {
    double *realbuf;       /* n2ft3d staging */
    double omega, cewald, rcut;
    int have_run;
+   /* moving ions (move = 1): positions r2 after the step, forces, packed
+    * Cartesian G, first projector of each ion, force partials; host copies
+    * of katm and the per-kind masses for dt/sqrt(M) */
+   int move;
+   double *r2d, *fion, *dti, *Gp, *fgpart, *fnlpart, *fnlsum;
+   int *poff, fchunk;
+   int *katm_h; double *amass_h;
+   double *Gp_h;
    /* profiling: event pairs recorded on the stream without synchronising;
     * they are resolved when the pool fills or the profile is read */
    int prof;
@@ -159,9 +168,13 @@ static void ctx_free(cpsd_dev *c)
                    c->s, c->vall, c->B, c->partial, c->Gm, c->P, c->Xkb, c->s11, c->s12, c->s21,
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
-                   c->fullbuf, c->realbuf};
+                   c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
+                   c->fnlpart, c->fnlsum, c->poff};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
+   free(c->katm_h);
+   free(c->amass_h);
+   free(c->Gp_h);
    if (c->pool) {
       for (int i = 0; i < 2 * c->pool_cap; ++i) (void)hipEventDestroy(c->pool[i]);
       free(c->pool);
@@ -212,6 +225,8 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
       nproj += lm;
    }
    c->nproj = nproj;
+   std::vector<int> poff(nion);
+   for (int i = 0, m = 0; i < nion; ++i) { poff[i] = m; m += lmmax[katm[i]]; }
 
    /* masked rows: each (ky,kz) row must hold one contiguous kx interval */
    std::vector<int> rs(n * n), rl(n * n), ro(n * n), pidx(c->nfft3d, -1);
@@ -309,6 +324,20 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    TRY(h2d(c->unitg, unitg, 9, st)); TRY(h2d(c->r1, r, 3 * (size_t)nion, st));
    TRY(h2d(c->rcell, rcell, 3 * (size_t)nsh, st));
    TRY(h2d(c->katm, katm, nion, st)); TRY(h2d(c->lmmax, lmmax, nkind, st));
+   TRY(dalloc(&c->poff, nion)); TRY(h2d(c->poff, poff.data(), nion, st));
+   c->katm_h = (int *)malloc((size_t)nion * sizeof(int));
+   c->Gp_h = (double *)calloc(3 * (size_t)ngp, sizeof(double));
+   if (!c->katm_h || !c->Gp_h) { cpsd_set_error("host allocation failed"); return -1; }
+   memcpy(c->katm_h, katm, (size_t)nion * sizeof(int));
+   /* Cartesian G of the packed entries, the expression of ccpsdv3/get_g.c:
+    * k1 b1 + k2 b2 + k3 b3 with ky, kz taken back to (-N/2, N/2] */
+   for (int p = 0; p < c->ng; ++p) {
+      const int k1 = kxv[p];
+      const int k2 = kyv[p] <= n / 2 ? kyv[p] : kyv[p] - n;
+      const int k3 = kzv[p] <= n / 2 ? kzv[p] : kzv[p] - n;
+      for (int d = 0; d < 3; ++d)
+         c->Gp_h[(size_t)d * ngp + p] = k1 * unitg[d] + k2 * unitg[3 + d] + k3 * unitg[6 + d];
+   }
 
    /* state and scratch */
    const size_t orb = (size_t)ne * ngp;
@@ -408,6 +437,16 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    double2 *C1 = c->Cbase + (size_t)c->cur * orb;          /* previous C2 */
    double2 *C2 = c->Cbase + (size_t)(c->cur ^ 1) * orb;
 
+   /* moving ions: r1 <- r2 (ccpsdv3/cpsd.c:174), then phase factors,
+    * structure factor, local potential and projectors at the new r1 */
+   if (c->move) {
+      CPSD_CHECK(hipMemcpyAsync(c->r1, c->r2d, 3 * (size_t)c->nion * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+      TRY(ph_begin(c, PH_FORCE));
+      TRY(run_structure(c));
+      TRY(ph_end(c, PH_FORCE));
+   }
+
    ZArgs z;
    PlaneArgs pa;
    const size_t plane = (size_t)n * n * c->nh;
@@ -441,6 +480,16 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    z.T = c->Tv; z.out = c->dng; z.ostride = ngp;
    TRY(launch_zpass_r2g(n, DST_DNG, z, 1, st));
    TRY(ph_end(c, PH_DNG));
+   if (c->move) {
+      TRY(ph_begin(c, PH_FORCE));
+      ForceArgs f;
+      f.n = n; f.ng = c->ng; f.ngp = ngp; f.nion = c->nion; f.nchunk = c->fchunk;
+      f.kx = c->kx; f.ky = c->ky; f.kz = c->kz; f.ex1 = c->ex1; f.ex2 = c->ex2; f.ex3 = c->ex3;
+      f.katm = c->katm; f.zv = c->zv; f.vl = c->vl; f.vg = c->vg; f.w = c->w; f.Gp = c->Gp;
+      f.dng = c->dng; f.s = c->s; f.scal2 = scal2; f.part = c->fgpart;
+      TRY(sd_force_gspace(f, st));
+      TRY(ph_end(c, PH_FORCE));
+   }
    /* 6 */
    TRY(ph_begin(c, PH_VTOT));
    zargs_base(c, z, 0);
@@ -476,6 +525,26 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
       TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
       TRY(ph_end(c, PH_KBPROJ));
+   }
+   if (c->move) {
+      TRY(ph_begin(c, PH_FORCE));
+      if (c->nproj > 0) {
+         /* Z_p = sum_i sw_pi c1_i, reduced against beta_p in the epilogue */
+         TallArgs t;
+         t.A = A2; t.lda = K; t.X = c->Xkb; t.xsm = c->nproj; t.ldx = 1;
+         t.Mr = ne; t.Nc = c->nproj; t.K = K;
+         t.C2 = nullptr; t.C1 = nullptr; t.CP = nullptr; t.ldc = K; t.dte = 0.0; t.write_cp = 0;
+         t.Bp = c->B; t.w = c->w; t.Gp = c->Gp; t.ngp = ngp; t.fpart = c->fnlpart;
+         TRY(launch_gemm_tall(EPI_NLFORCE, t, st));
+         TRY(sd_force_nlsum(c->fnlpart, (int)(K / 128), c->nproj, c->fnlsum, st));
+      }
+      ForceFinishArgs f;
+      f.nion = c->nion; f.nchunk = c->fchunk; f.nsh = c->nsh;
+      f.katm = c->katm; f.lmmax = c->lmmax; f.poff = c->poff;
+      f.zv = c->zv; f.rcell = c->rcell; f.r1 = c->r1; f.dti = c->dti; f.rcut = c->rcut;
+      f.gpart = c->fgpart; f.nlsum = c->fnlsum; f.fion = c->fion; f.r2 = c->r2d;
+      TRY(sd_force_finish(f, st));
+      TRY(ph_end(c, PH_FORCE));
    }
    /* 10 */
    TRY(ph_begin(c, PH_KBAPPLY));
@@ -564,7 +633,15 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
    const double dte = dt / sqrt(fmass);
    hipStream_t st = c->stream;
    TRY(h2d(c->E, E, 1, st));
-   TRY(run_structure(c));
+   if (c->move) {
+      /* dti = dt/sqrt(M) of each ion, ccpsdv3/cpsd.c:116-121 */
+      std::vector<double> dti(c->nion);
+      for (int i = 0; i < c->nion; ++i) dti[i] = dt / sqrt(c->amass_h[c->katm_h[i]]);
+      TRY(h2d(c->dti, dti.data(), c->nion, st));
+      CPSD_CHECK(hipStreamSynchronize(st));
+   } else {
+      TRY(run_structure(c));
+   }
    for (int it = 0; it < inner; ++it) TRY(sd_step(c, dte, it == inner - 1));
    TRY(energy_tail(c, inner, dt, dte));
    double out[12], info[2];
@@ -581,7 +658,57 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
    *deltae = out[10];
    *deltac = out[11];
    *deltar = 0.0;
+   if (c->move) {
+      /* largest |F_I|, ccpsdv3/cpsd.c:675-683 */
+      std::vector<double> f(3 * (size_t)c->nion);
+      CPSD_CHECK(hipMemcpy(f.data(), c->fion, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < c->nion; ++i) {
+         const double a = sqrt(f[3 * i] * f[3 * i] + f[3 * i + 1] * f[3 * i + 1] + f[3 * i + 2] * f[3 * i + 2]);
+         if (a > *deltar) *deltar = a;
+      }
+   }
    c->have_run = 1;
+   return 0;
+}
+
+static int ctx_set_ions(cpsd_dev *c, int move, const double *amass, const double *r2)
+{
+   hipStream_t st = c->stream;
+   if (move != 0 && move != 1) { cpsd_set_error("move=%d must be 0 or 1", move); return -1; }
+   if (!r2 || (move && !amass)) { cpsd_set_error("set_ions needs positions and, for move=1, masses"); return -1; }
+   const size_t n3i = 3 * (size_t)c->nion;
+   if (move && !c->r2d) {
+      c->fchunk = 64;
+      TRY(dalloc(&c->r2d, n3i)); TRY(dalloc(&c->fion, n3i)); TRY(dalloc(&c->dti, c->nion));
+      TRY(dalloc(&c->Gp, 3 * (size_t)c->ngp));
+      TRY(h2d(c->Gp, c->Gp_h, 3 * (size_t)c->ngp, st));
+      TRY(dalloc(&c->fgpart, (size_t)c->nion * c->fchunk * 3));
+      TRY(dalloc(&c->fnlpart, (size_t)(2 * (size_t)c->ngp / 128) * (c->nproj > 0 ? c->nproj : 1) * 3));
+      TRY(dalloc(&c->fnlsum, 3 * (size_t)(c->nproj > 0 ? c->nproj : 1)));
+      CPSD_CHECK(hipMemsetAsync(c->fion, 0, n3i * sizeof(double), st));
+   }
+   if (move) {
+      double *m = (double *)realloc(c->amass_h, (size_t)c->nkind * sizeof(double));
+      if (!m) { cpsd_set_error("host allocation failed"); return -1; }
+      c->amass_h = m;
+      for (int k = 0; k < c->nkind; ++k) {
+         if (!(amass[k] > 0.0)) { cpsd_set_error("amass[%d]=%g must be positive", k, amass[k]); return -1; }
+         c->amass_h[k] = amass[k];
+      }
+      TRY(h2d(c->r2d, r2, n3i, st));
+   }
+   c->move = move;
+   TRY(h2d(c->r1, r2, n3i, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   return 0;
+}
+
+static int ctx_get_ions(cpsd_dev *c, double *r1, double *r2, double *fion)
+{
+   const size_t n3i = 3 * (size_t)c->nion;
+   if (r1) CPSD_CHECK(hipMemcpy(r1, c->r1, n3i * sizeof(double), hipMemcpyDeviceToHost));
+   if (r2) CPSD_CHECK(hipMemcpy(r2, c->move ? c->r2d : c->r1, n3i * sizeof(double), hipMemcpyDeviceToHost));
+   if (fion && c->fion) CPSD_CHECK(hipMemcpy(fion, c->fion, n3i * sizeof(double), hipMemcpyDeviceToHost));
    return 0;
 }
 
@@ -664,6 +791,18 @@ extern "C" int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, dou
                             double *deltae, double *deltac, double *deltar)
 {
    return ctx_run(h, inner, dt, fmass, E, deltae, deltac, deltar);
+}
+
+extern "C" int cpsd_dev_set_ions(cpsd_dev *h, int move, const double *amass, const double *r2)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   return ctx_set_ions(h, move, amass, r2);
+}
+
+extern "C" int cpsd_dev_get_ions(cpsd_dev *h, double *r1, double *r2, double *fion)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   return ctx_get_ions(h, r1, r2, fion);
 }
 
 extern "C" int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn,
@@ -775,9 +914,9 @@ extern "C" void cpsd(const int inner, const int move, const int nion, const int 
                      double *c1, double *c2, double *cpsi, double *dn, double *hml, double *lmd,
                      double *cpsdwork, double *E, double *deltae, double *deltac, double *deltar)
 {
-   (void)amass; (void)G; (void)fion; (void)cpsdwork;
-   if (ispin != 1 || move != 0 || ne[1] != 0) {
-      cpsd_set_error("only ispin=1, move=0 (ne[1]=0) is supported (got ispin=%d move=%d ne[1]=%d)",
+   (void)G; (void)cpsdwork;
+   if (ispin != 1 || ne[1] != 0 || (move != 0 && move != 1)) {
+      cpsd_set_error("only ispin=1 (ne[1]=0), move=0 or 1 is supported (got ispin=%d move=%d ne[1]=%d)",
                      ispin, move, ne[1]);
       fail_loudly("cpsd");
       return;
@@ -795,9 +934,11 @@ extern "C" void cpsd(const int inner, const int move, const int nion, const int 
       fail_loudly("cpsd");
       return;
    }
-   if (ctx_set_orbitals(g_ctx, c2) != 0 ||
+   if ((move && ctx_set_ions(g_ctx, 1, amass, r2) != 0) ||
+       ctx_set_orbitals(g_ctx, c2) != 0 ||
        ctx_run(g_ctx, inner, dt, fmass, E, deltae, deltac, deltar) != 0 ||
-       ctx_get(g_ctx, c1, c2, cpsi, dn, hml, lmd) != 0) {
+       ctx_get(g_ctx, c1, c2, cpsi, dn, hml, lmd) != 0 ||
+       (move && ctx_get_ions(g_ctx, r1, r2, fion) != 0)) {
       fail_loudly("cpsd");
       return;
    }

@@ -95,14 +95,19 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s);
 
-enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1 };
+enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2 };
 struct TallArgs {
    const double *A;  size_t lda;   /* A rows (m < Mr), contiguous along k */
-   const double *X;  int ldx;      /* X[m + n*ldx] */
+   const double *X;  int ldx;      /* X[m*xsm + n*ldx] */
+   int xsm = 1;
    int Mr, Nc;                     /* reduction length, output columns */
    size_t K;                       /* output rows (multiple of 64) */
    double *C2; const double *C1; double *CP; size_t ldc;
    double dte; int write_cp;
+   /* EPI_NLFORCE: for output row n (a projector), sum over the block's G of
+    * w Gp (Z.im beta.re - Z.re beta.im) with beta = Bp[n][G]; one [Nc][3]
+    * record per K block in fpart */
+   const double2 *Bp; const double *w, *Gp; int ngp; double *fpart;
    int ntn, per;                   /* set by the launcher */
 };
 int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);

@@ -17,7 +17,9 @@
  *                      applied to the A operand.
  *   k_gemm_tall      : tall output [ncol][K] = X^T A with a short reduction
  *                      (projectors or orbitals), 64-column x 128-row output
- *                      tiles over 8 waves, SD epilogues fused.
+ *                      tiles over 8 waves, SD epilogues fused; the nonlocal
+ *                      force (ccpsdv3/cpsd.c:291-304) reduces its output
+ *                      against the projectors instead of storing it.
  * Instruction layout (measured on gfx950): with A lane l = A[l%16][l/16] and
  * B lane l = B[l/16][l%16], one instruction forms the four diagonal 4x4
  * blocks of the 16x16 product A(16x4) B(4x16); output lane o = 16q + 4a + b
@@ -254,7 +256,6 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
       const size_t stride = (size_t)M * N;
       const double *p = partial + (size_t)m * N + n;
       int z = g;
-#pragma unroll 4
       for (; z + 4 < splits; z += 8) {
          s0 += p[(size_t)z * stride];
          s1 += p[(size_t)(z + 4) * stride];
@@ -280,6 +281,14 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
 
 #define TK 128  /* output rows (K) per workgroup */
 #define MC 16   /* reduction step */
+
+/* sum over the 64 lanes of a wave, same tree in every wave */
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+   return v;
+}
 
 /* waves arranged WK (along K) x WN (along the output columns) = 8; at most
  * 128 VGPRs and 68 KB of LDS, so that two workgroups share a CU and one's
@@ -334,7 +343,7 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
          const int e = tid + 512 * u;
          const int n = e / MC, mm = e % MC;
          const int mg = m0 + mm, ng = n0 + n;
-         rx[u] = (mg < a.Mr && ng < a.Nc) ? a.X[mg + (size_t)ng * a.ldx] : 0.0;
+         rx[u] = (mg < a.Mr && ng < a.Nc) ? a.X[(size_t)mg * a.xsm + (size_t)ng * a.ldx] : 0.0;
       }
    };
    auto lstore = [&](int bf) {
@@ -403,7 +412,8 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
                      acc[i][j][r];
       }
       __syncthreads();
-#pragma unroll 2
+      constexpr int EU = EPI == EPI_NLFORCE ? 1 : 2;   /* the force reduction does not unroll */
+#pragma unroll EU
       for (int nl = wave; nl < 64; nl += 8) {
          const int n = n0 + 64 * half + nl;
          if (n >= a.Nc) break;
@@ -415,10 +425,23 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
             cp.x += v.x; cp.y += v.y;
             *reinterpret_cast<double2 *>(a.C2 + idx) = make_double2(c1.x + a.dte * cp.x, c1.y + a.dte * cp.y);
             if (a.write_cp) *reinterpret_cast<double2 *>(a.CP + idx) = cp;
-         } else {
+         } else if (EPI == EPI_CONSTRAINT) {
             double2 c2 = *reinterpret_cast<const double2 *>(a.C2 + idx);
             c2.x += a.dte * v.x; c2.y += a.dte * v.y;
             *reinterpret_cast<double2 *>(a.C2 + idx) = c2;
+         } else {
+            /* lane holds Z_n(G) for G = k0/2 + lane (ccpsdv3/cpsd.c:296-302 with
+             * the orbital sum moved inside) */
+            const size_t g = (k0 >> 1) + lane;
+            const double2 b = a.Bp[(size_t)n * a.ngp + g];
+            const double t = a.w[k0 + 2 * lane] * (v.y * b.x - v.x * b.y);
+            const double f0 = wave_sum(a.Gp[g] * t);
+            const double f1 = wave_sum(a.Gp[(size_t)a.ngp + g] * t);
+            const double f2 = wave_sum(a.Gp[2 * (size_t)a.ngp + g] * t);
+            if (lane == 0) {
+               double *o = a.fpart + ((size_t)(k0 / TK) * a.Nc + n) * 3;
+               o[0] = f0; o[1] = f1; o[2] = f2;
+            }
          }
       }
    }
@@ -432,7 +455,8 @@ int launch_gemm_tall(int epi, const TallArgs &args, hipStream_t s)
    const long blocks = (long)(a.K / TK) * a.ntn;
    a.per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
    if (epi == EPI_KBAPPLY) hipLaunchKernelGGL((k_gemm_tall<EPI_KBAPPLY, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
-   else hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
+   else if (epi == EPI_CONSTRAINT) hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
+   else hipLaunchKernelGGL((k_gemm_tall<EPI_NLFORCE, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

@@ -65,3 +65,28 @@ int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double sca
 int sd_lagrange(const LagArgs &a, hipStream_t s);
 int sd_energy(const EnergyArgs &a, hipStream_t s);
 int sd_hml(const double *M, int ne, double *hml, hipStream_t s);
+
+/* ---- ionic forces and ion update (force_kernels.hip) -------------------- */
+struct ForceArgs {
+   int n, ng, ngp, nion, nchunk;
+   const int *kx, *ky, *kz;          /* grid coordinates of packed G */
+   const double2 *ex1, *ex2, *ex3;   /* phase factors per ion */
+   const int *katm;
+   const double *zv, *vl, *vg, *w;   /* vl [kind][ngp]; w per real (2 per G) */
+   const double *Gp;                 /* [3][ngp] Cartesian G of packed entries */
+   const double2 *dng, *s;
+   double scal2;
+   double *part;                     /* [nion][nchunk][3] */
+};
+struct ForceFinishArgs {
+   int nion, nchunk, nsh;
+   const int *katm, *lmmax, *poff;   /* poff: first projector of each ion */
+   const double *zv, *rcell, *r1, *dti;
+   double rcut;
+   const double *gpart;              /* [nion][nchunk][3] from sd_force_gspace */
+   const double *nlsum;              /* [nproj][3] from sd_force_nlsum */
+   double *fion, *r2;
+};
+int sd_force_gspace(const ForceArgs &a, hipStream_t s);
+int sd_force_nlsum(const double *nlpart, int kblocks, int nproj, double *nlsum, hipStream_t s);
+int sd_force_finish(const ForceFinishArgs &a, hipStream_t s);

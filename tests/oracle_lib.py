@@ -199,9 +199,9 @@ class State:
         self.deltac = C.c_double(0.0)
         self.deltar = C.c_double(0.0)
 
-    def step(self, inner, dt=5.8, fmass=400000.0):
+    def step(self, inner, dt=5.8, fmass=400000.0, move=0):
         s = self.s
-        rc = lib().oracle_cpsd(inner, 0, s.nion, 1, self.ne, s.nfft, s.nfft3d, s.n2ft3d,
+        rc = lib().oracle_cpsd(inner, move, s.nion, 1, self.ne, s.nfft, s.nfft3d, s.n2ft3d,
                                s.unitg, s.omega, dt, fmass, s.katm, self.amass, s.zv, 27,
                                s.rcell, s.cewald, s.rcut, s.masker, s.G, s.tg, s.vc, s.vg,
                                self.lmmax, self.vnlnrm, self.vl, self.vnl, self.r1, self.r2,
