@@ -35,6 +35,20 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of each phase's kernel, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary of this workload (lines
+    'phase <name> <bytes>'), or {} when there is none."""
+    path = os.path.join(ROOT, "profiles", "sd_step_%s_hbm_traffic.md" % workload.replace("-", ""))
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            t = line.split()
+            if len(t) == 3 and t[0] == "phase":
+                out[t[1]] = float(t[2])
+    return out
+
+
 def algorithmic(phase, w, ng, nproj, ndnblk):
     """(kind, amount per SD step) of the algorithmic work of one phase:
     kind 'flop' for MFMA contractions (flops over the masked G vectors,
@@ -57,7 +71,7 @@ def algorithmic(phase, w, ng, nproj, ndnblk):
     if phase == "zpass_g2r":
         return "byte", ne * (packed + plane)
     if phase == "plane_density":
-        return "byte", ne * plane + ndnblk * n3 * 2 + 3 * n3
+        return "byte", ne * plane + ndnblk * n3
     if phase == "plane_vpsi":
         return "byte", ne * 2 * plane + n3
     if phase == "zpass_r2g_hpsi":
@@ -129,6 +143,7 @@ def run(args):
                             ms_per_step=round(per_step, 4),
                             launches_per_step=round(calls / args.steps, 3))
     roofline = None
+    traffic = measured_traffic(args.workload)
     for name in sorted(phases, key=lambda n: -phases[n]["ms_per_step"]):
         kind, amount = algorithmic(name, w, sizes["ng"], sizes["nproj"], ndnblk)
         if kind is None:
@@ -139,12 +154,12 @@ def run(args):
             ach = amount / sec / 1e12
             roofline = dict(bound="mfma", kernel=name, achieved=round(ach, 3),
                             peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
-                            frac=round(ach / FP64_MFMA_PEAK_TFLOPS, 4), traffic=None,
+                            frac=round(ach / FP64_MFMA_PEAK_TFLOPS, 4), traffic=traffic.get(name),
                             algorithmic_per_launch=per_launch)
         else:
             ach = amount / sec / 1e9
             roofline = dict(bound="hbm", kernel=name, achieved=round(ach, 1), peak=HBM_PEAK_GBS,
-                            unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                            unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic.get(name),
                             algorithmic_per_launch=per_launch)
         break
     for name in phases:

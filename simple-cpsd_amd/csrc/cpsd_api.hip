@@ -58,12 +58,14 @@ static void fail_loudly(const char *where)
 
 /* ---------------------------------------------------------- context */
 
-enum { PH_ZG2R, PH_DENSITY, PH_DNG, PH_VTOT, PH_VPSI, PH_ZR2G, PH_KBPROJ, PH_KBAPPLY,
-       PH_GRAM, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_FORCE, PH_COUNT };
+/* one phase per hot kernel where the bench reports a roofline for it;
+ * the split-K reductions are phases of their own */
+enum { PH_ZG2R, PH_DENSITY, PH_DNRED, PH_DNG, PH_VTOT, PH_VPSI, PH_ZR2G, PH_KBPROJ, PH_KBRED,
+       PH_KBAPPLY, PH_GRAM, PH_GRAMRED, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_FORCE, PH_COUNT };
 static const char *ph_names[PH_COUNT] = {
-   "zpass_g2r", "plane_density", "dng_r2c", "vtot_c2r", "plane_vpsi", "zpass_r2g_hpsi",
-   "kb_projection", "kb_apply_sd", "gram", "lagrange", "constraint", "energy_tail",
-   "ion_forces"};
+   "zpass_g2r", "plane_density", "dn_reduce_xc", "dng_r2c", "vtot_c2r", "plane_vpsi",
+   "zpass_r2g_hpsi", "kb_projection", "kb_reduce", "kb_apply_sd", "gram", "gram_reduce",
+   "lagrange", "constraint", "energy_tail", "ion_forces"};
 
 template<typename T>
 static int dalloc(T **p, size_t n)
@@ -468,9 +470,9 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(ph_end(c, PH_DENSITY));
    }
    /* 4 */
-   TRY(ph_begin(c, PH_DENSITY));
+   TRY(ph_begin(c, PH_DNRED));
    TRY(sd_dn_reduce_xc(c->dnpart, c->ndnblk, c->n3, scal2, c->dn, c->xcp, c->xce, st));
-   TRY(ph_end(c, PH_DENSITY));
+   TRY(ph_end(c, PH_DNRED));
    /* 5 */
    TRY(ph_begin(c, PH_DNG));
    memset(&pa, 0, sizeof pa);
@@ -522,9 +524,11 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       const int sp = splits_for(c->nproj, ne, K, 0);
       TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K,
                                 sp, 0, c->partial, st));
+      TRY(ph_end(c, PH_KBPROJ));
+      TRY(ph_begin(c, PH_KBRED));
       TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
       TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
-      TRY(ph_end(c, PH_KBPROJ));
+      TRY(ph_end(c, PH_KBRED));
    }
    if (c->move) {
       TRY(ph_begin(c, PH_FORCE));
@@ -563,10 +567,12 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       const int sp = splits_for(2 * ne, 2 * ne, K, 1);
       const double *Cb = reinterpret_cast<const double *>(c->Cbase);
       TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
+      TRY(ph_end(c, PH_GRAM));
+      TRY(ph_begin(c, PH_GRAMRED));
       TRY(launch_reduce_partials(c->partial, sp, 2 * ne, 2 * ne, c->Gm, 1, st));
       TRY(sd_smat(c->Gm, ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, st));
+      TRY(ph_end(c, PH_GRAMRED));
    }
-   TRY(ph_end(c, PH_GRAM));
    /* 12 */
    TRY(ph_begin(c, PH_LAGRANGE));
    {
