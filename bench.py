@@ -123,6 +123,9 @@ def run(args):
             if Eprev is not None and abs(Ec[0] - Eprev) < 1e-7:
                 done = True
                 break
+            if nsteps % 1000 == 0 and Eprev is not None:
+                print("converge: %d steps, E %.10f, dE %.3e" % (nsteps, Ec[0], Ec[0] - Eprev),
+                      file=sys.stderr, flush=True)
             last = None if Eprev is None else Ec[0] - Eprev
             Eprev = Ec[0]
         tconv = time.perf_counter() - t0
@@ -187,6 +190,9 @@ def run(args):
                    seconds_per_step=round(est["total"], 4),
                    parts={k: round(v, 5) for k, v in est.items()},
                    nproc=os.cpu_count())
+        if conv is not None and conv["converged"]:
+            # the CPU would need the same number of steps
+            cpu["time_to_converge_s_extrapolated"] = round(conv["steps"] * est["total"], 1)
 
     out = {
         "metric": "SD steps/s and time-to-|dE|<1e-7 Ha: mixed+learned vs FP64 GPU, 1 MI355X",
@@ -212,7 +218,7 @@ def run(args):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -222,7 +228,7 @@ def main():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default="h2o-32", choices=sorted(WORKLOADS))
     p.add_argument("--seed", type=int, default=0)
-    p.add_argument("--converge-steps", type=int, default=3000)
+    p.add_argument("--converge-steps", type=int, default=10000)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--cpu-sample", type=int, default=16)
     args = p.parse_args()
@@ -230,7 +236,20 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the SD step of one "
               "cell runs on one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    run(args)
+    # the library prints the reference's own notices (e.g. "ierr=10 ...", as
+    # ccpsdv3/cpsd.c:477 does) on C stdout; keep stdout for the one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        out = run(args)
+    finally:
+        sys.stdout.flush()
+        import ctypes
+        ctypes.CDLL(None).fflush(None)   # C stdio buffers go to the redirected fd
+        os.dup2(saved, 1)
+        os.close(saved)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -69,12 +69,14 @@ __device__ __forceinline__ void rotations(double b0, double b[4])
 #define KT 32   /* K step */
 #define TP (TT + 2)  /* row pitch of the epilogue staging tile */
 
-/* 8 waves, each a 64 x 32 block of the 128 x 128 tile.  One workgroup per
+/* 8 waves, each a (TMR/2) x 32 block of the TMR x 128 tile (TMR = 128, or 96
+ * when that pads the row count less, e.g. 192 projectors).  One workgroup per
  * (tile, K split); the grid is 1D over the valid pairs only.  Workgroups go
  * to the 8 XCDs round-robin by block id, so block b is given pair
  * (b % 8) * per + b / 8: the workgroups one XCD runs together hold
  * consecutive pairs, i.e. all tiles of the same K range, which then share
  * that XCD's L2 (per = pairs / 8; 0 falls back to b itself). */
+template<int TMR>
 __global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
                  const double *__restrict__ B, size_t ldb, int N,
@@ -109,14 +111,15 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    const int bsel = same ? 0 : 1;
    /* loader: row tid/4 of the tile, 8 consecutive k at 8*(tid%4) */
    const int lr = tid >> 2, lq = (tid & 3) * 8;
-   const int am = tm * TT + lr, bn = tn * TT + lr;
-   const bool aval = am < M, bval = bn < N && !same;
+   constexpr int BI = TMR / 32;   /* 16-row blocks per wave */
+   const int am = tm * TMR + lr, bn = tn * TT + lr;
+   const bool aval = lr < TMR && am < M, bval = bn < N && !same;
    const double *Ap = A + (size_t)(am < M ? am : 0) * lda;
    const double *Bp = B + (size_t)(bn < N ? bn : 0) * ldb;
 
-   double acc[4][2][4];   /* [row block][column block][rotation] */
+   double acc[BI][2][4];   /* [row block][column block][rotation] */
 #pragma unroll
-   for (int i = 0; i < 4; ++i)
+   for (int i = 0; i < BI; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -152,15 +155,15 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto mma = [&](int bf, int ks) {
       const int kk = 4 * ks + (lane >> 4);
       const double wk = Ws[bf][kk];
-      double a[4];
+      double a[BI];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = S[bf][0][64 * wm + 16 * i + (lane & 15)][kk] * wk;
+      for (int i = 0; i < BI; ++i) a[i] = S[bf][0][(TMR / 2) * wm + 16 * i + (lane & 15)][kk] * wk;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
          double b[4];
          rotations(S[bf][bsel][32 * wn + 16 * j + (lane & 15)][kk], b);
 #pragma unroll
-         for (int i = 0; i < 4; ++i)
+         for (int i = 0; i < BI; ++i)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[i][j][r] = mfma4(a[i], b[r], acc[i][j][r]);
       }
@@ -191,20 +194,20 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    {
       const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < BI; ++i)
 #pragma unroll
          for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-               T[64 * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
+               T[(TMR / 2) * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
                   acc[i][j][r];
    }
    __syncthreads();
    const int n = tn * TT + 2 * lane;
    if (n < N) {
 #pragma unroll 4
-      for (int ml = wave; ml < TT; ml += 8) {
-         const int m = tm * TT + ml;
+      for (int ml = wave; ml < TMR; ml += 8) {
+         const int m = tm * TMR + ml;
          if (m >= M) break;
          double *dst = partial + ((size_t)kz * M + m) * N + n;
          if (n + 1 < N && !(N & 1)) {
@@ -227,12 +230,20 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
    if (symm && M != N) { cpsd_set_error("gemm_tn: symmetric product needs M == N"); return -1; }
    size_t kchunk = (K + splits - 1) / splits;
    kchunk = (kchunk + KT - 1) / KT * KT;
-   const int ntm = (M + TT - 1) / TT, ntn = (N + TT - 1) / TT;
+   /* row tile: 128, or 96 when that leaves fewer padding rows (a symmetric
+    * product keeps square tiles) */
+   const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
+   const int tmr = (!symm && pad96 < pad128) ? 96 : TT;
+   const int ntm = (M + tmr - 1) / tmr, ntn = (N + TT - 1) / TT;
    const int ntile = symm ? ntm * (ntm + 1) / 2 : ntm * ntn;
    const long pairs = (long)ntile * splits;
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
-   hipLaunchKernelGGL(k_gemm_tn_splitk, dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B, ldb,
-                      N, w, K, kchunk, symm, ntm, ntile, per, partial);
+   if (tmr == 96)
+      hipLaunchKernelGGL((k_gemm_tn_splitk<96>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
+                         ldb, N, w, K, kchunk, symm, ntm, ntile, per, partial);
+   else
+      hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
+                         ldb, N, w, K, kchunk, symm, ntm, ntile, per, partial);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
