@@ -48,6 +48,9 @@ __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
    const int ncol = min(KXB, a.rows.nhc - kx0);
    const double2 *src = a.src + (size_t)b * a.sstride;
+   /* twiddles are read from LDS (the barrier in stage_rows publishes them) */
+   __shared__ double2 tws[N];
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
    stage_rows<N>(a.rows, ky, rt);
 
    /* every load of this thread is issued before any is consumed */
@@ -83,7 +86,7 @@ __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
       if (e < N * KXB) sm[e] = v[q];
    }
    __syncthreads();
-   lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, a.tw, 1);
+   lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, tws, 1);
    double2 *T = a.T + (size_t)b * N * N * NH;
    for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
       const int s = e / KXB, kxl = e % KXB;
@@ -105,6 +108,8 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
    const int ncol = min(KXB, a.rows.nhc - kx0);
    const double2 *T = a.T + (size_t)b * N * N * NH;
+   __shared__ double2 tws[N];
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
    stage_rows<N>(a.rows, ky, rt);
 
    {
@@ -126,7 +131,7 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
       }
    }
    __syncthreads();
-   lds_fft_tr2nat<C::NY1, C::NY2, -1>(sm, ncol, 1, KXB, a.tw, 1);
+   lds_fft_tr2nat<C::NY1, C::NY2, -1>(sm, ncol, 1, KXB, tws, 1);
    double2 *out = a.out + (size_t)b * a.ostride;
    int pe[EPT];
 #pragma unroll
@@ -288,6 +293,10 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
    extern __shared__ double2 sm[];
    const int z = blockIdx.x;
    const size_t plane = (size_t)N * NH;
+   /* twiddles live in LDS after the plane; every mode passes a barrier
+    * before its first transform */
+   double2 *tws = sm + plane;
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
 
    if constexpr (MODE == PM_DENSITY) {
       constexpr int ACC = (N * N + NT - 1) / NT;
@@ -298,7 +307,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
       const int bb1 = min(bb0 + a.nb, a.nbatch);
       for (int b = bb0; b < bb1; ++b) {
          load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
-         inverse_yx<N>(sm, a.tw);
+         inverse_yx<N>(sm, tws);
          const double *smr = reinterpret_cast<const double *>(sm);
 #pragma unroll
          for (int q = 0; q < ACC; ++q) {
@@ -326,7 +335,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
    } else if constexpr (MODE == PM_VTOT || MODE == PM_C2R_OUT) {
       const int b = blockIdx.y;
       load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
-      inverse_yx<N>(sm, a.tw);
+      inverse_yx<N>(sm, tws);
       for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
          const int s = it / M, m = it % M;
          int y, x0;
@@ -360,7 +369,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
          }
       }
       load_plane<N>(sm, Tp, a.nhload);
-      inverse_yx<N>(sm, a.tw);
+      inverse_yx<N>(sm, tws);
 #pragma unroll
       for (int q = 0; q < EPV; ++q) {
          const int it = threadIdx.x + NT * q;
@@ -373,7 +382,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
          }
       }
       __syncthreads();
-      forward_xy<N>(sm, a.tw);
+      forward_xy<N>(sm, tws);
       for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
    } else {  /* PM_R2C_DN, PM_R2C_IN */
       const int b = blockIdx.y;
@@ -393,7 +402,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
          sm[s * NH + m] = v;
       }
       __syncthreads();
-      forward_xy<N>(sm, a.tw);
+      forward_xy<N>(sm, tws);
       double2 *Tp = a.T + ((size_t)b * N + z) * plane;
       for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
    }
@@ -437,7 +446,7 @@ static int zr2g(int dst, const ZArgs &a, int nbatch, hipStream_t s)
 template<int N, int MODE>
 static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
 {
-   const size_t lds = (size_t)N * (N / 2 + 1) * sizeof(double2);
+   const size_t lds = ((size_t)N * (N / 2 + 1) + N) * sizeof(double2);   /* plane + twiddles */
    static bool attr_set = false;
    if (!attr_set) {
       CPSD_CHECK(hipFuncSetAttribute((const void *)k_plane<N, MODE>,
