@@ -219,23 +219,25 @@ __device__ __forceinline__ void lds_fft_tr2nat(double2 *buf, int count, int fs, 
 
 /* N = NY1*NY2 for the y and z passes, N/2 = NX1*NX2 for the real x pass */
 template<int N> struct GridCfg;
-#define GRID_CFG(n, y1, y2, x1, x2, pt, kxb)                                   \
+/* z pass: KXB kx columns per workgroup of ZT threads; whole kx rows up to
+ * N = 96 (no row is split between workgroups), two balanced halves at 128 */
+#define GRID_CFG(n, y1, y2, x1, x2, pt, kxb, zt)                               \
    template<> struct GridCfg<n> {                                              \
       static constexpr int NY1 = y1, NY2 = y2, NX1 = x1, NX2 = x2;             \
-      static constexpr int PLANE_THREADS = pt, KXB = kxb;                      \
+      static constexpr int PLANE_THREADS = pt, KXB = kxb, ZT = zt;             \
    };
-GRID_CFG(8,   4,  2, 2, 2, 256, 16)
-GRID_CFG(12,  4,  3, 3, 2, 256, 16)
-GRID_CFG(16,  4,  4, 4, 2, 256, 16)
-GRID_CFG(20,  5,  4, 5, 2, 256, 16)
-GRID_CFG(24,  6,  4, 4, 3, 256, 16)
-GRID_CFG(32,  8,  4, 4, 4, 256, 16)
-GRID_CFG(40,  8,  5, 5, 4, 256, 16)
-GRID_CFG(48,  8,  6, 6, 4, 256, 16)
-GRID_CFG(64,  8,  8, 8, 4, 256, 32)
-GRID_CFG(80,  10, 8, 8, 5, 512, 32)
-GRID_CFG(96,  8, 12, 8, 6, 512, 32)
-GRID_CFG(128, 16, 8, 8, 8, 512, 32)
+GRID_CFG(8,   4,  2, 2, 2, 256, 5,  256)
+GRID_CFG(12,  4,  3, 3, 2, 256, 7,  256)
+GRID_CFG(16,  4,  4, 4, 2, 256, 9,  256)
+GRID_CFG(20,  5,  4, 5, 2, 256, 11, 256)
+GRID_CFG(24,  6,  4, 4, 3, 256, 13, 256)
+GRID_CFG(32,  8,  4, 4, 4, 256, 17, 256)
+GRID_CFG(40,  8,  5, 5, 4, 256, 21, 256)
+GRID_CFG(48,  8,  6, 6, 4, 256, 25, 256)
+GRID_CFG(64,  8,  8, 8, 4, 256, 33, 256)
+GRID_CFG(80,  10, 8, 8, 5, 512, 41, 512)
+GRID_CFG(96,  8, 12, 8, 6, 512, 49, 512)
+GRID_CFG(128, 8, 16, 8, 8, 512, 33, 512)
 #undef GRID_CFG
 
 #define CPSD_FOR_EACH_GRID(X) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128)

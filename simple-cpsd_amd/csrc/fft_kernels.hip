@@ -38,11 +38,11 @@ __device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*r
 }
 
 template<int N, int SRC>
-__global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
+__global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_g2r(ZArgs a)
 {
    using C = GridCfg<N>;
-   constexpr int NH = N / 2 + 1, KXB = C::KXB;
-   constexpr int EPT = (N * KXB + 255) / 256;   /* elements per thread */
+   constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;
+   constexpr int EPT = (N * KXB + ZT - 1) / ZT;   /* elements per thread */
    __shared__ double2 sm[N * KXB];
    __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
    int pe[EPT];
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = threadIdx.x + ZT * q;
       const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
       pe[q] = -1;
       if (e < N * KXB && kxl < ncol) {
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
    }
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = threadIdx.x + ZT * q;
       if (e < N * KXB) sm[e] = v[q];
    }
    __syncthreads();
@@ -98,11 +98,11 @@ __global__ void __launch_bounds__(256) k_zpass_g2r(ZArgs a)
 }
 
 template<int N, int DST>
-__global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
+__global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
 {
    using C = GridCfg<N>;
-   constexpr int NH = N / 2 + 1, KXB = C::KXB;
-   constexpr int EPT = (N * KXB + 255) / 256;
+   constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;
+   constexpr int EPT = (N * KXB + ZT - 1) / ZT;
    __shared__ double2 sm[N * KXB];
    __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
       double2 v[EPT];
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
-         const int e = threadIdx.x + 256 * q;
+         const int e = threadIdx.x + ZT * q;
          const int s = e / KXB, kxl = e % KXB;
          v[q] = make_double2(0.0, 0.0);
          if (e < N * KXB && kxl < ncol) {
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
       }
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
-         const int e = threadIdx.x + 256 * q;
+         const int e = threadIdx.x + ZT * q;
          if (e < N * KXB) sm[e] = v[q];
       }
    }
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
    int pe[EPT];
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
-      const int e = threadIdx.x + 256 * q;
+      const int e = threadIdx.x + ZT * q;
       const int kz = e / KXB, kxl = e % KXB, kx = kx0 + kxl;
       pe[q] = -1;
       if (e < N * KXB && kxl < ncol) {
@@ -157,14 +157,14 @@ __global__ void __launch_bounds__(256) k_zpass_r2g(ZArgs a)
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
          if (pe[q] >= 0) {
-            const double2 v = sm[threadIdx.x + 256 * q];
+            const double2 v = sm[threadIdx.x + ZT * q];
             out[pe[q]] = make_double2(-a.scal1 * v.x - t[q] * c[q].x, -a.scal1 * v.y - t[q] * c[q].y);
          }
       }
    } else {
 #pragma unroll
       for (int q = 0; q < EPT; ++q)
-         if (pe[q] >= 0) out[pe[q]] = sm[threadIdx.x + 256 * q];
+         if (pe[q] >= 0) out[pe[q]] = sm[threadIdx.x + ZT * q];
    }
 }
 
@@ -425,8 +425,9 @@ static int zg2r(int src, const ZArgs &a, int nbatch, hipStream_t s)
 {
    constexpr int KXB = GridCfg<N>::KXB;
    dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
-   if (src == SRC_PACKED) hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED>), grid, dim3(256), 0, s, a);
-   else hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG>), grid, dim3(256), 0, s, a);
+   constexpr int ZT = GridCfg<N>::ZT;
+   if (src == SRC_PACKED) hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED>), grid, dim3(ZT), 0, s, a);
+   else hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG>), grid, dim3(ZT), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -436,9 +437,10 @@ static int zr2g(int dst, const ZArgs &a, int nbatch, hipStream_t s)
 {
    constexpr int KXB = GridCfg<N>::KXB;
    dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
-   if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI>), grid, dim3(256), 0, s, a);
-   else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG>), grid, dim3(256), 0, s, a);
-   else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL>), grid, dim3(256), 0, s, a);
+   constexpr int ZT = GridCfg<N>::ZT;
+   if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI>), grid, dim3(ZT), 0, s, a);
+   else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG>), grid, dim3(ZT), 0, s, a);
+   else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL>), grid, dim3(ZT), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
