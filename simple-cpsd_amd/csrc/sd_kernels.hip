@@ -481,6 +481,114 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
    }
 }
 
+/* ne <= NM: each workgroup keeps its constant panels (rows ti of s11 and
+ * s21, columns tj of s12) in LDS for all iterations; an iteration loads the
+ * X panels and, after the first barrier, the st1 panel, i.e. two round trips
+ * to L2 and two grid barriers per iteration (the general kernel above makes
+ * one round trip per 32-wide K chunk of each of its four products).
+ * Matrices are column-major: M(r, c) = M[r + c*ne]. */
+template<int TS, int NM>
+__global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
+{
+   constexpr int P = NM + 1;
+   extern __shared__ double lpan[];
+   double (*s11r)[P] = reinterpret_cast<double (*)[P]>(lpan);
+   double (*s21r)[P] = s11r + TS;
+   double (*s12c)[P] = s21r + TS;
+   double (*xc)[P] = s12c + TS;
+   double (*xr)[P] = xc + TS;
+   double (*stc)[P] = xr + TS;
+   __shared__ double red[256];
+   const int ne = a.ne, nt = (ne + TS - 1) / TS;
+   const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+   const int nwg = gridDim.x;
+   const size_t nn = (size_t)ne * ne;
+   /* R[r][k] = M(ti*TS + r, k) */
+   auto load_rows = [&](const double *M, double (*R)[P]) {
+      for (int e = threadIdx.x; e < TS * ne; e += 256) {
+         const int r = e % TS, k = e / TS, row = ti * TS + r;
+         R[r][k] = row < ne ? M[row + (size_t)k * ne] : 0.0;
+      }
+   };
+   /* Cp[c][k] = M(k, tj*TS + c) */
+   auto load_cols = [&](const double *M, double (*Cp)[P]) {
+      for (int e = threadIdx.x; e < TS * ne; e += 256) {
+         const int k = e % ne, c = e / ne, col = tj * TS + c;
+         Cp[c][k] = col < ne ? M[k + (size_t)col * ne] : 0.0;
+      }
+   };
+   constexpr int PER = TS * TS / 256;
+   auto dot = [&](double (*R)[P], double (*Cp)[P], int r, int c) {
+      double s0 = 0.0, s1 = 0.0;
+      int k = 0;
+      for (; k + 1 < ne; k += 2) { s0 += R[r][k] * Cp[c][k]; s1 += R[r][k + 1] * Cp[c][k + 1]; }
+      if (k < ne) s0 += R[r][k] * Cp[c][k];
+      return s0 + s1;
+   };
+   unsigned nbar = 0;
+   int it = 0;
+   double gmax = 0.0;
+   load_rows(a.s11, s11r);
+   load_rows(a.s21, s21r);
+   load_cols(a.s12, s12c);
+   /* X_0 = s22 */
+   for (int q = 0; q < PER; ++q) {
+      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+      if (r < ne && c < ne) a.X[r + (size_t)c * ne] = a.s22[r + (size_t)c * ne];
+   }
+   grid_barrier(a.bar, (++nbar) * nwg, a.err);
+   while (it < 30) {
+      const double *X = a.X + (size_t)(it & 1) * nn;
+      double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
+      ++it;
+      load_cols(X, xc);
+      load_rows(X, xr);
+      __syncthreads();
+      /* st1 = s11 X */
+      for (int q = 0; q < PER; ++q) {
+         const int o = threadIdx.x + q * 256, rl = o % TS, cl = o / TS;
+         const int r = ti * TS + rl, c = tj * TS + cl;
+         if (r < ne && c < ne) a.st1[r + (size_t)c * ne] = dot(s11r, xc, rl, cl);
+      }
+      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      load_cols(a.st1, stc);
+      __syncthreads();
+      /* X_new = s22 + s21 X + X s12 + X st1 */
+      double lmax = 0.0;
+      for (int q = 0; q < PER; ++q) {
+         const int o = threadIdx.x + q * 256, rl = o % TS, cl = o / TS;
+         const int r = ti * TS + rl, c = tj * TS + cl;
+         if (r < ne && c < ne) {
+            const size_t e = r + (size_t)c * ne;
+            const double v = a.s22[e] + (dot(s21r, xc, rl, cl) + dot(xr, s12c, rl, cl) + dot(xr, stc, rl, cl));
+            Xn[e] = v;
+            lmax = fmax(lmax, fabs(v - X[e]));
+         }
+      }
+      red[threadIdx.x] = lmax;
+      __syncthreads();
+      for (int s2 = 128; s2 > 0; s2 >>= 1) {
+         if ((int)threadIdx.x < s2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s2]);
+         __syncthreads();
+      }
+      if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
+      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      gmax = 0.0;
+      for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (gmax < 1e-15) break;
+      if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+   }
+   const double *Xf = a.X + (size_t)(it & 1) * nn;
+   for (int q = 0; q < PER; ++q) {
+      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
+      if (r < ne && c < ne) a.lmd[r + (size_t)c * ne] = Xf[r + (size_t)c * ne];
+   }
+   if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.info[0] = (double)it;
+      a.info[1] = gmax;
+   }
+}
+
 __global__ void k_zero_u32(unsigned *p, int n)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -691,7 +799,17 @@ int sd_lagrange(const LagArgs &a, hipStream_t s)
 {
    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.bar, 1);
    CPSD_CHECK(hipGetLastError());
-   if (a.ne <= 256) {
+   if (a.ne <= 128) {
+      const int nt = (a.ne + 15) / 16;
+      const int lds = 6 * 16 * (128 + 1) * (int)sizeof(double);
+      static bool attr = false;
+      if (!attr) {
+         CPSD_CHECK(hipFuncSetAttribute((const void *)k_lagrange_res<16, 128>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+         attr = true;
+      }
+      hipLaunchKernelGGL((k_lagrange_res<16, 128>), dim3(nt * nt), dim3(256), lds, s, a);
+   } else if (a.ne <= 256) {
       const int nt = (a.ne + 15) / 16;
       hipLaunchKernelGGL(k_lagrange<16>, dim3(nt * nt), dim3(256), 0, s, a);
    } else if (a.ne <= 512) {
