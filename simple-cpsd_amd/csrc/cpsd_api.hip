@@ -569,7 +569,8 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
       TRY(ph_end(c, PH_GRAM));
       TRY(ph_begin(c, PH_GRAMRED));
-      TRY(launch_reduce_partials(c->partial, sp, 2 * ne, 2 * ne, c->Gm, 1, st));
+      /* diagonal tiles hold their upper 16 x 16 blocks only */
+      TRY(launch_reduce_partials(c->partial, sp, 2 * ne, 2 * ne, c->Gm, 16, st));
       TRY(sd_smat(c->Gm, ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, st));
       TRY(ph_end(c, PH_GRAMRED));
    }

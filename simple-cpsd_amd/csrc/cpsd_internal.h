@@ -92,6 +92,7 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
                           int N, const double *w, size_t K, int splits, int symm,
                           double *partial, hipStream_t s);
 /* out[m*ldo + n] = sum_z partial[z][m][n] */
+/* symm: 0, or the block edge below whose diagonal the partials were not formed */
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s);
 

@@ -152,7 +152,37 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       }
       if (tid < KT / 2) { Ws[bf][2 * tid] = rw.x; Ws[bf][2 * tid + 1] = rw.y; }
    };
+   /* diagonal tile of a symmetric product: only its 36 16 x 16 blocks on or
+    * above the diagonal are formed.  Wave w takes part of the block-row pair
+    * (dp, 7 - dp), dp = w % 4: blocks 0-4 (w < 4) or 5-8 (w >= 4) of the
+    * pair's 9, so each SIMD (waves w, w+4) forms 9 blocks instead of 16.
+    * Block b is accumulated in acc[b / 2][b % 2]. */
+   const int dp = wave & 3, dh = wave >> 2, dnb = dh ? 4 : 5;
+   auto dblk = [&](int b, int &I, int &J) {
+      const int bp = 5 * dh + b;
+      if (bp < 8 - dp) { I = dp; J = dp + bp; }
+      else { I = 7 - dp; J = bp - 1; }
+   };
+   auto mma_diag = [&](int bf, int ks) {
+      const int kk = 4 * ks + (lane >> 4);
+      const double wk = Ws[bf][kk];
+      const double a0 = S[bf][0][16 * dp + (lane & 15)][kk] * wk;
+      const double a1 = S[bf][0][16 * (7 - dp) + (lane & 15)][kk] * wk;
+#pragma unroll
+      for (int b = 0; b < 5; ++b) {
+         if (b < dnb) {
+            int I, J;
+            dblk(b, I, J);
+            double bb[4];
+            rotations(S[bf][0][16 * J + (lane & 15)][kk], bb);
+            const double ao = (I == dp) ? a0 : a1;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[b >> 1][b & 1][r] = mfma4(ao, bb[r], acc[b >> 1][b & 1][r]);
+         }
+      }
+   };
    auto mma = [&](int bf, int ks) {
+      if (TMR == TT && same) { mma_diag(bf, ks); return; }
       const int kk = 4 * ks + (lane >> 4);
       const double wk = Ws[bf][kk];
       double a[BI];
@@ -193,14 +223,27 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    double (*T)[TP] = reinterpret_cast<double (*)[TP]>(&S[0][0][0][0]);
    {
       const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
+      if (TMR == TT && same) {
 #pragma unroll
-      for (int i = 0; i < BI; ++i)
+         for (int b = 0; b < 5; ++b) {
+            if (b < dnb) {
+               int I, J;
+               dblk(b, I, J);
 #pragma unroll
-         for (int j = 0; j < 2; ++j)
+               for (int r = 0; r < 4; ++r)
+                  T[16 * I + 4 * ab + q][16 * J + 4 * ((ab + s1 * r) & 3) + bb] = acc[b >> 1][b & 1][r];
+            }
+         }
+      } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-               T[(TMR / 2) * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
-                  acc[i][j][r];
+         for (int i = 0; i < BI; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+               for (int r = 0; r < 4; ++r)
+                  T[(TMR / 2) * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
+                     acc[i][j][r];
+      }
    }
    __syncthreads();
    const int n = tn * TT + 2 * lane;
@@ -209,6 +252,8 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       for (int ml = wave; ml < TMR; ml += 8) {
          const int m = tm * TMR + ml;
          if (m >= M) break;
+         /* a diagonal tile holds only its upper 16 x 16 blocks */
+         if (same && 2 * lane < 16 * (ml >> 4)) continue;
          double *dst = partial + ((size_t)kz * M + m) * N + n;
          if (n + 1 < N && !(N & 1)) {
             *reinterpret_cast<double2 *>(dst) = make_double2(T[ml][2 * lane], T[ml][2 * lane + 1]);
@@ -248,8 +293,8 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
    return 0;
 }
 
-/* out[m][n] = sum_z partial[z][m][n]; with symm only the tiles with
- * tile(m) <= tile(n) exist and the rest is mirrored.  A block covers 64
+/* out[m][n] = sum_z partial[z][m][n]; with symm > 0 only the blocks of
+ * edge symm with block(m) <= block(n) exist and the rest is mirrored.  A block covers 64
  * consecutive outputs with four thread groups: group g sums the splits
  * z = g mod 4 in two interleaved chains, and the four group sums are added
  * in g order, so the result does not depend on scheduling. */
@@ -259,11 +304,11 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
    __shared__ double red[4][64];
    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
    const size_t idx = (size_t)blockIdx.x * 64 + l;
-   const bool valid = idx < (size_t)M * N;
+   const int m = (int)(idx / N), n = (int)(idx % N);
+   /* symmetric: the blocks below the diagonal are written from their mirror */
+   const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm));
    double s0 = 0.0, s1 = 0.0;
    if (valid) {
-      int m = (int)(idx / N), n = (int)(idx % N);
-      if (symm && (m / TT) > (n / TT)) { const int t = m; m = n; n = t; }
       const size_t stride = (size_t)M * N;
       const double *p = partial + (size_t)m * N + n;
       int z = g;
@@ -275,7 +320,11 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
    }
    red[g][l] = s0 + s1;
    __syncthreads();
-   if (g == 0 && valid) out[idx] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+   if (g == 0 && valid) {
+      const double v = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+      out[idx] = v;
+      if (symm && (m / symm) < (n / symm)) out[(size_t)n * N + m] = v;
+   }
 }
 
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
