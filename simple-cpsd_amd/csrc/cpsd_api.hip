@@ -186,13 +186,11 @@ static void ctx_free(cpsd_dev *c)
    memset(c, 0, sizeof(*c));
 }
 
-int gemm_tn_tile(void);
+int gemm_tn_tiles(int M, int N, int symm);
 
 static int splits_for(int M, int N, size_t K, int symm)
 {
-   const int t = gemm_tn_tile();
-   const int tm = (M + t - 1) / t, tn = (N + t - 1) / t;
-   int tiles = symm ? tm * (tm + 1) / 2 : tm * tn;
+   const int tiles = gemm_tn_tiles(M, N, symm);
    int sp = 2048 / (tiles > 0 ? tiles : 1);
    const int maxsp = (int)(K / (16 * 8));
    if (sp > maxsp) sp = maxsp;

@@ -28,6 +28,7 @@
  * so four rotations cover a 16 x 16 tile.
  */
 #include "cpsd_internal.h"
+#include <type_traits>
 
 __device__ __forceinline__ double mfma4(double a, double b, double c)
 {
@@ -69,13 +70,37 @@ __device__ __forceinline__ void rotations(double b0, double b[4])
 #define KT 32   /* K step */
 #define TP (TT + 2)  /* row pitch of the epilogue staging tile */
 
+/* Diagonal-pair roles (see k_gemm_tn_splitk).  A diagonal tile's 36 upper
+ * 16 x 16 blocks are split between four waves: two take half of the upper
+ * right quadrant each (block rows 2h, 2h+1 x columns 4-7, 8 blocks), two the
+ * upper triangle of a diagonal quadrant (rows and columns o..o+3, o = 0 or
+ * 4, 10 blocks).  Role 1 is the quadrant half, role 2 the triangle; block b
+ * of a role multiplies A row block dg_ai and B column block dg_bj. */
+__device__ constexpr int dg_nblk(int dm) { return dm == 1 ? 8 : 10; }
+__device__ constexpr int dg_na(int dm) { return dm == 1 ? 2 : 4; }
+__device__ constexpr int dg_ai(int dm, int b)
+{
+   return dm == 1 ? b >> 2 : (b < 4 ? 0 : b < 7 ? 1 : b < 9 ? 2 : 3);
+}
+__device__ constexpr int dg_bj(int dm, int b)
+{
+   return dm == 1 ? b & 3 : (b < 4 ? b : b < 7 ? b - 3 : b < 9 ? b - 5 : 3);
+}
+
 /* 8 waves, each a (TMR/2) x 32 block of the TMR x 128 tile (TMR = 128, or 96
  * when that pads the row count less, e.g. 192 projectors).  One workgroup per
  * (tile, K split); the grid is 1D over the valid pairs only.  Workgroups go
  * to the 8 XCDs round-robin by block id, so block b is given pair
  * (b % 8) * per + b / 8: the workgroups one XCD runs together hold
  * consecutive pairs, i.e. all tiles of the same K range, which then share
- * that XCD's L2 (per = pairs / 8; 0 falls back to b itself). */
+ * that XCD's L2 (per = pairs / 8; 0 falls back to b itself).
+ *
+ * A symmetric product (A = B, M = N) forms the tiles above the diagonal and
+ * the diagonal tiles two at a time: a diagonal-pair workgroup loads the same
+ * 256 rows as the off-diagonal tile (2d, 2d+1) and forms the 2 x 36 blocks on
+ * or above the diagonals of tiles 2d and 2d+1 (waves 0-3 the first, 4-7 the
+ * second), 18 blocks per SIMD against 16 for an off-diagonal tile, so the
+ * workgroups of a launch take about the same time. */
 template<int TMR>
 __global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
@@ -87,11 +112,19 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
    const int t = g % ntile, kz = g / ntile;
    int tm, tn;
-   if (symm) {   /* t enumerates the upper triangle row by row */
-      tm = 0;
-      int rem = t;
-      while (rem >= ntm - tm) { rem -= ntm - tm; ++tm; }
-      tn = tm + rem;
+   bool dd = false;
+   if (symm) {
+      const int noff = ntm * (ntm - 1) / 2;
+      if (t < noff) {   /* tiles strictly above the diagonal, row by row */
+         tm = 0;
+         int rem = t;
+         while (rem >= ntm - 1 - tm) { rem -= ntm - 1 - tm; ++tm; }
+         tn = tm + 1 + rem;
+      } else {          /* diagonal pair (tm, tm + 1); tile ntm, if named, is empty */
+         tm = 2 * (t - noff);
+         tn = tm + 1;
+         dd = true;
+      }
    } else {
       tm = t % ntm;
       tn = t / ntm;
@@ -99,31 +132,37 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    const size_t kb = (size_t)kz * kchunk;
    const size_t ke = min(K, kb + kchunk);
    /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
-    * stages the output tile for the epilogue */
-   __shared__ double S[2][2][TT][KT + 1];
+    * stages the output tile for the epilogue.  Row pitch KT + 2: the 16 rows
+    * x 2 k of a half-wave's operand read fall in distinct bank pairs, and
+    * rows stay 16-byte aligned for the stores. */
+   __shared__ double S[2][2][TT][KT + 2];
    __shared__ double Ws[2][KT];
    static_assert(sizeof(S) >= sizeof(double) * TT * TP, "staging tile does not fit");
-   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   /* the wave index is uniform: read it as a scalar */
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wm = wave & 1, wn = wave >> 1;
    const int s1 = rot_step(lane);
-   /* a diagonal tile of a symmetric product reads one operand block */
-   const bool same = symm && tm == tn;
-   const int bsel = same ? 0 : 1;
    /* loader: row tid/4 of the tile, 8 consecutive k at 8*(tid%4) */
    const int lr = tid >> 2, lq = (tid & 3) * 8;
    constexpr int BI = TMR / 32;   /* 16-row blocks per wave */
+   constexpr int BA = BI > 4 ? BI : 4;
    const int am = tm * TMR + lr, bn = tn * TT + lr;
-   const bool aval = lr < TMR && am < M, bval = bn < N && !same;
+   const bool aval = lr < TMR && am < M, bval = bn < N;
    const double *Ap = A + (size_t)(am < M ? am : 0) * lda;
    const double *Bp = B + (size_t)(bn < N ? bn : 0) * ldb;
 
-   double acc[BI][2][4];   /* [row block][column block][rotation] */
+   double acc[BA][2][4];   /* [row block][column block][rotation] */
+   double accx[2][4];      /* blocks 8, 9 of a triangle role */
 #pragma unroll
-   for (int i = 0; i < BI; ++i)
+   for (int i = 0; i < BA; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
          for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+#pragma unroll
+   for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) accx[j][r] = 0.0;
 
    /* K tile k+1 waits in registers while tile k is multiplied; it is moved
     * to the other LDS buffer halfway through tile k's MFMA stream and tile
@@ -143,46 +182,36 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto lstore = [&](int bf) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-         S[bf][0][lr][lq + 2 * h] = ra[h].x;
-         S[bf][0][lr][lq + 2 * h + 1] = ra[h].y;
-         if (!same) {
-            S[bf][1][lr][lq + 2 * h] = rb[h].x;
-            S[bf][1][lr][lq + 2 * h + 1] = rb[h].y;
-         }
+         *reinterpret_cast<double2 *>(&S[bf][0][lr][lq + 2 * h]) = ra[h];
+         *reinterpret_cast<double2 *>(&S[bf][1][lr][lq + 2 * h]) = rb[h];
       }
       if (tid < KT / 2) { Ws[bf][2 * tid] = rw.x; Ws[bf][2 * tid + 1] = rw.y; }
    };
-   /* diagonal tile of a symmetric product: only its 36 16 x 16 blocks on or
-    * above the diagonal are formed.  Wave w takes part of the block-row pair
-    * (dp, 7 - dp), dp = w % 4: blocks 0-4 (w < 4) or 5-8 (w >= 4) of the
-    * pair's 9, so each SIMD (waves w, w+4) forms 9 blocks instead of 16.
-    * Block b is accumulated in acc[b / 2][b % 2]. */
-   const int dp = wave & 3, dh = wave >> 2, dnb = dh ? 4 : 5;
-   auto dblk = [&](int b, int &I, int &J) {
-      const int bp = 5 * dh + b;
-      if (bp < 8 - dp) { I = dp; J = dp + bp; }
-      else { I = 7 - dp; J = bp - 1; }
-   };
-   auto mma_diag = [&](int bf, int ks) {
+
+   /* diagonal pair: wave w works on tile half hs = w / 4 (operand S[.][hs]);
+    * waves 0, 1, 6, 7 take role 1 (quadrant half 2h, 2h+1 with h = w % 2),
+    * waves 2-5 role 2 (triangle o = 4 (w % 2)).  Block b is accumulated in
+    * acc[b / 2][b % 2] (b < 8) or accx[b - 8]. */
+   const int hs = wave >> 2;
+   const int dm = !dd ? 0 : (wave == 0 || wave == 1 || wave == 6 || wave == 7) ? 1 : 2;
+   const int drow0 = dm == 1 ? 2 * (wave & 1) : 4 * (wave & 1);   /* first A row block */
+   const int dcol0 = dm == 1 ? 4 : drow0;                         /* first B column block */
+   auto blk = [&](int b, int r) -> double & { return b < 8 ? acc[(b & 7) >> 1][b & 1][r] : accx[b & 1][r]; };
+   auto mma_diag = [&](auto role, int bf, int ks) {
+      constexpr int DM = decltype(role)::value;
       const int kk = 4 * ks + (lane >> 4);
       const double wk = Ws[bf][kk];
-      const double a0 = S[bf][0][16 * dp + (lane & 15)][kk] * wk;
-      const double a1 = S[bf][0][16 * (7 - dp) + (lane & 15)][kk] * wk;
+      double a[4], bv[4][4];
 #pragma unroll
-      for (int b = 0; b < 5; ++b) {
-         if (b < dnb) {
-            int I, J;
-            dblk(b, I, J);
-            double bb[4];
-            rotations(S[bf][0][16 * J + (lane & 15)][kk], bb);
-            const double ao = (I == dp) ? a0 : a1;
+      for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kk] * wk;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) acc[b >> 1][b & 1][r] = mfma4(ao, bb[r], acc[b >> 1][b & 1][r]);
-         }
-      }
+      for (int j = 0; j < 4; ++j) rotations(S[bf][hs][16 * (dcol0 + j) + (lane & 15)][kk], bv[j]);
+#pragma unroll
+      for (int b = 0; b < dg_nblk(DM); ++b)
+#pragma unroll
+         for (int r = 0; r < 4; ++r) blk(b, r) = mfma4(a[dg_ai(DM, b)], bv[dg_bj(DM, b)][r], blk(b, r));
    };
-   auto mma = [&](int bf, int ks) {
-      if (TMR == TT && same) { mma_diag(bf, ks); return; }
+   auto mma_full = [&](int bf, int ks) {
       const int kk = 4 * ks + (lane >> 4);
       const double wk = Ws[bf][kk];
       double a[BI];
@@ -191,7 +220,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
          double b[4];
-         rotations(S[bf][bsel][32 * wn + 16 * j + (lane & 15)][kk], b);
+         rotations(S[bf][1][32 * wn + 16 * j + (lane & 15)][kk], b);
 #pragma unroll
          for (int i = 0; i < BI; ++i)
 #pragma unroll
@@ -199,52 +228,85 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       }
    };
 
-   if (kb < ke) {
-      gload(kb);
-      lstore(0);
-   }
-   __syncthreads();
-   if (kb + KT < ke) gload(kb + KT);
-   int buf = 0;
-   for (size_t k0 = kb; k0 < ke; k0 += KT) {
-#pragma unroll
-      for (int ks = 0; ks < KT / 8; ++ks) mma(buf, ks);
-      if (k0 + KT < ke) {
-         lstore(buf ^ 1);
-         if (k0 + 2 * KT < ke) gload(k0 + 2 * KT);
+   /* the K loop, specialised per wave role so that no branch sits between
+    * the unrolled MFMA steps */
+   auto kloop = [&](auto role) {
+      constexpr int DM = decltype(role)::value;
+      if (kb < ke) {
+         gload(kb);
+         lstore(0);
       }
-#pragma unroll
-      for (int ks = KT / 8; ks < KT / 4; ++ks) mma(buf, ks);
       __syncthreads();
-      buf ^= 1;
-   }
+      if (kb + KT < ke) gload(kb + KT);
+      int buf = 0;
+      for (size_t k0 = kb; k0 < ke; k0 += KT) {
+#pragma unroll
+         for (int ks = 0; ks < KT / 8; ++ks) DM ? mma_diag(role, buf, ks) : mma_full(buf, ks);
+         if (k0 + KT < ke) {
+            lstore(buf ^ 1);
+            if (k0 + 2 * KT < ke) gload(k0 + 2 * KT);
+         }
+#pragma unroll
+         for (int ks = KT / 8; ks < KT / 4; ++ks) DM ? mma_diag(role, buf, ks) : mma_full(buf, ks);
+         __syncthreads();
+         buf ^= 1;
+      }
+   };
+   if (TMR != TT || dm == 0) kloop(std::integral_constant<int, 0>{});
+   else if (dm == 1) kloop(std::integral_constant<int, 1>{});
+   else kloop(std::integral_constant<int, 2>{});
 
    /* stage the tile in LDS, then write whole 1 KB rows of the partial */
    double (*T)[TP] = reinterpret_cast<double (*)[TP]>(&S[0][0][0][0]);
-   {
-      const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
-      if (TMR == TT && same) {
+   const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
+   auto put = [&](int m, int n, int ml) {
+      double *dst = partial + ((size_t)kz * M + m) * N + n;
+      if (n + 1 < N && !(N & 1)) {
+         *reinterpret_cast<double2 *>(dst) = make_double2(T[ml][2 * lane], T[ml][2 * lane + 1]);
+      } else {
+         dst[0] = T[ml][2 * lane];
+         if (n + 1 < N) dst[1] = T[ml][2 * lane + 1];
+      }
+   };
+   if (TMR == TT && dd) {
+      auto stage = [&](auto role) {
+         constexpr int DM = decltype(role)::value;
 #pragma unroll
-         for (int b = 0; b < 5; ++b) {
-            if (b < dnb) {
-               int I, J;
-               dblk(b, I, J);
+         for (int b = 0; b < dg_nblk(DM); ++b) {
+            const int I = drow0 + dg_ai(DM, b), J = dcol0 + dg_bj(DM, b);
 #pragma unroll
-               for (int r = 0; r < 4; ++r)
-                  T[16 * I + 4 * ab + q][16 * J + 4 * ((ab + s1 * r) & 3) + bb] = acc[b >> 1][b & 1][r];
+            for (int r = 0; r < 4; ++r) T[16 * I + 4 * ab + q][16 * J + 4 * ((ab + s1 * r) & 3) + bb] = blk(b, r);
+         }
+      };
+      /* one diagonal tile at a time; only its upper 16 x 16 blocks exist */
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+         if (hs == h) {
+            if (dm == 1) stage(std::integral_constant<int, 1>{});
+            else stage(std::integral_constant<int, 2>{});
+         }
+         __syncthreads();
+         const int t0 = (tm + h) * TT, n = t0 + 2 * lane;
+         if (n < N) {
+            for (int ml = wave; ml < TT; ml += 8) {
+               const int m = t0 + ml;
+               if (m >= M) break;
+               if (2 * lane < 16 * (ml >> 4)) continue;
+               put(m, n, ml);
             }
          }
-      } else {
-#pragma unroll
-         for (int i = 0; i < BI; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-               for (int r = 0; r < 4; ++r)
-                  T[(TMR / 2) * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
-                     acc[i][j][r];
+         __syncthreads();
       }
+      return;
    }
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+         for (int r = 0; r < 4; ++r)
+            T[(TMR / 2) * wm + 16 * i + 4 * ab + q][32 * wn + 16 * j + 4 * ((ab + s1 * r) & 3) + bb] =
+               acc[i][j][r];
    __syncthreads();
    const int n = tn * TT + 2 * lane;
    if (n < N) {
@@ -252,20 +314,24 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       for (int ml = wave; ml < TMR; ml += 8) {
          const int m = tm * TMR + ml;
          if (m >= M) break;
-         /* a diagonal tile holds only its upper 16 x 16 blocks */
-         if (same && 2 * lane < 16 * (ml >> 4)) continue;
-         double *dst = partial + ((size_t)kz * M + m) * N + n;
-         if (n + 1 < N && !(N & 1)) {
-            *reinterpret_cast<double2 *>(dst) = make_double2(T[ml][2 * lane], T[ml][2 * lane + 1]);
-         } else {
-            dst[0] = T[ml][2 * lane];
-            if (n + 1 < N) dst[1] = T[ml][2 * lane + 1];
-         }
+         put(m, n, ml);
       }
    }
 }
 
-int gemm_tn_tile(void) { return TT; }
+/* tiles of one K split: all of them, or for a symmetric product those above
+ * the diagonal plus the diagonal pairs */
+int gemm_tn_tiles(int M, int N, int symm)
+{
+   if (symm) {
+      const int ntm = (M + TT - 1) / TT;
+      return ntm * (ntm - 1) / 2 + (ntm + 1) / 2;
+   }
+   const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
+   const int tmr = pad96 < pad128 ? 96 : TT;
+   return ((M + tmr - 1) / tmr) * ((N + TT - 1) / TT);
+}
+
 
 int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, size_t ldb,
                           int N, const double *w, size_t K, int splits, int symm,
@@ -279,8 +345,8 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
     * product keeps square tiles) */
    const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
    const int tmr = (!symm && pad96 < pad128) ? 96 : TT;
-   const int ntm = (M + tmr - 1) / tmr, ntn = (N + TT - 1) / TT;
-   const int ntile = symm ? ntm * (ntm + 1) / 2 : ntm * ntn;
+   const int ntm = (M + tmr - 1) / tmr;
+   const int ntile = gemm_tn_tiles(M, N, symm);
    const long pairs = (long)ntile * splits;
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
    if (tmr == 96)
