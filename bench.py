@@ -109,6 +109,14 @@ def run(args):
     dev.profile(False)
     lag_it, lag_ad = dev.lagrange_info()
 
+    # the run epilogue once (orthonormality check, eigenpairs of hml, c1 = c2 V;
+    # outside the timed steps)
+    t0 = time.perf_counter()
+    ep = dev.epilogue()
+    epilogue = dict(seconds=round(time.perf_counter() - t0, 4),
+                    ortho_max_dev=float(np.max(np.abs(ep["ortho"] - np.eye(w["ne"])))),
+                    eig_max=float(ep["eig"][0]), eig_min=float(ep["eig"][-1]))
+
     # time to |dE| < 1e-7 Ha between energy evaluations, from the same start
     conv = None
     if args.converge_steps > 0:
@@ -213,6 +221,7 @@ def run(args):
                    "precision": "fp64", "start": "gaussian", "setup_s": round(t_setup, 2)},
         "time_to_converge": conv,
         "lagrange_iterations_last_step": lag_it,
+        "epilogue": epilogue,
         "E_last": float(E[0]),
         "phases": phases,
         "roofline": roofline,

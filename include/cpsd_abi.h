@@ -19,6 +19,10 @@
  *                rows of N+2 (pads set to zero).
  *   rc_fft3f_()  replaces fftpack3d/fftpack3d.f:107-196: the forward
  *                transform, Im(kx=0) = Im(kx=N/2) = 0.
+ *   cpsd_rotate() replaces the three calls of the run epilogue at
+ *                ccpsdv3/main.c:466-484 (esyev('V','U'), eigsrt, egemm):
+ *                hml <- its eigenvectors, eig <- eigenvalues (descending),
+ *                c1 = c2 hml, for ispin = 1.
  *
  * Supported inputs: ispin = 1, move = 0 or 1, ne[1] = 0, nx = ny = nz = nfft
  * with nfft in {8,12,16,20,24,32,40,48,64,80,96,128}, the reference's
@@ -57,6 +61,16 @@ void cr_fft3b_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *n
 void rc_fft3f_(double *A, int *inc2c, int *inc3c, int *inc2r, int *inc3r, int *nx, int *ny,
                int *nz, double *tmp, int *ntmp);
 
+/* ccpsdv3/main.c:466-484 (ispin = 1): esyev('V','U', ne, hml) + eigsrt
+ * (ccpsdv3/eigsrt.c:11-39) + egemm('N','N', n2ft3d, ne, ne, 1, c2, hml, 0,
+ * c1).  On exit hml holds the eigenvectors in columns and eig the
+ * eigenvalues in eigsrt's descending order; c1 = c2 hml.  Eigenvectors are
+ * unique up to sign (and rotation within degenerate eigenvalues), so c1
+ * matches the reference's up to the sign of each orbital.  Follows a cpsd()
+ * call with the same nfft and ne, whose device tables it reuses. */
+void cpsd_rotate(const int ispin, const int *ne, const int n2ft3d, double *hml, double *eig,
+                 const double *c2, double *c1);
+
 /* ---- error handling (no reference counterpart) ---------------------- */
 const char *cpsd_last_error(void);
 void cpsd_set_abort_on_error(int on);
@@ -90,6 +104,12 @@ int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E, dou
 /* any pointer may be NULL; layouts as in cpsd() */
 int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn, double *hml,
                  double *lmd);
+/* the run epilogue of ccpsdv3/main.c:451-484 on the resident state, after
+ * a run: ortho[ne*ne] = gcdotc(c1_i, c1_j) of the last step's starting
+ * orbitals (the orthonormality table of :456-464), eig[ne] and V[ne*ne] the
+ * eigenpairs of hml as cpsd_rotate() returns them, and the device's c1
+ * replaced by c2 V (read it with cpsd_dev_get); any output may be NULL */
+int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
 /* last step's Lagrange iteration count and final max|dX| */
 int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff);
 /* sizes: [0] ng (masked G vectors), [1] ngp, [2] nproj */
@@ -112,6 +132,11 @@ long cpsd_dev_debug_fetch(cpsd_dev *h, int what, double *out, long cap);
 int cpsd_fft3d_batch(int dir, int nfft, int nbatch, double *A);  /* dir +1: cr, -1: rc */
 int cpsd_lda_device(int n, const double *dn, double *xcp, double *xce);
 int cpsd_erfc_device(int n, const double *w, double *out);
+/* the host eigensolver of cpsd_rotate() and cpsd_dev_epilogue() (cyclic
+ * Jacobi, no GPU work): a[n*n] column-major symmetric (upper triangle read)
+ * <- eigenvectors in columns, eig[n] <- eigenvalues in eigsrt's descending
+ * order (ccpsdv3/eigsrt.c:11-39 applied to dsyev's ascending output) */
+int cpsd_syev_sorted(int n, double *a, double *eig);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,8 @@
  *                      reference reader misparses
  *   cpsdh_kbpp         ckbppv3/dkbppv3.c:335-444 (KB transform to the grid)
  *   cpsdh_makepsi      cmakepsi/dmakepsi.c:86-133 (plane-wave start)
+ *   cpsdh_write_elc, cpsdh_read_elc, cpsdh_write_ionout
+ *                      the restart files of ccpsdv3/main.c:115-140, 539-570
  * and, with no counterpart in the reference, a seeded Gaussian starting
  * state orthonormalised under the reference's G-space inner product
  * (ccpsdv3/gcdotc.c), used for the large workloads.
@@ -50,6 +52,21 @@ int cpsdh_random_orbitals(int nfft, int ne, const int *masker, const double *G, 
 
 /* G-space inner product of ccpsdv3/gcdotc.c */
 double cpsdh_gcdotc(int nfft, const double *a, const double *b);
+
+/* ELCIN/ELCOUT restart files (read at ccpsdv3/main.c:115-140, written at
+ * :539-547): int icube, int nfft, double unit, int ispin, int ne[2], then
+ * (ne[0]+ne[1])*n2ft3d doubles, in native byte order with no padding.
+ * cpsdh_read_elc reads the header and, if c is not NULL, the orbitals into
+ * c[cap].  Return 0 on success. */
+int cpsdh_write_elc(const char *path, int icube, int nfft, double unit, int ispin, const int *ne,
+                    const double *c);
+int cpsdh_read_elc(const char *path, int *icube, int *nfft, double *unit, int *ispin, int *ne,
+                   double *c, long cap);
+
+/* IONOUT (ccpsdv3/main.c:551-570): kinds with their psp names and lmax,
+ * then per ion its 1-based kind, r2 and the zeroed r1. */
+int cpsdh_write_ionout(const char *path, int nkatm, const char *const *fname, const int *lmax,
+                       int nion, const int *katm, const double *r2);
 
 #ifdef __cplusplus
 }

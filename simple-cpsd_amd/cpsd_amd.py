@@ -35,9 +35,11 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_run", "cpsd_dev_get",
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
-               "cpsd_lda_device", "cpsd_erfc_device")
+               "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
+               "cpsd_syev_sorted")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
-                "cpsdh_random_orbitals", "cpsdh_gcdotc")
+                "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
+                "cpsdh_write_ionout")
 
 
 class CpsdError(RuntimeError):
@@ -78,6 +80,10 @@ def hip_lib():
         L.cpsd_fft3d_batch.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         L.cpsd_lda_device.argtypes = [C.c_int, _dp, _dp, _dp]
         L.cpsd_erfc_device.argtypes = [C.c_int, _dp, _dp]
+        L.cpsd_dev_epilogue.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.cpsd_syev_sorted.argtypes = [C.c_int, _dp, _dp]
+        L.cpsd_rotate.restype = None
+        L.cpsd_rotate.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp, _dp]
         L.cpsd.restype = None
         L.cpsd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int, C.c_int, _dp,
                            C.c_double, C.c_double, C.c_double, _ip, _dp, _dp, C.c_int, _dp,
@@ -107,6 +113,11 @@ def host_lib():
         L.cpsdh_random_orbitals.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_ulonglong, C.c_double, _dp]
         L.cpsdh_gcdotc.argtypes = [C.c_int, _dp, _dp]
         L.cpsdh_gcdotc.restype = C.c_double
+        L.cpsdh_write_elc.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]
+        L.cpsdh_read_elc.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _DPTR,
+                                     C.POINTER(C.c_int), _ip, C.c_void_p, C.c_long]
+        L.cpsdh_write_ionout.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), _ip, C.c_int, _ip,
+                                         _dp]
         _host = L
     return _host
 
@@ -118,6 +129,45 @@ def _err():
 def _check(rc, what):
     if rc != 0:
         raise CpsdError("%s failed: %s" % (what, _err()))
+
+
+# ------------------------------------------------------------- restart files
+
+def write_elc(path, icube, nfft, unit, c, ne):
+    """ELCOUT/ELCIN file (ccpsdv3/main.c:539-547) for ispin = 1."""
+    nea = np.array([ne, 0], dtype=np.int32)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    if c.size != ne * (nfft + 2) * nfft * nfft:
+        raise CpsdError("write_elc: %d doubles for %d orbitals at %d^3" % (c.size, ne, nfft))
+    if host_lib().cpsdh_write_elc(os.fsencode(path), icube, nfft, unit, 1, nea, c) != 0:
+        raise CpsdError("cannot write %s" % path)
+
+
+def read_elc(path):
+    """Header and orbitals of an ELCIN/ELCOUT file (ccpsdv3/main.c:115-140)."""
+    L = host_lib()
+    ic, nf, isp = C.c_int(), C.c_int(), C.c_int()
+    un = C.c_double()
+    ne = np.zeros(2, dtype=np.int32)
+    if L.cpsdh_read_elc(os.fsencode(path), C.byref(ic), C.byref(nf), C.byref(un), C.byref(isp), ne,
+                        None, 0) != 0:
+        raise CpsdError("cannot read the header of %s" % path)
+    n = int(ne[0] + ne[1]) * (nf.value + 2) * nf.value * nf.value
+    c = np.zeros(n)
+    if L.cpsdh_read_elc(os.fsencode(path), C.byref(ic), C.byref(nf), C.byref(un), C.byref(isp), ne,
+                        c.ctypes.data_as(C.c_void_p), n) != 0:
+        raise CpsdError("cannot read the orbitals of %s" % path)
+    return dict(icube=ic.value, nfft=nf.value, unit=un.value, ispin=isp.value, ne=ne.tolist(), c=c)
+
+
+def write_ionout(path, fnames, lmax, katm, r2):
+    """IONOUT (ccpsdv3/main.c:551-570)."""
+    names = (C.c_char_p * len(fnames))(*[n.encode() for n in fnames])
+    lm = np.ascontiguousarray(lmax, dtype=np.int32)
+    ka = np.ascontiguousarray(katm, dtype=np.int32)
+    r = np.ascontiguousarray(r2, dtype=np.float64)
+    if host_lib().cpsdh_write_ionout(os.fsencode(path), len(fnames), names, lm, ka.size, ka, r) != 0:
+        raise CpsdError("cannot write %s" % path)
 
 
 # ------------------------------------------------------------------ inputs
@@ -324,6 +374,17 @@ class DeviceSD:
         _check(hip_lib().cpsd_dev_get(self.h, *arrs), "cpsd_dev_get")
         return out
 
+    def epilogue(self):
+        """The run epilogue (ccpsdv3/main.c:451-484) on the resident state:
+        orthonormality matrix of c1, eigenpairs of hml in eigsrt order, and
+        the device's c1 replaced by c2 V."""
+        ne = self.ne
+        ortho, eig, V = np.zeros(ne * ne), np.zeros(ne), np.zeros(ne * ne)
+        _check(hip_lib().cpsd_dev_epilogue(self.h, ortho.ctypes.data_as(C.c_void_p),
+                                           eig.ctypes.data_as(C.c_void_p),
+                                           V.ctypes.data_as(C.c_void_p)), "cpsd_dev_epilogue")
+        return dict(ortho=ortho.reshape(ne, ne), eig=eig, V=V)
+
     def lagrange_info(self):
         it, ad = C.c_int(), C.c_double()
         hip_lib().cpsd_dev_lagrange_info(self.h, C.byref(it), C.byref(ad))
@@ -390,6 +451,29 @@ def cpsd_dropin(cell, ne, inner, c2, E, dt=5.8, fmass=400000.0, move=0, r2=None)
            work, E, C.byref(de), C.byref(dc), C.byref(dr))
     return dict(c1=c1, c2=c2, cpsi=cpsi, dn=dn, hml=hml, lmd=lmd, E=E, deltae=de.value,
                 deltac=dc.value, deltar=dr.value, r1=r1, r2=r2, fion=fion)
+
+
+def rotate_dropin(ne, n2ft3d, hml, c2):
+    """The drop-in cpsd_rotate() (ccpsdv3/main.c:466-484) after a cpsd_dropin()
+    call: returns the eigenvectors (hml layout), eigenvalues and c1 = c2 V."""
+    h = np.ascontiguousarray(hml, dtype=np.float64).copy()
+    eig = np.zeros(ne)
+    c1 = np.zeros(ne * n2ft3d)
+    nea = np.array([ne, 0], dtype=np.int32)
+    hip_lib().cpsd_rotate(1, nea, n2ft3d, h, eig, np.ascontiguousarray(c2, dtype=np.float64), c1)
+    return dict(V=h, eig=eig, c1=c1)
+
+
+def syev_sorted(a):
+    """Eigenpairs of a symmetric matrix (upper triangle of the column-major
+    array read) in eigsrt order: (eigenvalues, eigenvectors as columns of a
+    column-major n*n array)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = int(round(np.sqrt(a.size)))
+    v = a.reshape(-1).copy()
+    eig = np.zeros(n)
+    _check(hip_lib().cpsd_syev_sorted(n, v, eig), "cpsd_syev_sorted")
+    return eig, v
 
 
 def fft3d(direction, nfft, a):

@@ -768,6 +768,43 @@ static int ctx_get(cpsd_dev *c, double *c1, double *c2, double *cpsi, double *dn
    return 0;
 }
 
+/* The run epilogue of ccpsdv3/main.c:451-484 on the resident state:
+ * ortho = gcdotc(c1_i, c1_j) of the last step's starting orbitals (:456-464),
+ * the eigenpairs of hml in eigsrt order (:466-476), and c1 <- c2 V on the
+ * device (:478-483).  c2 is left as it is. */
+static int ctx_epilogue(cpsd_dev *c, double *ortho, double *eig, double *V)
+{
+   if (!c->have_run) { cpsd_set_error("epilogue: no SD step has run on this context"); return -1; }
+   const int ne = c->ne;
+   const size_t K = 2 * (size_t)c->ngp, orb = (size_t)ne * c->ngp, ee = (size_t)ne * ne;
+   hipStream_t st = c->stream;
+   double2 *C1 = c->Cbase + (size_t)(c->cur ^ 1) * orb;
+   const double2 *C2 = c->Cbase + (size_t)c->cur * orb;
+   if (ortho) {
+      const int sp = splits_for(ne, ne, K, 1);
+      if ((size_t)sp * ee > c->partial_cap) { cpsd_set_error("epilogue: split-K buffer too small"); return -1; }
+      const double *c1r = reinterpret_cast<const double *>(C1);
+      TRY(launch_gemm_tn_splitk(c1r, K, ne, c1r, K, ne, c->w, K, sp, 1, c->partial, st));
+      TRY(launch_reduce_partials(c->partial, sp, ne, ne, c->Gm, 16, st));
+      CPSD_CHECK(hipMemcpyAsync(ortho, c->Gm, ee * sizeof(double), hipMemcpyDeviceToHost, st));
+   }
+   std::vector<double> a(ee), w(ne);
+   CPSD_CHECK(hipMemcpyAsync(a.data(), c->hml, ee * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   TRY(cpsd_syev_sorted(ne, a.data(), w.data()));
+   /* Mh (the energy tail's scratch) takes V: X[m][n] = V[m + n ne] */
+   CPSD_CHECK(hipMemcpyAsync(c->Mh, a.data(), ee * sizeof(double), hipMemcpyHostToDevice, st));
+   TallArgs t;
+   t.A = reinterpret_cast<const double *>(C2); t.lda = K;
+   t.X = c->Mh; t.ldx = ne; t.xsm = 1; t.Mr = ne; t.Nc = ne; t.K = K;
+   t.C2 = reinterpret_cast<double *>(C1); t.ldc = K;
+   TRY(launch_gemm_tall(EPI_STORE, t, st));
+   CPSD_CHECK(hipStreamSynchronize(st));
+   if (eig) memcpy(eig, w.data(), (size_t)ne * sizeof(double));
+   if (V) memcpy(V, a.data(), ee * sizeof(double));
+   return 0;
+}
+
 /* ---------------------------------------------------- public C-ABI */
 
 extern "C" int cpsd_dev_create(cpsd_dev **h, int nfft, int ne, int nion, const int *katm,
@@ -814,6 +851,12 @@ extern "C" int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, d
                             double *hml, double *lmd)
 {
    return ctx_get(h, c1, c2, cpsi, dn, hml, lmd);
+}
+
+extern "C" int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   return ctx_epilogue(h, ortho, eig, V);
 }
 
 extern "C" int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff)
@@ -947,6 +990,45 @@ extern "C" void cpsd(const int inner, const int move, const int nion, const int 
       fail_loudly("cpsd");
       return;
    }
+}
+
+/* ccpsdv3/main.c:466-484 for ispin = 1 with host arrays: esyev('V','U') and
+ * eigsrt of hml (hml <- eigenvectors in columns, eig <- eigenvalues,
+ * descending), then c1 = c2 hml over the n2ft3d reals of each orbital.  It
+ * reuses the device tables of the preceding cpsd() call (same nfft and ne),
+ * as the reference's epilogue follows its SD loop; c2 is uploaded again, so
+ * the host arrays are the ones rotated. */
+extern "C" void cpsd_rotate(const int ispin, const int *ne, const int n2ft3d, double *hml, double *eig,
+                            const double *c2, double *c1)
+{
+   if (ispin != 1 || ne[1] != 0) {
+      cpsd_set_error("cpsd_rotate: only ispin=1 (ne[1]=0) is supported (got ispin=%d ne[1]=%d)", ispin, ne[1]);
+      fail_loudly("cpsd_rotate");
+      return;
+   }
+   if (!g_ctx || g_ctx->ne != ne[0] || g_ctx->n2ft3d != n2ft3d) {
+      cpsd_set_error("cpsd_rotate: needs a preceding cpsd() call with ne=%d and n2ft3d=%d", ne[0], n2ft3d);
+      fail_loudly("cpsd_rotate");
+      return;
+   }
+   cpsd_dev *c = g_ctx;
+   const size_t ee = (size_t)c->ne * c->ne;
+   if (ctx_set_orbitals(c, c2) != 0) { fail_loudly("cpsd_rotate"); return; }
+   std::vector<double> a(hml, hml + ee), w(c->ne);
+   if (cpsd_syev_sorted(c->ne, a.data(), w.data()) != 0) { fail_loudly("cpsd_rotate"); return; }
+   const size_t K = 2 * (size_t)c->ngp, orb = (size_t)c->ne * c->ngp;
+   hipStream_t st = c->stream;
+   TallArgs t;
+   t.A = reinterpret_cast<const double *>(c->Cbase + (size_t)c->cur * orb); t.lda = K;
+   t.X = c->Mh; t.ldx = c->ne; t.xsm = 1; t.Mr = c->ne; t.Nc = c->ne; t.K = K;
+   t.C2 = reinterpret_cast<double *>(c->Cbase + (size_t)(c->cur ^ 1) * orb); t.ldc = K;
+   if (hipMemcpyAsync(c->Mh, a.data(), ee * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+       launch_gemm_tall(EPI_STORE, t, st) != 0 || ctx_get(c, c1, nullptr, nullptr, nullptr, nullptr, nullptr) != 0) {
+      fail_loudly("cpsd_rotate");
+      return;
+   }
+   memcpy(hml, a.data(), ee * sizeof(double));
+   memcpy(eig, w.data(), (size_t)c->ne * sizeof(double));
 }
 
 /* ---------------------------------------- FFT entry points (Fortran) */

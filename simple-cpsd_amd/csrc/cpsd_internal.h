@@ -86,8 +86,9 @@ int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s
 /* ---- GEMM ------------------------------------------------------------- */
 
 /* partial[z][m][n] = sum_{k in chunk z} w[k] A[m][k] B[n][k]
- * (A: M rows of stride lda, B: N rows of stride ldb; K multiple of 16).
- * symm: only tiles with tile_m <= tile_n are computed (A == B).           */
+ * (A: M rows of stride lda, B: N rows of stride ldb; K multiple of 32).
+ * symm (A == B, M == N): only the 16 x 16 blocks on or above the diagonal
+ * are formed.                                                             */
 int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, size_t ldb,
                           int N, const double *w, size_t K, int splits, int symm,
                           double *partial, hipStream_t s);
@@ -96,7 +97,8 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s);
 
-enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2 };
+/* EPI_STORE: C2[n][k] = result (the rotation c1 = c2 V of the epilogue) */
+enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2, EPI_STORE = 3 };
 struct TallArgs {
    const double *A;  size_t lda;   /* A rows (m < Mr), contiguous along k */
    const double *X;  int ldx;      /* X[m*xsm + n*ldx] */
@@ -112,6 +114,12 @@ struct TallArgs {
    int ntn, per;                   /* set by the launcher */
 };
 int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);
+
+/* ---- epilogue (host) ---------------------------------------------------- */
+/* eigenpairs of the symmetric n x n column-major matrix a (upper triangle
+ * read): a <- eigenvectors in columns, eig <- eigenvalues, both in the order
+ * dsyev + eigsrt leave them (descending) */
+extern "C" int cpsd_syev_sorted(int n, double *a, double *eig);
 
 /* ---- SD kernels --------------------------------------------------------- */
 int sd_kernels_probe(void);

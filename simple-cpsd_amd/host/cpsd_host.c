@@ -380,3 +380,56 @@ int cpsdh_random_orbitals(int nfft, int ne, const int *masker, const double *G, 
    free(gl); free(wt); free(cp);
    return 0;
 }
+
+/* ------------------------------------------------ restart files */
+
+/* ELCIN / ELCOUT: the fields one after the other, as ccpsdv3/main.c:115-140
+ * reads and :539-547 writes them with fread/fwrite */
+int cpsdh_write_elc(const char *path, int icube, int nfft, double unit, int ispin, const int *ne,
+                    const double *c)
+{
+   FILE *f = fopen(path, "wb");
+   if (!f) return -1;
+   const size_t n = (size_t)(ne[0] + ne[1]) * (size_t)(nfft + 2) * nfft * nfft;
+   int ok = fwrite(&icube, sizeof(int), 1, f) == 1 && fwrite(&nfft, sizeof(int), 1, f) == 1 &&
+            fwrite(&unit, sizeof(double), 1, f) == 1 && fwrite(&ispin, sizeof(int), 1, f) == 1 &&
+            fwrite(ne, sizeof(int), 2, f) == 2 && fwrite(c, sizeof(double), n, f) == n;
+   if (fclose(f) != 0) ok = 0;
+   return ok ? 0 : -1;
+}
+
+int cpsdh_read_elc(const char *path, int *icube, int *nfft, double *unit, int *ispin, int *ne,
+                   double *c, long cap)
+{
+   FILE *f = fopen(path, "rb");
+   if (!f) return -1;
+   int ok = fread(icube, sizeof(int), 1, f) == 1 && fread(nfft, sizeof(int), 1, f) == 1 &&
+            fread(unit, sizeof(double), 1, f) == 1 && fread(ispin, sizeof(int), 1, f) == 1 &&
+            fread(ne, sizeof(int), 2, f) == 2;
+   if (ok && (*nfft < 1 || ne[0] < 0 || ne[1] < 0)) ok = 0;
+   if (ok && c) {
+      const size_t n = (size_t)(ne[0] + ne[1]) * (size_t)(*nfft + 2) * (*nfft) * (*nfft);
+      ok = (long)n <= cap && fread(c, sizeof(double), n, f) == n;
+   }
+   fclose(f);
+   return ok ? 0 : -1;
+}
+
+/* IONOUT, ccpsdv3/main.c:551-570.  The reference zeroes r1 just before
+ * writing it (:552-553), so the second triple of every ion is 0. */
+int cpsdh_write_ionout(const char *path, int nkatm, const char *const *fname, const int *lmax,
+                       int nion, const int *katm, const double *r2)
+{
+   FILE *f = fopen(path, "w");
+   if (!f) return -1;
+   fprintf(f, "%d\n", nkatm);
+   for (int ia = 0; ia < nkatm; ++ia) fprintf(f, "'%s ' %d\n", fname[ia], lmax[ia]);
+   fprintf(f, "%d\n", nion);
+   for (int ii = 0; ii < nion; ++ii) {
+      fprintf(f, "%d ", katm[ii] + 1);
+      for (int d = 0; d < 3; ++d) fprintf(f, "%lf ", r2[d + 3 * ii]);
+      for (int d = 0; d < 3; ++d) fprintf(f, "%lf ", 0.0);
+      fprintf(f, "\n");
+   }
+   return fclose(f) == 0 ? 0 : -1;
+}

@@ -213,6 +213,42 @@ class State:
         return self.E.copy(), self.deltae.value, self.deltac.value
 
 
+def eigsrt(D, V):
+    """ccpsdv3/eigsrt.c:11-39: selection sort of D to descending order (the
+    last of equal maxima is taken), swapping the columns of the column-major
+    n x n V along.  Works in place on copies and returns them."""
+    D = np.array(D, dtype=np.float64)
+    n = D.size
+    V = np.array(V, dtype=np.float64).reshape(n, n)   # row i = column i of the column-major array
+    for i in range(n - 1):
+        k, p = i, D[i]
+        for j in range(i + 1, n):
+            if D[j] >= p:
+                k, p = j, D[j]
+        if k != i:
+            D[k] = D[i]
+            D[i] = p
+            V[[i, k]] = V[[k, i]]
+    return D, V.reshape(-1)
+
+
+def epilogue(ne, n2ft3d, nfft, c1, c2, hml):
+    """ccpsdv3/main.c:451-484 for ispin = 1: the orthonormality matrix
+    gcdotc(c1_i, c1_j), dsyev('V','U') of hml followed by eigsrt, and
+    c1 = c2 V.  dsyev is LAPACK's (third party, not in the reference tree);
+    numpy.linalg.eigh (LAPACK dsyevd, ascending eigenvalues, upper triangle
+    read) stands in for it here as the checker."""
+    c1 = np.asarray(c1).reshape(ne, n2ft3d)
+    c2 = np.asarray(c2).reshape(ne, n2ft3d)
+    ortho = np.array([[gcdotc(nfft, c1[i], c1[j]) for j in range(ne)] for i in range(ne)])
+    H = np.asarray(hml).reshape(ne, ne).T          # H[i, j] = hml[i + j*ne]
+    w, U = np.linalg.eigh(H, UPLO="U")
+    D, V = eigsrt(w, U.T.reshape(-1))               # column-major: V[j + i*ne]
+    Vm = V.reshape(ne, ne)                          # row i = eigenvector i
+    c1r = Vm @ c2                                   # c1_i = sum_j V[j + i*ne] c2_j
+    return dict(ortho=ortho, eig=D, V=V, c1=c1r.reshape(-1))
+
+
 SAMPLE_PARTS = ("fft", "orbital_passes", "kb_pairs", "structure", "overlaps", "lagrange",
                 "constraint", "lda", "energy_tail", "total")
 
