@@ -126,6 +126,15 @@ void cpsd_dev_destroy(cpsd_dev *h);
  * returns the number of doubles or -1 (test hook) */
 long cpsd_dev_debug_fetch(cpsd_dev *h, int what, double *out, long cap);
 
+/* ---- input generation on the device ---------------------------------
+ * The Kleinman-Bylander transform of ckbppv3/dkbppv3.c:335-444 (the .vpp
+ * generator) with the arguments of cpsdh_kbpp (include/cpsd_host.h): radial
+ * table rho/vp/wp[(lmax+1)*nrho] of a PSP file, G[3*nfft3d] ->
+ * vl[nfft3d], vnl[lmax^2 channels of nfft3d], vnlnrm[9]; vp is not modified. */
+int cpsd_kbpp_device(double zv, int lmax, int nrho, double drho, const double *rho,
+                     const double *vp, const double *wp, int nfft3d, const double *G,
+                     double *vl, double *vnl, double *vnlnrm);
+
 /* ---- kernel-level test hooks (no reference counterpart) -------------
  * Batched versions of the FFT entry points and the LDA kernel, used by
  * the parity tests.                                                     */

@@ -36,7 +36,7 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
-               "cpsd_syev_sorted")
+               "cpsd_syev_sorted", "cpsd_kbpp_device")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
                 "cpsdh_write_ionout")
@@ -82,6 +82,8 @@ def hip_lib():
         L.cpsd_erfc_device.argtypes = [C.c_int, _dp, _dp]
         L.cpsd_dev_epilogue.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.cpsd_syev_sorted.argtypes = [C.c_int, _dp, _dp]
+        L.cpsd_kbpp_device.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_int,
+                                       _dp, _dp, _dp, _dp]
         L.cpsd_rotate.restype = None
         L.cpsd_rotate.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp, _dp, _dp]
         L.cpsd.restype = None
@@ -196,7 +198,7 @@ class Cell:
     """One periodic cell: setup tables (ccpsdv3/main.c:145-320) and the KB
     pseudopotential of every kind on its grid (ckbppv3/dkbppv3.c)."""
 
-    def __init__(self, icube, unit, nfft, kinds, katm, positions):
+    def __init__(self, icube, unit, nfft, kinds, katm, positions, kb_device=False):
         """kinds: list of (psp path, lmax); katm: 0-based kind per ion;
         positions: (nion, 3) bohr."""
         L = host_lib()
@@ -235,10 +237,11 @@ class Cell:
             vl = np.zeros(self.nfft3d)
             vnl = np.zeros(9 * self.nfft3d)
             nrm = np.zeros(9)
-            rc = L.cpsdh_kbpp(p["zv"], lm, p["nrho"], p["drho"], p["rho"], p["vp"], p["wp"],
-                              self.nfft3d, self.G, vl, vnl, nrm)
+            kb = hip_lib().cpsd_kbpp_device if kb_device else L.cpsdh_kbpp
+            rc = kb(p["zv"], lm, p["nrho"], p["drho"], p["rho"], p["vp"], p["wp"], self.nfft3d, self.G,
+                    vl, vnl, nrm)
             if rc != 0:
-                raise CpsdError("cpsdh_kbpp failed")
+                raise CpsdError("KB transform failed" + (": " + _err() if kb_device else ""))
             self.vl[k * self.nfft3d:(k + 1) * self.nfft3d] = vl
             self.vnl[k * 9 * self.nfft3d:(k + 1) * 9 * self.nfft3d] = vnl
             self.vnlnrm[9 * k:9 * k + 9] = nrm
@@ -309,11 +312,13 @@ def water_box(nmol, L, seed=0, jitter=0.3):
     return np.array(katm, dtype=np.int32), np.concatenate(pos)
 
 
-def water_cell(nmol, L, nfft, seed=0, jitter=0.3, psp_dir=None):
+def water_cell(nmol, L, nfft, seed=0, jitter=0.3, psp_dir=None, kb_device=False):
+    """nmol waters (O s+p, H s projectors) in an L-bohr cubic cell; kb_device
+    runs the KB transform on the GPU (cpsd_kbpp_device) instead of the host."""
     d = psp_dir or os.path.join(DATA, "psp")
     katm, pos = water_box(nmol, L, seed, jitter)
     return Cell(1, L, nfft, [(os.path.join(d, "O00.dat"), 2), (os.path.join(d, "H00.dat"), 1)],
-                katm, pos)
+                katm, pos, kb_device=kb_device)
 
 
 # --------------------------------------------------------- device SD step
