@@ -171,60 +171,67 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
 /* ------------------------------------------------------ plane helpers */
 
 /* Z_k = (X_k + conj X_{M-k}) + i W^k (X_k - conj X_{M-k}), W = e^{2 pi i/N};
- * only Re X_0 and Re X_M are used (drfftb ignores their imaginary parts). */
+ * only Re X_0 and Re X_M are used (drfftb ignores their imaginary parts).
+ * The pairs (k, M-k) with 0 < k < M/2 run in one loop and the self-paired
+ * columns k = 0 and k = M/2 in a second one (they touch other elements, so
+ * no barrier is needed between them): every wave then takes one path. */
 template<int N>
 __device__ __forceinline__ void c2r_pre(double2 *sm, const double2 *__restrict__ tw)
 {
-   constexpr int NH = N / 2 + 1, M = N / 2, KP = M / 2 + 1;
-   for (int it = threadIdx.x; it < N * KP; it += blockDim.x) {
-      const int s = it / KP, k = it % KP;
+   constexpr int NH = N / 2 + 1, M = N / 2, KG = M / 2 - 1;
+   static_assert(M % 2 == 0, "N/2 must be even");
+   for (int it = threadIdx.x; it < N * KG; it += blockDim.x) {
+      const int s = it / KG, k = 1 + it % KG;
       double2 *row = sm + s * NH;
-      if (k == 0) {
+      const double2 a = row[k], b = row[M - k];
+      const double2 w = tw[k];
+      const double2 wm = make_double2(-w.x, w.y);  /* W^{M-k} = -conj(W^k) */
+      row[k] = cadd(cadd(a, cconj(b)), cmuli(cmul(w, csub(a, cconj(b)))));
+      row[M - k] = cadd(cadd(b, cconj(a)), cmuli(cmul(wm, csub(b, cconj(a)))));
+   }
+   for (int it = threadIdx.x; it < 2 * N; it += blockDim.x) {
+      double2 *row = sm + (it >> 1) * NH;
+      if (it & 1) {
+         const double2 a = row[M / 2];
+         const double2 w = tw[M / 2];
+         row[M / 2] = cadd(cadd(a, cconj(a)), cmuli(cmul(w, csub(a, cconj(a)))));
+      } else {
          const double x0 = row[0].x, xm = row[M].x;
          row[0] = make_double2(x0 + xm, x0 - xm);
-      } else if (k < M - k) {
-         const double2 a = row[k], b = row[M - k];
-         const double2 w = tw[k];
-         const double2 wm = make_double2(-w.x, w.y);  /* W^{M-k} = -conj(W^k) */
-         const double2 za = cadd(cadd(a, cconj(b)), cmuli(cmul(w, csub(a, cconj(b)))));
-         const double2 zb = cadd(cadd(b, cconj(a)), cmuli(cmul(wm, csub(b, cconj(a)))));
-         row[k] = za;
-         row[M - k] = zb;
-      } else if (k == M - k) {
-         const double2 a = row[k];
-         const double2 w = tw[k];
-         row[k] = cadd(cadd(a, cconj(a)), cmuli(cmul(w, csub(a, cconj(a)))));
       }
    }
 }
 
 /* X_k = (Z_k + conj Z_{M-k})/2 - (i/2) W^{-k} (Z_k - conj Z_{M-k}), k = 0..M,
- * with Im X_0 = Im X_M = 0 exactly (fftpack3d.f:160-161). */
+ * with Im X_0 = Im X_M = 0 exactly (fftpack3d.f:160-161); same split of the
+ * work as c2r_pre. */
 template<int N>
 __device__ __forceinline__ void r2c_post(double2 *sm, const double2 *__restrict__ tw)
 {
-   constexpr int NH = N / 2 + 1, M = N / 2, KP = M / 2 + 1;
-   for (int it = threadIdx.x; it < N * KP; it += blockDim.x) {
-      const int s = it / KP, k = it % KP;
+   constexpr int NH = N / 2 + 1, M = N / 2, KG = M / 2 - 1;
+   for (int it = threadIdx.x; it < N * KG; it += blockDim.x) {
+      const int s = it / KG, k = 1 + it % KG;
       double2 *row = sm + s * NH;
-      if (k == 0) {
+      const double2 za = row[k], zb = row[M - k];
+      const double2 w = cconj(tw[k]);                        /* W^{-k} */
+      const double2 wm = make_double2(-tw[k].x, -tw[k].y);    /* W^{-(M-k)} = -W^k */
+      const double2 sa = cadd(za, cconj(zb)), da = csub(za, cconj(zb));
+      const double2 sb = cadd(zb, cconj(za)), db = csub(zb, cconj(za));
+      const double2 ta = cmuli(cmul(w, da)), tb = cmuli(cmul(wm, db));
+      row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
+      row[M - k] = make_double2(0.5 * (sb.x - tb.x), 0.5 * (sb.y - tb.y));
+   }
+   for (int it = threadIdx.x; it < 2 * N; it += blockDim.x) {
+      double2 *row = sm + (it >> 1) * NH;
+      if (it & 1) {
+         const double2 z = row[M / 2];
+         const double2 w = cconj(tw[M / 2]);
+         const double2 sa = cadd(z, cconj(z)), ta = cmuli(cmul(w, csub(z, cconj(z))));
+         row[M / 2] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
+      } else {
          const double2 z0 = row[0];
          row[0] = make_double2(z0.x + z0.y, 0.0);
          row[M] = make_double2(z0.x - z0.y, 0.0);
-      } else if (k < M - k) {
-         const double2 za = row[k], zb = row[M - k];
-         const double2 w = cconj(tw[k]);                        /* W^{-k} */
-         const double2 wm = make_double2(-tw[k].x, -tw[k].y);    /* W^{-(M-k)} = -W^k */
-         const double2 sa = cadd(za, cconj(zb)), da = csub(za, cconj(zb));
-         const double2 sb = cadd(zb, cconj(za)), db = csub(zb, cconj(za));
-         const double2 ta = cmuli(cmul(w, da)), tb = cmuli(cmul(wm, db));
-         row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
-         row[M - k] = make_double2(0.5 * (sb.x - tb.x), 0.5 * (sb.y - tb.y));
-      } else if (k == M - k) {
-         const double2 z = row[k];
-         const double2 w = cconj(tw[k]);
-         const double2 sa = cadd(z, cconj(z)), ta = cmuli(cmul(w, csub(z, cconj(z))));
-         row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
       }
    }
 }
