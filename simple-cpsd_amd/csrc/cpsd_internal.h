@@ -59,6 +59,9 @@ struct ZArgs {
    const double2 *c1;    /* DST_HPSI: kinetic term uses C1 */
    const double *tg;
    double scal1;
+   /* set by the launcher: orbitals per workgroup (blockIdx.z covers
+    * [z*bper, min(nbatch, (z+1)*bper))) */
+   int nbatch, bper;
 };
 
 struct PlaneArgs {

@@ -37,23 +37,27 @@ __device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*r
    __syncthreads();
 }
 
+/* G -> r along kz for bper orbitals of one (kx block, ky) row: the gather of
+ * orbital b+1 is issued into the registers it is loaded into while orbital
+ * b is transformed and written, so each workgroup keeps one column block of
+ * loads in flight */
 template<int N, int SRC>
-__global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_g2r(ZArgs a)
+__global__ void __launch_bounds__(GridCfg<N>::ZT, 4) k_zpass_g2r(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;
    constexpr int EPT = (N * KXB + ZT - 1) / ZT;   /* elements per thread */
    __shared__ double2 sm[N * KXB];
    __shared__ int rt[3][N];
-   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
+   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y;
+   const int b0 = blockIdx.z * a.bper, b1 = min(a.nbatch, b0 + a.bper);
    const int ncol = min(KXB, a.rows.nhc - kx0);
-   const double2 *src = a.src + (size_t)b * a.sstride;
    /* twiddles are read from LDS (the barrier in stage_rows publishes them) */
    __shared__ double2 tws[N];
    for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
    stage_rows<N>(a.rows, ky, rt);
 
-   /* every load of this thread is issued before any is consumed */
+   /* gather positions, the same for every orbital of the row */
    int pe[EPT];
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
@@ -66,34 +70,42 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_g2r(ZArgs a)
       }
    }
    double2 v[EPT];
-   if (SRC == SRC_PACKED) {
+   auto gather = [&](int b) {
+      if (SRC == SRC_PACKED) {
+         const double2 *src = a.src + (size_t)b * a.sstride;
 #pragma unroll
-      for (int q = 0; q < EPT; ++q) v[q] = pe[q] >= 0 ? src[pe[q]] : make_double2(0.0, 0.0);
-   } else {
+         for (int q = 0; q < EPT; ++q) v[q] = pe[q] >= 0 ? src[pe[q]] : make_double2(0.0, 0.0);
+      } else {
 #pragma unroll
-      for (int q = 0; q < EPT; ++q) {
-         v[q] = make_double2(0.0, 0.0);
-         if (pe[q] >= 0) {
-            const double2 vl = a.vall[pe[q]], dg = a.dng[pe[q]];
-            const double c = a.vc[pe[q]];
-            v[q] = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
+         for (int q = 0; q < EPT; ++q) {
+            v[q] = make_double2(0.0, 0.0);
+            if (pe[q] >= 0) {
+               const double2 vl = a.vall[pe[q]], dg = a.dng[pe[q]];
+               const double c = a.vc[pe[q]];
+               v[q] = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
+            }
          }
       }
-   }
+   };
+   if (b0 < b1) gather(b0);
+   for (int b = b0; b < b1; ++b) {
 #pragma unroll
-   for (int q = 0; q < EPT; ++q) {
-      const int e = threadIdx.x + ZT * q;
-      if (e < N * KXB) sm[e] = v[q];
-   }
-   __syncthreads();
-   lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, tws, 1);
-   double2 *T = a.T + (size_t)b * N * N * NH;
-   for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
-      const int s = e / KXB, kxl = e % KXB;
-      if (kxl < ncol) {
-         const int z = tr_index<C::NY1, C::NY2>(s);
-         T[((size_t)z * N + ky) * NH + kx0 + kxl] = sm[e];
+      for (int q = 0; q < EPT; ++q) {
+         const int e = threadIdx.x + ZT * q;
+         if (e < N * KXB) sm[e] = v[q];
       }
+      __syncthreads();
+      if (SRC == SRC_PACKED && b + 1 < b1) gather(b + 1);   /* the field pass runs one row */
+      lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, tws, 1);
+      double2 *T = a.T + (size_t)b * N * N * NH;
+      for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
+         const int s = e / KXB, kxl = e % KXB;
+         if (kxl < ncol) {
+            const int z = tr_index<C::NY1, C::NY2>(s);
+            T[((size_t)z * N + ky) * NH + kx0 + kxl] = sm[e];
+         }
+      }
+      __syncthreads();
    }
 }
 
@@ -428,10 +440,17 @@ int fft_supported(int n)
 }
 
 template<int N>
-static int zg2r(int src, const ZArgs &a, int nbatch, hipStream_t s)
+static int zg2r(int src, const ZArgs &args, int nbatch, hipStream_t s)
 {
    constexpr int KXB = GridCfg<N>::KXB;
-   dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
+   ZArgs a = args;
+   const int nkb = (a.rows.nhc + KXB - 1) / KXB;
+   /* several orbitals per workgroup while the grid still has >= 8 workgroups
+    * per CU to spread */
+   a.nbatch = nbatch;
+   a.bper = 1;
+   while (a.bper < 4 && (long)nkb * N * ((nbatch + 2 * a.bper - 1) / (2 * a.bper)) >= 2048) a.bper *= 2;
+   dim3 grid(nkb, N, (nbatch + a.bper - 1) / a.bper);
    constexpr int ZT = GridCfg<N>::ZT;
    if (src == SRC_PACKED) hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED>), grid, dim3(ZT), 0, s, a);
    else hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG>), grid, dim3(ZT), 0, s, a);

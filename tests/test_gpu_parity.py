@@ -48,6 +48,25 @@ def test_fft_cr_rc_vs_oracle(gpu, n):
         assert g.shape == (n, n, n + 2)
 
 
+@pytest.mark.parametrize("n,nb", [(32, 128), (32, 129), (48, 198)])
+def test_fft_large_batch_vs_oracle(gpu, n, nb):
+    """batches large enough that a z-pass workgroup takes several orbitals of
+    one row (and a ragged last group): every transform against the oracle"""
+    say("fft batch parity N=%d nb=%d" % (n, nb))
+    rng = np.random.default_rng(100 + nb)
+    a = rng.standard_normal((nb, (n + 2) * n * n))
+    got = cpsd_amd.fft3d(+1, n, a)
+    for b in range(nb):
+        ref = O.cr_fft3b(a[b], n)
+        scale = np.sqrt(n ** 3) * np.max(np.abs(a[b]))
+        assert np.max(np.abs(got[b] - ref)) < 1e-13 * scale * np.log2(n) * 4, b
+    got = cpsd_amd.fft3d(-1, n, a)
+    for b in range(nb):
+        ref = O.rc_fft3f(a[b], n)
+        scale = np.sqrt(n ** 3) * np.max(np.abs(a[b]))
+        assert np.max(np.abs(got[b] - ref)) < 1e-13 * scale * np.log2(n) * 4, b
+
+
 def test_fortran_abi_entry_points(gpu):
     n = 32
     rng = np.random.default_rng(5)
