@@ -357,9 +357,29 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
       if (fb > ne) fb = ne;
       c->fb = (int)fb;
    }
-   c->ndnblk = c->fb < 8 ? c->fb : 8;
-   c->nob = (c->fb + c->ndnblk - 1) / c->ndnblk;
-   c->ndnblk = (c->fb + c->nob - 1) / c->nob;
+   /* orbital blocks of the density pass (each sums its orbitals' psi^2 into
+    * one partial plane set): the fewest blocks whose n x blocks workgroups
+    * fill at least 90% of whole rounds of the GPU's workgroup slots, else
+    * the best filling up to 32 blocks; CPSD_DN_BLOCKS sets the count */
+   {
+      int dev = 0, cus = 0;
+      CPSD_CHECK(hipGetDevice(&dev));
+      CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      const long slots = (long)(cus > 0 ? cus : 1) * plane_density_occupancy(n);
+      int best = 1;
+      double beff = -1.0;
+      for (int d = 1; d <= (c->fb < 32 ? c->fb : 32); ++d) {
+         const int nob = (c->fb + d - 1) / d, dd = (c->fb + nob - 1) / nob;
+         const long wgs = (long)n * dd;
+         const double eff = (double)wgs / ((double)slots * (double)((wgs + slots - 1) / slots));
+         if (eff >= 0.9) { best = dd; break; }
+         if (eff > beff + 1e-12) { beff = eff; best = dd; }
+      }
+      const char *env = getenv("CPSD_DN_BLOCKS");   /* tuning override */
+      if (env && atoi(env) > 0) best = atoi(env) < c->fb ? atoi(env) : c->fb;
+      c->nob = (c->fb + best - 1) / best;
+      c->ndnblk = (c->fb + c->nob - 1) / c->nob;
+   }
    TRY(dalloc(&c->dnpart, (size_t)c->ndnblk * c->n3));
    TRY(dalloc(&c->dn, c->n3)); TRY(dalloc(&c->xcp, c->n3)); TRY(dalloc(&c->xce, c->n3));
    TRY(dalloc(&c->V, c->n3));

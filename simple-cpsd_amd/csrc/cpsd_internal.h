@@ -85,6 +85,8 @@ int fft_supported(int n);
 int launch_zpass_g2r(int n, int src, const ZArgs &a, int nbatch, hipStream_t s);
 int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s);
 int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s);
+/* workgroups of the density plane pass one CU holds at once */
+int plane_density_occupancy(int n);
 
 /* ---- GEMM ------------------------------------------------------------- */
 

@@ -524,6 +524,28 @@ int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s)
    return -1;
 }
 
+template<int N>
+static int density_blocks_per_cu()
+{
+   const size_t lds = ((size_t)N * (N / 2 + 1) + N) * sizeof(double2);
+   const void *k = (const void *)k_plane<N, PM_DENSITY>;
+   int nb = 0;
+   if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
+       hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, GridCfg<N>::PLANE_THREADS, lds) != hipSuccess)
+      return 1;
+   return nb > 0 ? nb : 1;
+}
+
+int plane_density_occupancy(int n)
+{
+   switch (n) {
+#define CASE(nn) case nn: return density_blocks_per_cu<nn>();
+      CPSD_FOR_EACH_GRID(CASE)
+#undef CASE
+   }
+   return 1;
+}
+
 int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s)
 {
    switch (n) {
