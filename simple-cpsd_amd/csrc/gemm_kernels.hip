@@ -142,14 +142,12 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wm = wave & 1, wn = wave >> 1;
    const int s1 = rot_step(lane);
-   /* loader: row tid/4 of the tile, 8 consecutive k at 8*(tid%4) */
-   const int lr = tid >> 2, lq = (tid & 3) * 8;
+   /* loader: rows lr + 32 h (h < 4) of the tile, k pair 2 (tid % 16): 16
+    * lanes cover one 256-byte row segment, contiguous in global memory and,
+    * as 16-byte stores, conflict-free in LDS */
+   const int lr = tid >> 4, lq = (tid & 15) * 2;
    constexpr int BI = TMR / 32;   /* 16-row blocks per wave */
    constexpr int BA = BI > 4 ? BI : 4;
-   const int am = tm * TMR + lr, bn = tn * TT + lr;
-   const bool aval = lr < TMR && am < M, bval = bn < N;
-   const double *Ap = A + (size_t)(am < M ? am : 0) * lda;
-   const double *Bp = B + (size_t)(bn < N ? bn : 0) * ldb;
 
    double acc[BA][2][4];   /* [row block][column block][rotation] */
    double accx[2][4];      /* blocks 8, 9 of a triangle role */
@@ -172,18 +170,19 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto gload = [&](size_t k0) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
+         const int r = lr + 32 * h, am = tm * TMR + r, bn = tn * TT + r;
          ra[h] = make_double2(0.0, 0.0);
          rb[h] = make_double2(0.0, 0.0);
-         if (aval) ra[h] = *reinterpret_cast<const double2 *>(Ap + k0 + lq + 2 * h);
-         if (bval) rb[h] = *reinterpret_cast<const double2 *>(Bp + k0 + lq + 2 * h);
+         if (r < TMR && am < M) ra[h] = *reinterpret_cast<const double2 *>(A + (size_t)am * lda + k0 + lq);
+         if (bn < N) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + lq);
       }
       if (tid < KT / 2) rw = *reinterpret_cast<const double2 *>(w + k0 + 2 * tid);
    };
    auto lstore = [&](int bf) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-         *reinterpret_cast<double2 *>(&S[bf][0][lr][lq + 2 * h]) = ra[h];
-         *reinterpret_cast<double2 *>(&S[bf][1][lr][lq + 2 * h]) = rb[h];
+         *reinterpret_cast<double2 *>(&S[bf][0][lr + 32 * h][lq]) = ra[h];
+         *reinterpret_cast<double2 *>(&S[bf][1][lr + 32 * h][lq]) = rb[h];
       }
       if (tid < KT / 2) { Ws[bf][2 * tid] = rw.x; Ws[bf][2 * tid + 1] = rw.y; }
    };
@@ -425,12 +424,13 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
    constexpr int WN = NT / 64, WK = 8 / WN;
    constexpr int BI = TK / WK / 16;          /* 16-row blocks of K per wave */
    constexpr int BJ = 4;                     /* 16-column blocks per wave */
-   constexpr int AS = 2 * MC * (TK + 4), XS = 2 * NT * (MC + 1), TS = 64 * (TK + 2);
+   constexpr int AP = TK + 16;   /* A panel pitch: a 64-bit read of 16 rows x 2 k is conflict-free */
+   constexpr int AS = 2 * MC * AP, XS = 2 * NT * (MC + 1), TS = 64 * (TK + 2);
    constexpr int LD = AS + XS > TS ? AS + XS : TS;
-   /* main loop: As[2][MC][TK+4] and Xs[2][NT][MC+1]; epilogue: T[64][TK+2],
+   /* main loop: As[2][MC][AP] and Xs[2][NT][MC+1]; epilogue: T[64][TK+2],
     * one 64-column half of the output tile at a time */
    __shared__ double lds[LD];
-   double (*As)[MC][TK + 4] = reinterpret_cast<double (*)[MC][TK + 4]>(lds);
+   double (*As)[MC][AP] = reinterpret_cast<double (*)[MC][AP]>(lds);
    double (*Xs)[NT][MC + 1] = reinterpret_cast<double (*)[NT][MC + 1]>(lds + AS);
    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
    const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
@@ -452,17 +452,19 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
 #pragma unroll
          for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
 
-   /* loaders: A chunk MC x TK (4 doubles per thread), X chunk NT x MC */
+   /* loaders: A chunk MC x TK, rows ar + 8 h with 64 lanes per 1 KB row (two
+    * 16-byte pieces per thread); X chunk NT x MC */
    constexpr int XQ = NT * MC / 512;
-   const int ar = tid / (TK / 4), ac = (tid % (TK / 4)) * 4;
+   static_assert(TK == 128 && MC == 16, "A loader covers 16 rows of 128");
+   const int ar = tid >> 6, ac = (tid & 63) * 2;
    double2 ra[2];
    double rx[XQ];
    auto gload = [&](int m0) {
-      const int m = m0 + ar;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
+         const int m = m0 + ar + 8 * h;
          ra[h] = make_double2(0.0, 0.0);
-         if (m < a.Mr) ra[h] = *reinterpret_cast<const double2 *>(a.A + (size_t)m * a.lda + k0 + ac + 2 * h);
+         if (m < a.Mr) ra[h] = *reinterpret_cast<const double2 *>(a.A + (size_t)m * a.lda + k0 + ac);
       }
 #pragma unroll
       for (int u = 0; u < XQ; ++u) {
@@ -474,10 +476,7 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
    };
    auto lstore = [&](int bf) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-         As[bf][ar][ac + 2 * h] = ra[h].x;
-         As[bf][ar][ac + 2 * h + 1] = ra[h].y;
-      }
+      for (int h = 0; h < 2; ++h) *reinterpret_cast<double2 *>(&As[bf][ar + 8 * h][ac]) = ra[h];
 #pragma unroll
       for (int u = 0; u < XQ; ++u) {
          const int e = tid + 512 * u;
