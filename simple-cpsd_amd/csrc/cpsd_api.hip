@@ -530,9 +530,13 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(ph_end(c, PH_VPSI));
       TRY(ph_begin(c, PH_ZR2G));
       zargs_base(c, z, 0);
-      z.T = c->T + (size_t)b0 * plane; z.out = c->CP + (size_t)b0 * ngp; z.ostride = ngp;
-      z.c1 = C1 + (size_t)b0 * ngp; z.tg = c->tg; z.scal1 = scal1;
-      TRY(launch_zpass_r2g(n, DST_HPSI, z, bs, st));
+      /* -H psi (local part) is kept only on the last step, for the energy
+       * tail; the other steps write c1 + dte (local -H psi) straight into c2
+       * and the KB apply adds dte times the nonlocal part in place */
+      z.T = c->T + (size_t)b0 * plane; z.ostride = ngp;
+      z.out = (last ? c->CP : C2) + (size_t)b0 * ngp;
+      z.c1 = C1 + (size_t)b0 * ngp; z.tg = c->tg; z.scal1 = scal1; z.dte = dte;
+      TRY(launch_zpass_r2g(n, last ? DST_HPSI : DST_SD, z, bs, st));
       TRY(ph_end(c, PH_ZR2G));
    }
    /* 9 */
@@ -570,13 +574,14 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    }
    /* 10 */
    TRY(ph_begin(c, PH_KBAPPLY));
-   {
+   if (last || c->nproj > 0) {
       TallArgs t;
       t.A = reinterpret_cast<const double *>(c->B); t.lda = K;
       t.X = c->Xkb; t.ldx = c->nproj; t.Mr = c->nproj; t.Nc = ne; t.K = K;
       t.C2 = reinterpret_cast<double *>(C2); t.C1 = reinterpret_cast<const double *>(C1);
       t.CP = reinterpret_cast<double *>(c->CP); t.ldc = K; t.dte = dte; t.write_cp = last;
-      TRY(launch_gemm_tall(EPI_KBAPPLY, t, st));
+      /* last step: CP += B X, C2 = C1 + dte CP; otherwise C2 += dte B X */
+      TRY(launch_gemm_tall(last ? EPI_KBAPPLY : EPI_CONSTRAINT, t, st));
    }
    TRY(ph_end(c, PH_KBAPPLY));
    /* 11 */

@@ -35,7 +35,9 @@ void cpsd_set_error(const char *fmt, ...);
 /* ---- FFT ------------------------------------------------------------ */
 
 enum { SRC_PACKED = 0, SRC_VG = 1 };
-enum { DST_HPSI = 0, DST_DNG = 1, DST_FULL = 2 };
+/* DST_SD: out = c1 + dte (-scal1 V psi - tg c1), the SD update before the
+ * nonlocal term (used when -H psi itself is not needed afterwards) */
+enum { DST_HPSI = 0, DST_DNG = 1, DST_FULL = 2, DST_SD = 3 };
 enum { PM_DENSITY = 0, PM_VTOT = 1, PM_VPSI = 2, PM_C2R_OUT = 3, PM_R2C_DN = 4, PM_R2C_IN = 5 };
 
 struct RowTable {
@@ -56,9 +58,10 @@ struct ZArgs {
    /* r->G outputs */
    double2 *out;         /* packed or full grid, batch stride ostride */
    size_t ostride;
-   const double2 *c1;    /* DST_HPSI: kinetic term uses C1 */
+   const double2 *c1;    /* DST_HPSI, DST_SD: kinetic term uses C1 */
    const double *tg;
    double scal1;
+   double dte;           /* DST_SD */
    /* set by the launcher: orbitals per workgroup (blockIdx.z covers
     * [z*bper, min(nbatch, (z+1)*bper))) */
    int nbatch, bper;

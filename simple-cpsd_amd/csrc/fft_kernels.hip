@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
          if (kx >= rs && kx < rs + rl) pe[q] = rt[2][kz] + kx - rs;
       }
    }
-   if (DST == DST_HPSI) {
+   if (DST == DST_HPSI || DST == DST_SD) {
       /* kinetic term inputs, all loads issued first */
       double2 c[EPT];
       double t[EPT];
@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
       for (int q = 0; q < EPT; ++q) {
          if (pe[q] >= 0) {
             const double2 v = sm[threadIdx.x + ZT * q];
-            out[pe[q]] = make_double2(-a.scal1 * v.x - t[q] * c[q].x, -a.scal1 * v.y - t[q] * c[q].y);
+            const double2 h = make_double2(-a.scal1 * v.x - t[q] * c[q].x, -a.scal1 * v.y - t[q] * c[q].y);
+            out[pe[q]] = DST == DST_SD ? make_double2(c[q].x + a.dte * h.x, c[q].y + a.dte * h.y) : h;
          }
       }
    } else {
@@ -465,6 +466,7 @@ static int zr2g(int dst, const ZArgs &a, int nbatch, hipStream_t s)
    dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
    constexpr int ZT = GridCfg<N>::ZT;
    if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI>), grid, dim3(ZT), 0, s, a);
+   else if (dst == DST_SD) hipLaunchKernelGGL((k_zpass_r2g<N, DST_SD>), grid, dim3(ZT), 0, s, a);
    else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG>), grid, dim3(ZT), 0, s, a);
    else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL>), grid, dim3(ZT), 0, s, a);
    CPSD_CHECK(hipGetLastError());
