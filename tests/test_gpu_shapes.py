@@ -1,4 +1,4 @@
-"""GPU parity at the shapes of the BASELINE workloads: the grids of the
+"""GPU parity at the shapes of the benchmark workloads: the grids of the
 32-water (96^3) and 64-water (128^3) cells through the whole SD step against
 the oracle, orbital counts above 128 (the 256-orbital tile grid of the
 symmetric Gram with its diagonal pairs, an odd tile count, the general
