@@ -252,8 +252,8 @@ def main():
     p.add_argument("--cpu-sample", type=int, default=16)
     args = p.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the SD step of one "
-              "cell runs on one MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the SD step of one cell runs on one MI355X and there "
+              "is no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     # the library prints the reference's own notices (e.g. "ierr=10 ...", as
     # ccpsdv3/cpsd.c:477 does) on C stdout; keep stdout for the one JSON line
