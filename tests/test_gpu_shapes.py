@@ -115,3 +115,12 @@ def test_full_h2o32_properties(gpu):
     F2, _, _ = dev.run(10)
     assert np.array_equal(F1, E1) and np.array_equal(F2, E2)
     dev.close()
+
+
+@pytest.mark.parametrize("nfft,L,nmol,ne", [(16, 10.0, 1, 1), (16, 10.0, 1, 5), (24, 12.0, 2, 17),
+                                            (24, 12.0, 2, 33)])
+def test_odd_orbital_counts_vs_oracle(gpu, nfft, L, nmol, ne):
+    """one orbital, odd counts and counts just past a 16-row block or a
+    32-wide split step, against the oracle"""
+    say("ne=%d at %d^3" % (ne, nfft))
+    run_pair(nmol, L, nfft, ne, steps=2, inner=3)
