@@ -390,8 +390,12 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    {
       const size_t K = 2 * (size_t)ngp;
       size_t cap = 0;
-      int sp = splits_for(2 * ne, 2 * ne, K, 1);
-      cap = (size_t)sp * 4 * ne * ne;
+      int sp = splits_for(2 * ne, 2 * ne, K, 1), so, sd;
+      gemm_tn_symm_splits(2 * ne, K, sp, &so, &sd);
+      cap = (size_t)(so > sd ? so : sd) * 4 * ne * ne;
+      sp = splits_for(ne, ne, K, 1);   /* the epilogue's Gram of c1 */
+      gemm_tn_symm_splits(ne, K, sp, &so, &sd);
+      if ((size_t)(so > sd ? so : sd) * ne * ne > cap) cap = (size_t)(so > sd ? so : sd) * ne * ne;
       sp = splits_for(nproj, ne, K, 0);
       if ((size_t)sp * nproj * ne > cap) cap = (size_t)sp * nproj * ne;
       sp = splits_for(ne, ne, K, 0);
@@ -593,7 +597,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(ph_end(c, PH_GRAM));
       TRY(ph_begin(c, PH_GRAMRED));
       /* diagonal tiles hold their upper 16 x 16 blocks only */
-      TRY(launch_reduce_partials(c->partial, sp, 2 * ne, 2 * ne, c->Gm, 16, st));
+      TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
       TRY(sd_smat(c->Gm, ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, st));
       TRY(ph_end(c, PH_GRAMRED));
    }
@@ -807,10 +811,15 @@ static int ctx_epilogue(cpsd_dev *c, double *ortho, double *eig, double *V)
    const double2 *C2 = c->Cbase + (size_t)c->cur * orb;
    if (ortho) {
       const int sp = splits_for(ne, ne, K, 1);
-      if ((size_t)sp * ee > c->partial_cap) { cpsd_set_error("epilogue: split-K buffer too small"); return -1; }
+      int so, sd;
+      gemm_tn_symm_splits(ne, K, sp, &so, &sd);
+      if ((size_t)(so > sd ? so : sd) * ee > c->partial_cap) {
+         cpsd_set_error("epilogue: split-K buffer too small");
+         return -1;
+      }
       const double *c1r = reinterpret_cast<const double *>(C1);
       TRY(launch_gemm_tn_splitk(c1r, K, ne, c1r, K, ne, c->w, K, sp, 1, c->partial, st));
-      TRY(launch_reduce_partials(c->partial, sp, ne, ne, c->Gm, 16, st));
+      TRY(launch_reduce_partials_symm(c->partial, ne, K, sp, c->Gm, st));
       CPSD_CHECK(hipMemcpyAsync(ortho, c->Gm, ee * sizeof(double), hipMemcpyDeviceToHost, st));
    }
    std::vector<double> a(ee), w(ne);

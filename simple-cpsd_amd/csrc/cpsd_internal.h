@@ -100,10 +100,17 @@ int plane_density_occupancy(int n);
 int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, size_t ldb,
                           int N, const double *w, size_t K, int splits, int symm,
                           double *partial, hipStream_t s);
-/* out[m*ldo + n] = sum_z partial[z][m][n] */
-/* symm: 0, or the block edge below whose diagonal the partials were not formed */
+/* out[m*N + n] = sum_z partial[z][m][n], z < splits (symm: 0, or the block
+ * edge below whose diagonal the partials were not formed, all with the same
+ * split count) */
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s);
+/* the same for a symmetric launch_gemm_tn_splitk(..., splits, symm = 1, ...),
+ * whose diagonal and off-diagonal tiles use different split counts */
+int launch_reduce_partials_symm(const double *partial, int M, size_t K, int splits, double *out,
+                                hipStream_t s);
+/* split counts a symmetric product uses: off-diagonal tiles and diagonal pairs */
+void gemm_tn_symm_splits(int M, size_t K, int splits, int *sp_off, int *sp_dd);
 
 /* EPI_STORE: C2[n][k] = result (the rotation c1 = c2 V of the epilogue) */
 enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2, EPI_STORE = 3 };

@@ -105,32 +105,38 @@ template<int TMR>
 __global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
                  const double *__restrict__ B, size_t ldb, int N,
-                 const double *__restrict__ w, size_t K, size_t kchunk, int symm, int ntm,
-                 int ntile, int per, double *__restrict__ partial)
+                 const double *__restrict__ w, size_t K, size_t kchunk, size_t kchunk_dd, int symm,
+                 int ntm, int ntile, long poff, int per, double *__restrict__ partial)
 {
    const int bid = blockIdx.x;
-   const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
-   const int t = g % ntile, kz = g / ntile;
-   int tm, tn;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   int tm, tn, kz;
    bool dd = false;
+   size_t kc = kchunk;
    if (symm) {
-      const int noff = ntm * (ntm - 1) / 2;
-      if (t < noff) {   /* tiles strictly above the diagonal, row by row */
+      const int noff = ntm * (ntm - 1) / 2, ndd = (ntm + 1) / 2;
+      if (g < poff) {   /* tiles strictly above the diagonal, row by row */
+         int rem = (int)(g % noff);
+         kz = (int)(g / noff);
          tm = 0;
-         int rem = t;
          while (rem >= ntm - 1 - tm) { rem -= ntm - 1 - tm; ++tm; }
          tn = tm + 1 + rem;
       } else {          /* diagonal pair (tm, tm + 1); tile ntm, if named, is empty */
-         tm = 2 * (t - noff);
+         const long h = g - poff;
+         tm = 2 * (int)(h % ndd);
+         kz = (int)(h / ndd);
          tn = tm + 1;
          dd = true;
+         kc = kchunk_dd;
       }
    } else {
+      const int t = (int)(g % ntile);
+      kz = (int)(g / ntile);
       tm = t % ntm;
       tn = t / ntm;
    }
-   const size_t kb = (size_t)kz * kchunk;
-   const size_t ke = min(K, kb + kchunk);
+   const size_t kb = (size_t)kz * kc;
+   const size_t ke = min(K, kb + kc);
    /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
     * stages the output tile for the epilogue.  Row pitch KT + 2: the 16 rows
     * x 2 k of a half-wave's operand read fall in distinct bank pairs, and
@@ -318,6 +324,29 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    }
 }
 
+/* K splits of a symmetric product: the tiles above the diagonal take sp_off
+ * and the diagonal pairs sp_dd, in the ratio of their work per SIMD (16 : 18
+ * blocks of 16 x 16), so that both kinds of workgroup take about as long;
+ * the total is about what `splits` for every tile would give */
+void gemm_tn_symm_splits(int M, size_t K, int splits, int *sp_off, int *sp_dd)
+{
+   const int ntm = (M + TT - 1) / TT;
+   const int noff = ntm * (ntm - 1) / 2, ndd = (ntm + 1) / 2;
+   const int kmax = (int)(K / KT) > 0 ? (int)(K / KT) : 1;
+   if (noff == 0) {
+      *sp_off = 0;
+      *sp_dd = splits < kmax ? splits : kmax;
+      return;
+   }
+   const long W = (long)(noff + ndd) * splits;
+   int so = (int)(W * 16 / (16L * noff + 18L * ndd));
+   if (so < 1) so = 1;
+   int sd = (int)((W - (long)noff * so + ndd - 1) / ndd);
+   if (sd < 1) sd = 1;
+   *sp_off = so < kmax ? so : kmax;
+   *sp_dd = sd < kmax ? sd : kmax;
+}
+
 /* tiles of one K split: all of them, or for a symmetric product those above
  * the diagonal plus the diagonal pairs */
 int gemm_tn_tiles(int M, int N, int symm)
@@ -338,33 +367,51 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
 {
    if (K % KT) { cpsd_set_error("gemm_tn: K=%zu not a multiple of %d", K, KT); return -1; }
    if (symm && M != N) { cpsd_set_error("gemm_tn: symmetric product needs M == N"); return -1; }
-   size_t kchunk = (K + splits - 1) / splits;
-   kchunk = (kchunk + KT - 1) / KT * KT;
+   auto chunk = [&](int sp) {
+      size_t kc = (K + sp - 1) / sp;
+      return (kc + KT - 1) / KT * KT;
+   };
    /* row tile: 128, or 96 when that leaves fewer padding rows (a symmetric
     * product keeps square tiles) */
    const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
    const int tmr = (!symm && pad96 < pad128) ? 96 : TT;
    const int ntm = (M + tmr - 1) / tmr;
    const int ntile = gemm_tn_tiles(M, N, symm);
-   const long pairs = (long)ntile * splits;
+   long pairs, poff = 0;
+   size_t kchunk, kchunk_dd = 0;
+   if (symm) {
+      int so, sd;
+      gemm_tn_symm_splits(M, K, splits, &so, &sd);
+      const int noff = ntm * (ntm - 1) / 2, ndd = (ntm + 1) / 2;
+      poff = (long)noff * so;
+      pairs = poff + (long)ndd * sd;
+      kchunk = so > 0 ? chunk(so) : 0;
+      kchunk_dd = chunk(sd);
+   } else {
+      pairs = (long)ntile * splits;
+      kchunk = chunk(splits);
+   }
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
    if (tmr == 96)
       hipLaunchKernelGGL((k_gemm_tn_splitk<96>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
-                         ldb, N, w, K, kchunk, symm, ntm, ntile, per, partial);
+                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, partial);
    else
       hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
-                         ldb, N, w, K, kchunk, symm, ntm, ntile, per, partial);
+                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, partial);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
 
-/* out[m][n] = sum_z partial[z][m][n]; with symm > 0 only the blocks of
- * edge symm with block(m) <= block(n) exist and the rest is mirrored.  A block covers 64
- * consecutive outputs with four thread groups: group g sums the splits
- * z = g mod 4 in two interleaved chains, and the four group sums are added
- * in g order, so the result does not depend on scheduling. */
+/* out[m][n] = sum_z partial[z][m][n] over the splits that formed (m, n):
+ * `splits`, or for a symmetric product (symm = block edge of the formed
+ * blocks) sp_dd inside the 128 x 128 diagonal tiles and `splits` elsewhere;
+ * only blocks with block(m) <= block(n) exist and the rest is mirrored.  A
+ * workgroup covers 64 consecutive outputs with four thread groups: group g
+ * sums the splits z = g mod 4 in two interleaved chains, and the four group
+ * sums are added in g order, so the result does not depend on scheduling. */
 __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restrict__ partial, int splits,
-                                                         int M, int N, int symm, double *__restrict__ out)
+                                                         int sp_dd, int M, int N, int symm,
+                                                         double *__restrict__ out)
 {
    __shared__ double red[4][64];
    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -372,16 +419,17 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
    const int m = (int)(idx / N), n = (int)(idx % N);
    /* symmetric: the blocks below the diagonal are written from their mirror */
    const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm));
+   const int nz = (symm && m / TT == n / TT) ? sp_dd : splits;
    double s0 = 0.0, s1 = 0.0;
    if (valid) {
       const size_t stride = (size_t)M * N;
       const double *p = partial + (size_t)m * N + n;
       int z = g;
-      for (; z + 4 < splits; z += 8) {
+      for (; z + 4 < nz; z += 8) {
          s0 += p[(size_t)z * stride];
          s1 += p[(size_t)(z + 4) * stride];
       }
-      if (z < splits) s0 += p[(size_t)z * stride];
+      if (z < nz) s0 += p[(size_t)z * stride];
    }
    red[g][l] = s0 + s1;
    __syncthreads();
@@ -397,7 +445,19 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
 {
    const size_t tot = (size_t)M * N;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
-                      splits, M, N, symm, out);
+                      splits, splits, M, N, symm, out);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_reduce_partials_symm(const double *partial, int M, size_t K, int splits, double *out,
+                                hipStream_t s)
+{
+   int so, sd;
+   gemm_tn_symm_splits(M, K, splits, &so, &sd);
+   const size_t tot = (size_t)M * M;
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
+                      so, sd, M, M, 16, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
