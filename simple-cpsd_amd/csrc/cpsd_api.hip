@@ -113,6 +113,7 @@ struct cpsd_dev {
     * Cartesian G, first projector of each ion, force partials; host copies
     * of katm and the per-kind masses for dt/sqrt(M) */
    int move;
+   int struct_valid;      /* structure factors and projectors match r1 */
    double *r2d, *fion, *dti, *Gp, *fgpart, *fnlpart, *fnlsum;
    int *poff, fchunk;
    int *katm_h; double *amass_h;
@@ -673,10 +674,14 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
       for (int i = 0; i < c->nion; ++i) dti[i] = dt / sqrt(c->amass_h[c->katm_h[i]]);
       TRY(h2d(c->dti, dti.data(), c->nion, st));
       CPSD_CHECK(hipStreamSynchronize(st));
-   } else {
+   } else if (!c->struct_valid) {
+      /* fixed ions: the structure factors and projectors hold until the
+       * positions change */
       TRY(run_structure(c));
+      c->struct_valid = 1;
    }
    for (int it = 0; it < inner; ++it) TRY(sd_step(c, dte, it == inner - 1));
+   if (c->move) c->struct_valid = 0;
    TRY(energy_tail(c, inner, dt, dte));
    double out[12], info[2];
    int err = 0;
@@ -733,6 +738,7 @@ static int ctx_set_ions(cpsd_dev *c, int move, const double *amass, const double
    }
    c->move = move;
    TRY(h2d(c->r1, r2, n3i, st));
+   c->struct_valid = 0;
    CPSD_CHECK(hipStreamSynchronize(st));
    return 0;
 }
