@@ -16,6 +16,12 @@ library's stream around each phase during a second pass of min(--steps, 100)
 steps right after it (the events cost ~2% of a step, so they stay out of
 `value`).  A run from the same start measures the time until
 |E(k) - E(k-1)| < 1e-7 Ha between energy evaluations (10 steps apart).
+
+The "mixed" object repeats the steps/s, phase and time-to-converge legs in
+the FP32-transform mode (cpsd_dev_set_precision(h, 1): the orbital FFTs of
+9 of every 10 steps on FP32 values, everything else and every energy
+evaluation FP64) and reports its converged energy's error against the FP64
+run's, per atom.  `value` stays the FP64 rate.
 """
 import argparse
 import json
@@ -127,8 +133,8 @@ def run(args):
                     eig_max=float(ep["eig"][0]), eig_min=float(ep["eig"][-1]))
 
     # time to |dE| < 1e-7 Ha between energy evaluations, from the same start
-    conv = None
-    if args.converge_steps > 0:
+    def converge(mode):
+        dev.set_precision(mode)
         dev.set_orbitals(psi)
         dev.E[:] = 0.0
         t0 = time.perf_counter()
@@ -141,13 +147,44 @@ def run(args):
                 done = True
                 break
             if nsteps % 1000 == 0 and Eprev is not None:
-                print("converge: %d steps, E %.10f, dE %.3e" % (nsteps, Ec[0], Ec[0] - Eprev),
+                print("converge(%s): %d steps, E %.10f, dE %.3e" %
+                      ("mixed" if mode else "fp64", nsteps, Ec[0], Ec[0] - Eprev),
                       file=sys.stderr, flush=True)
             last = None if Eprev is None else Ec[0] - Eprev
             Eprev = Ec[0]
         tconv = time.perf_counter() - t0
-        conv = dict(converged=done, steps=nsteps, seconds=round(tconv, 4),
+        dev.set_precision(0)
+        return dict(converged=done, steps=nsteps, seconds=round(tconv, 4),
                     E=float(Ec[0]), last_dE=None if last is None else float(last))
+
+    conv = converge(0) if args.converge_steps > 0 else None
+
+    # the FP32-transform mode: steps/s, phases, time to converge and the
+    # converged energy's error against FP64
+    mixed = None
+    if args.mixed:
+        dev.set_orbitals(psi)
+        dev.set_precision(1)
+        steps(args.warmup)
+        t0 = time.perf_counter()
+        Em, _, _ = steps(args.steps)
+        el_m = time.perf_counter() - t0
+        dev.profile(True)
+        steps(prof_steps)
+        prof_m = dev.profile_read()
+        dev.profile(False)
+        dev.set_precision(0)
+        mixed = dict(value=round(args.steps / el_m, 3), unit="SD steps/s",
+                     ms_per_step=round(el_m * 1000.0 / args.steps, 4),
+                     speedup_vs_fp64=round(elapsed / el_m, 4),
+                     fp32_steps_per_call=inner - 1, E_last=float(Em[0]),
+                     phases={k: round(t / prof_steps, 4) for k, (t, c) in prof_m.items() if c > 0})
+        if args.converge_steps > 0:
+            cm = converge(1)
+            if conv is not None and conv["converged"] and cm["converged"]:
+                cm["abs_E_error_vs_fp64_per_atom"] = abs(cm["E"] - conv["E"]) / (3 * w["nmol"])
+                cm["speedup_vs_fp64"] = round(conv["seconds"] / cm["seconds"], 4)
+            mixed["time_to_converge"] = cm
     dev.close()
 
     ms = elapsed * 1000.0 / args.steps
@@ -229,6 +266,7 @@ def run(args):
                    "projectors": sizes["nproj"], "masked_G": sizes["ng"], "inner": inner,
                    "precision": "fp64", "start": "gaussian", "setup_s": round(t_setup, 2)},
         "time_to_converge": conv,
+        "mixed": mixed,
         "profiled_pass": {"steps": prof_steps, "ms_per_step": round(elapsed_prof * 1000.0 / prof_steps, 4)},
         "lagrange_iterations_last_step": lag_it,
         "epilogue": epilogue,
@@ -249,6 +287,7 @@ def main():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--converge-steps", type=int, default=10000)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    p.add_argument("--no-mixed", dest="mixed", action="store_false")
     p.add_argument("--cpu-sample", type=int, default=16)
     args = p.parse_args()
     if args.gpus != 1:

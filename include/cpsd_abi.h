@@ -110,6 +110,17 @@ int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn, 
  * eigenpairs of hml as cpsd_rotate() returns them, and the device's c1
  * replaced by c2 V (read it with cpsd_dev_get); any output may be NULL */
 int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
+/* precision of the orbital transforms: mode 0 runs every step in FP64 (the
+ * default, the reference's float_double.h build); mode 1 runs the 3D FFTs of
+ * the orbitals (G->r, the density sum, V psi and r->G) on FP32 values in
+ * every step of a cpsd_dev_run() call but its last, while the density
+ * accumulation, the potentials, the projector, overlap and constraint
+ * products, the Lagrange iteration and the energies stay FP64.  The last
+ * step, whose density and -H psi the energies are formed from, is FP64, so
+ * E, deltae and deltac are FP64 values of the current orbitals.  The
+ * reference has no such mode; its all-FP32 build (float_single.h with
+ * sfftpack3d) is the nearest. */
+int cpsd_dev_set_precision(cpsd_dev *h, int mode);
 /* last step's Lagrange iteration count and final max|dX| */
 int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff);
 /* sizes: [0] ng (masked G vectors), [1] ngp, [2] nproj */

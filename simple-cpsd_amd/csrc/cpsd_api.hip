@@ -98,7 +98,10 @@ struct cpsd_dev {
    int cur;               /* C2 lives in block cur */
    double2 *T, *Tv;
    double *dnpart, *dn, *xcp, *xce, *V;
-   int ndnblk, nob, fb;
+   int ndnblk[2], nob[2], fb;   /* density blocks of the FP64 [0] and FP32 [1] pass */
+   /* precision of the orbital transforms of every step but the last of a
+    * call: 0 FP64, 1 FP32 (the last step, which feeds the energy, is FP64) */
+   int mixed;
    double2 *dng, *s, *vall, *B;
    double *partial; size_t partial_cap;
    double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
@@ -361,12 +364,13 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    /* orbital blocks of the density pass (each sums its orbitals' psi^2 into
     * one partial plane set): the fewest blocks whose n x blocks workgroups
     * fill at least 90% of whole rounds of the GPU's workgroup slots, else
-    * the best filling up to 32 blocks; CPSD_DN_BLOCKS sets the count */
-   {
-      int dev = 0, cus = 0;
-      CPSD_CHECK(hipGetDevice(&dev));
-      CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      const long slots = (long)(cus > 0 ? cus : 1) * plane_density_occupancy(n);
+    * the best filling up to 32 blocks; CPSD_DN_BLOCKS sets the count.  The
+    * FP32 pass holds more workgroups per CU and is sized on its own. */
+   int dev = 0, cus = 0;
+   CPSD_CHECK(hipGetDevice(&dev));
+   CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+   for (int f = 0; f < 2; ++f) {
+      const long slots = (long)(cus > 0 ? cus : 1) * plane_density_occupancy(n, f);
       int best = 1;
       double beff = -1.0;
       for (int d = 1; d <= (c->fb < 32 ? c->fb : 32); ++d) {
@@ -378,10 +382,10 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
       }
       const char *env = getenv("CPSD_DN_BLOCKS");   /* tuning override */
       if (env && atoi(env) > 0) best = atoi(env) < c->fb ? atoi(env) : c->fb;
-      c->nob = (c->fb + best - 1) / best;
-      c->ndnblk = (c->fb + c->nob - 1) / c->nob;
+      c->nob[f] = (c->fb + best - 1) / best;
+      c->ndnblk[f] = (c->fb + c->nob[f] - 1) / c->nob[f];
    }
-   TRY(dalloc(&c->dnpart, (size_t)c->ndnblk * c->n3));
+   TRY(dalloc(&c->dnpart, (size_t)(c->ndnblk[0] > c->ndnblk[1] ? c->ndnblk[0] : c->ndnblk[1]) * c->n3));
    TRY(dalloc(&c->dn, c->n3)); TRY(dalloc(&c->xcp, c->n3)); TRY(dalloc(&c->xce, c->n3));
    TRY(dalloc(&c->V, c->n3));
    TRY(dalloc(&c->dng, ngp)); TRY(dalloc(&c->s, ngp)); TRY(dalloc(&c->vall, ngp));
@@ -475,6 +479,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    ZArgs z;
    PlaneArgs pa;
    const size_t plane = (size_t)n * n * c->nh;
+   const int f32 = c->mixed && !last;
    /* 2-3: orbitals in batches of fb, so that each batch's z-transformed
     * planes are read by the density pass while they sit in the Infinity
     * Cache; T keeps every orbital for the V*psi pass */
@@ -483,18 +488,20 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(ph_begin(c, PH_ZG2R));
       zargs_base(c, z, 0);
       z.src = C1 + (size_t)b0 * ngp; z.sstride = ngp; z.T = c->T + (size_t)b0 * plane;
+      z.fp32 = f32;
       TRY(launch_zpass_g2r(n, SRC_PACKED, z, bs, st));
       TRY(ph_end(c, PH_ZG2R));
       TRY(ph_begin(c, PH_DENSITY));
       memset(&pa, 0, sizeof pa);
       pa.T = c->T + (size_t)b0 * plane; pa.nhload = c->nhc; pa.tw = c->tw; pa.b0 = 0;
-      pa.nb = c->nob; pa.nbatch = bs; pa.dnpart = c->dnpart; pa.accumulate = b0 > 0;
-      TRY(launch_plane(n, PM_DENSITY, pa, (bs + c->nob - 1) / c->nob, st));
+      pa.nb = c->nob[f32]; pa.nbatch = bs; pa.dnpart = c->dnpart; pa.accumulate = b0 > 0;
+      pa.fp32 = f32;
+      TRY(launch_plane(n, PM_DENSITY, pa, (bs + c->nob[f32] - 1) / c->nob[f32], st));
       TRY(ph_end(c, PH_DENSITY));
    }
    /* 4 */
    TRY(ph_begin(c, PH_DNRED));
-   TRY(sd_dn_reduce_xc(c->dnpart, c->ndnblk, c->n3, scal2, c->dn, c->xcp, c->xce, st));
+   TRY(sd_dn_reduce_xc(c->dnpart, c->ndnblk[f32], c->n3, scal2, c->dn, c->xcp, c->xce, st));
    TRY(ph_end(c, PH_DNRED));
    /* 5 */
    TRY(ph_begin(c, PH_DNG));
@@ -530,7 +537,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(ph_begin(c, PH_VPSI));
       memset(&pa, 0, sizeof pa);
       pa.T = c->T + (size_t)b0 * plane; pa.nhload = c->nhc; pa.tw = c->tw; pa.V = c->V;
-      pa.nbatch = bs;
+      pa.nbatch = bs; pa.fp32 = f32;
       TRY(launch_plane(n, PM_VPSI, pa, bs, st));
       TRY(ph_end(c, PH_VPSI));
       TRY(ph_begin(c, PH_ZR2G));
@@ -541,6 +548,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       z.T = c->T + (size_t)b0 * plane; z.ostride = ngp;
       z.out = (last ? c->CP : C2) + (size_t)b0 * ngp;
       z.c1 = C1 + (size_t)b0 * ngp; z.tg = c->tg; z.scal1 = scal1; z.dte = dte;
+      z.fp32 = f32;
       TRY(launch_zpass_r2g(n, last ? DST_HPSI : DST_SD, z, bs, st));
       TRY(ph_end(c, PH_ZR2G));
    }
@@ -897,6 +905,17 @@ extern "C" int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double
 {
    if (!h) { cpsd_set_error("null handle"); return -1; }
    return ctx_epilogue(h, ortho, eig, V);
+}
+
+extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   if (mode != 0 && mode != 1) {
+      cpsd_set_error("cpsd_dev_set_precision: mode %d is not 0 (FP64) or 1 (FP32 orbital FFTs)", mode);
+      return -1;
+   }
+   h->mixed = mode;
+   return 0;
 }
 
 extern "C" int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff)

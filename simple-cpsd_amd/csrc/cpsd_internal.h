@@ -65,6 +65,9 @@ struct ZArgs {
    /* set by the launcher: orbitals per workgroup (blockIdx.z covers
     * [z*bper, min(nbatch, (z+1)*bper))) */
    int nbatch, bper;
+   /* FP32 transform (SRC_PACKED, DST_HPSI, DST_SD): T holds float2 at the
+    * same base pointer; inputs and outputs stay FP64 */
+   int fp32;
 };
 
 struct PlaneArgs {
@@ -82,6 +85,7 @@ struct PlaneArgs {
    double scal1;
    const double *rin;    /* PM_R2C_IN: padded real input [z][y][N+2] */
    double *rout;         /* PM_C2R_OUT: padded real output */
+   int fp32;             /* PM_DENSITY, PM_VPSI: float2 planes, FP64 sums and V */
 };
 
 int fft_supported(int n);
@@ -89,7 +93,7 @@ int launch_zpass_g2r(int n, int src, const ZArgs &a, int nbatch, hipStream_t s);
 int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s);
 int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s);
 /* workgroups of the density plane pass one CU holds at once */
-int plane_density_occupancy(int n);
+int plane_density_occupancy(int n, int fp32);
 
 /* ---- GEMM ------------------------------------------------------------- */
 

@@ -1,5 +1,6 @@
 /*
- * fft_core.h -- in-register and LDS-resident DFT building blocks (FP64).
+ * fft_core.h -- in-register and LDS-resident DFT building blocks, FP64
+ * (double2) or FP32 (float2).
  *
  * Conventions (those of fftpack3d/fftpack3d.f and FFTPACK):
  *   DIR = -1 : X_k = sum_n x_n exp(-2 pi i nk/N)   (forward, rc_fft3f)
@@ -19,41 +20,63 @@
 #include <hip/hip_runtime.h>
 #include "small_twiddles.h"
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+/* complex helpers for the FP64 (double2) and FP32 (float2) transforms */
+template<typename V> struct Cx;
+template<> struct Cx<double2> {
+   using R = double;
+   static __device__ __forceinline__ double2 mk(double x, double y) { return make_double2(x, y); }
+};
+template<> struct Cx<float2> {
+   using R = float;
+   static __device__ __forceinline__ float2 mk(float x, float y) { return make_float2(x, y); }
+};
+
+/* storage conversions: FP64 tables and orbitals in, FP64 results out */
+template<typename V> __device__ __forceinline__ V cvt(double2 a);
+template<> __device__ __forceinline__ double2 cvt<double2>(double2 a) { return a; }
+template<> __device__ __forceinline__ float2 cvt<float2>(double2 a)
 {
-   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+   return make_float2((float)a.x, (float)a.y);
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2(a.x, a.y); }
+
+template<typename V> __device__ __forceinline__ V cmul(V a, V b)
+{
+   return Cx<V>::mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template<typename V> __device__ __forceinline__ V cadd(V a, V b) { return Cx<V>::mk(a.x + b.x, a.y + b.y); }
+template<typename V> __device__ __forceinline__ V csub(V a, V b) { return Cx<V>::mk(a.x - b.x, a.y - b.y); }
+template<typename V> __device__ __forceinline__ V cconj(V a) { return Cx<V>::mk(a.x, -a.y); }
 /* i * a */
-__device__ __forceinline__ double2 cmuli(double2 a) { return make_double2(-a.y, a.x); }
-__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+template<typename V> __device__ __forceinline__ V cmuli(V a) { return Cx<V>::mk(-a.y, a.x); }
+template<typename V> __device__ __forceinline__ V cscale(V a, typename Cx<V>::R s) { return Cx<V>::mk(a.x * s, a.y * s); }
 
 /* ---------------------------------------------------------------- DFTs */
 
 template<int L, int DIR> struct DFT;
 
 template<int DIR> struct DFT<1, DIR> {
-   static __device__ __forceinline__ void run(double2 *) {}
+   template<typename V> static __device__ __forceinline__ void run(V *) {}
 };
 
 template<int DIR> struct DFT<2, DIR> {
-   static __device__ __forceinline__ void run(double2 *v)
+   template<typename V> static __device__ __forceinline__ void run(V *v)
    {
-      double2 a = v[0], b = v[1];
+      V a = v[0], b = v[1];
       v[0] = cadd(a, b);
       v[1] = csub(a, b);
    }
 };
 
 template<int DIR> struct DFT<3, DIR> {
-   static __device__ __forceinline__ void run(double2 *v)
+   template<typename V> static __device__ __forceinline__ void run(V *v)
    {
-      const double s3 = 0.8660254037844386 * DIR;
-      double2 a = v[0], t1 = cadd(v[1], v[2]);
-      double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
-      double2 t3 = cscale(csub(v[1], v[2]), s3);
+      using R = typename Cx<V>::R;
+      const R s3 = R(0.8660254037844386) * DIR;
+      V a = v[0], t1 = cadd(v[1], v[2]);
+      V t2 = Cx<V>::mk(a.x - R(0.5) * t1.x, a.y - R(0.5) * t1.y);
+      V t3 = cscale(csub(v[1], v[2]), s3);
       v[0] = cadd(a, t1);
       v[1] = cadd(t2, cmuli(t3));
       v[2] = csub(t2, cmuli(t3));
@@ -61,12 +84,12 @@ template<int DIR> struct DFT<3, DIR> {
 };
 
 template<int DIR> struct DFT<4, DIR> {
-   static __device__ __forceinline__ void run(double2 *v)
+   template<typename V> static __device__ __forceinline__ void run(V *v)
    {
-      double2 ac = cadd(v[0], v[2]), amc = csub(v[0], v[2]);
-      double2 bd = cadd(v[1], v[3]), bmd = csub(v[1], v[3]);
-      double2 j = cmuli(bmd);
-      if (DIR < 0) j = make_double2(-j.x, -j.y);
+      V ac = cadd(v[0], v[2]), amc = csub(v[0], v[2]);
+      V bd = cadd(v[1], v[3]), bmd = csub(v[1], v[3]);
+      V j = cmuli(bmd);
+      if (DIR < 0) j = Cx<V>::mk(-j.x, -j.y);
       v[0] = cadd(ac, bd);
       v[2] = csub(ac, bd);
       v[1] = cadd(amc, j);
@@ -75,17 +98,18 @@ template<int DIR> struct DFT<4, DIR> {
 };
 
 template<int DIR> struct DFT<5, DIR> {
-   static __device__ __forceinline__ void run(double2 *v)
+   template<typename V> static __device__ __forceinline__ void run(V *v)
    {
-      const double c1 = 0.30901699437494745, c2 = -0.8090169943749475;
-      const double s1 = 0.9510565162951535 * DIR, s2 = 0.5877852522924731 * DIR;
-      double2 a = v[0];
-      double2 be = cadd(v[1], v[4]), bme = csub(v[1], v[4]);
-      double2 cd = cadd(v[2], v[3]), cmd = csub(v[2], v[3]);
-      double2 r1 = make_double2(a.x + c1 * be.x + c2 * cd.x, a.y + c1 * be.y + c2 * cd.y);
-      double2 r2 = make_double2(a.x + c2 * be.x + c1 * cd.x, a.y + c2 * be.y + c1 * cd.y);
-      double2 i1 = cmuli(make_double2(s1 * bme.x + s2 * cmd.x, s1 * bme.y + s2 * cmd.y));
-      double2 i2 = cmuli(make_double2(s2 * bme.x - s1 * cmd.x, s2 * bme.y - s1 * cmd.y));
+      using R = typename Cx<V>::R;
+      const R c1 = R(0.30901699437494745), c2 = R(-0.8090169943749475);
+      const R s1 = R(0.9510565162951535) * DIR, s2 = R(0.5877852522924731) * DIR;
+      V a = v[0];
+      V be = cadd(v[1], v[4]), bme = csub(v[1], v[4]);
+      V cd = cadd(v[2], v[3]), cmd = csub(v[2], v[3]);
+      V r1 = Cx<V>::mk(a.x + c1 * be.x + c2 * cd.x, a.y + c1 * be.y + c2 * cd.y);
+      V r2 = Cx<V>::mk(a.x + c2 * be.x + c1 * cd.x, a.y + c2 * be.y + c1 * cd.y);
+      V i1 = cmuli(Cx<V>::mk(s1 * bme.x + s2 * cmd.x, s1 * bme.y + s2 * cmd.y));
+      V i2 = cmuli(Cx<V>::mk(s2 * bme.x - s1 * cmd.x, s2 * bme.y - s1 * cmd.y));
       v[0] = cadd(a, cadd(be, cd));
       v[1] = cadd(r1, i1);
       v[4] = csub(r1, i1);
@@ -95,14 +119,15 @@ template<int DIR> struct DFT<5, DIR> {
 };
 
 /* composite L = A*B in registers: n = B*a + b, k = ka + A*kb */
-template<int A, int B, int DIR>
-__device__ __forceinline__ void dft_composite(double2 *v)
+template<int A, int B, int DIR, typename V>
+__device__ __forceinline__ void dft_composite(V *v)
 {
+   using R = typename Cx<V>::R;
    constexpr int L = A * B;
-   double2 u[L];
+   V u[L];
 #pragma unroll
    for (int b = 0; b < B; ++b) {
-      double2 t[A];
+      V t[A];
 #pragma unroll
       for (int a = 0; a < A; ++a) t[a] = v[B * a + b];
       DFT<A, DIR>::run(t);
@@ -112,14 +137,14 @@ __device__ __forceinline__ void dft_composite(double2 *v)
             u[b * A + ka] = t[ka];
          } else {
             const int m = (b * ka) % L;
-            double2 w = make_double2(SmallTw<L>::re(m), DIR * SmallTw<L>::im(m));
+            V w = Cx<V>::mk(R(SmallTw<L>::re(m)), R(DIR * SmallTw<L>::im(m)));
             u[b * A + ka] = cmul(t[ka], w);
          }
       }
    }
 #pragma unroll
    for (int ka = 0; ka < A; ++ka) {
-      double2 t[B];
+      V t[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) t[b] = u[b * A + ka];
       DFT<B, DIR>::run(t);
@@ -128,17 +153,17 @@ __device__ __forceinline__ void dft_composite(double2 *v)
    }
 }
 
-template<int DIR> struct DFT<6, DIR>  { static __device__ __forceinline__ void run(double2 *v) { dft_composite<3, 2, DIR>(v); } };
-template<int DIR> struct DFT<8, DIR>  { static __device__ __forceinline__ void run(double2 *v) { dft_composite<4, 2, DIR>(v); } };
-template<int DIR> struct DFT<10, DIR> { static __device__ __forceinline__ void run(double2 *v) { dft_composite<5, 2, DIR>(v); } };
-template<int DIR> struct DFT<12, DIR> { static __device__ __forceinline__ void run(double2 *v) { dft_composite<4, 3, DIR>(v); } };
-template<int DIR> struct DFT<16, DIR> { static __device__ __forceinline__ void run(double2 *v) { dft_composite<4, 4, DIR>(v); } };
+template<int DIR> struct DFT<6, DIR>  { template<typename V> static __device__ __forceinline__ void run(V *v) { dft_composite<3, 2, DIR>(v); } };
+template<int DIR> struct DFT<8, DIR>  { template<typename V> static __device__ __forceinline__ void run(V *v) { dft_composite<4, 2, DIR>(v); } };
+template<int DIR> struct DFT<10, DIR> { template<typename V> static __device__ __forceinline__ void run(V *v) { dft_composite<5, 2, DIR>(v); } };
+template<int DIR> struct DFT<12, DIR> { template<typename V> static __device__ __forceinline__ void run(V *v) { dft_composite<4, 3, DIR>(v); } };
+template<int DIR> struct DFT<16, DIR> { template<typename V> static __device__ __forceinline__ void run(V *v) { dft_composite<4, 4, DIR>(v); } };
 
 /* twiddle exp(DIR 2 pi i m / N) from a table of exp(+2 pi i m / N), m < N */
-template<int DIR>
-__device__ __forceinline__ double2 twiddle(const double2 *__restrict__ tw, int m)
+template<int DIR, typename V>
+__device__ __forceinline__ V twiddle(const V *__restrict__ tw, int m)
 {
-   double2 w = tw[m];
+   V w = tw[m];
    if (DIR < 0) w.y = -w.y;
    return w;
 }
@@ -153,15 +178,15 @@ __device__ __forceinline__ int tr_slot(int j) { return N2 * (j % N1) + j / N1; }
 
 /* count transforms; element j of transform f sits at buf[f*fs + j*es].
  * tw: table of exp(+2 pi i m / (N*tws)), twiddle W_N^m read at m*tws.  */
-template<int N1, int N2, int DIR>
-__device__ __forceinline__ void lds_fft_nat2tr(double2 *buf, int count, int fs, int es,
-                                               const double2 *__restrict__ tw, int tws)
+template<int N1, int N2, int DIR, typename V>
+__device__ __forceinline__ void lds_fft_nat2tr(V *buf, int count, int fs, int es,
+                                               const V *__restrict__ tw, int tws)
 {
    constexpr int N = N1 * N2;
    for (int it = threadIdx.x; it < count * N2; it += blockDim.x) {
       const int f = it % count, n2 = it / count;
-      double2 *p = buf + f * fs;
-      double2 v[N1];
+      V *p = buf + f * fs;
+      V v[N1];
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1) v[n1] = p[(N2 * n1 + n2) * es];
       DFT<N1, DIR>::run(v);
@@ -173,8 +198,8 @@ __device__ __forceinline__ void lds_fft_nat2tr(double2 *buf, int count, int fs, 
    __syncthreads();
    for (int it = threadIdx.x; it < count * N1; it += blockDim.x) {
       const int f = it % count, k1 = it / count;
-      double2 *p = buf + f * fs;
-      double2 v[N2];
+      V *p = buf + f * fs;
+      V v[N2];
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) v[n2] = p[(N2 * k1 + n2) * es];
       DFT<N2, DIR>::run(v);
@@ -184,15 +209,15 @@ __device__ __forceinline__ void lds_fft_nat2tr(double2 *buf, int count, int fs, 
    __syncthreads();
 }
 
-template<int N1, int N2, int DIR>
-__device__ __forceinline__ void lds_fft_tr2nat(double2 *buf, int count, int fs, int es,
-                                               const double2 *__restrict__ tw, int tws)
+template<int N1, int N2, int DIR, typename V>
+__device__ __forceinline__ void lds_fft_tr2nat(V *buf, int count, int fs, int es,
+                                               const V *__restrict__ tw, int tws)
 {
    constexpr int N = N1 * N2;
    for (int it = threadIdx.x; it < count * N1; it += blockDim.x) {
       const int f = it % count, n1 = it / count;
-      double2 *p = buf + f * fs;
-      double2 v[N2];
+      V *p = buf + f * fs;
+      V v[N2];
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) v[n2] = p[(N2 * n1 + n2) * es];
       DFT<N2, DIR>::run(v);
@@ -204,8 +229,8 @@ __device__ __forceinline__ void lds_fft_tr2nat(double2 *buf, int count, int fs, 
    __syncthreads();
    for (int it = threadIdx.x; it < count * N2; it += blockDim.x) {
       const int f = it % count, k2 = it / count;
-      double2 *p = buf + f * fs;
-      double2 v[N1];
+      V *p = buf + f * fs;
+      V v[N1];
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1) v[n1] = p[(N2 * n1 + k2) * es];
       DFT<N1, DIR>::run(v);

@@ -1,6 +1,8 @@
 /*
  * fft_kernels.hip -- batched 3D real<->half-complex FFTs of the SD step on
- * gfx950, FP64.
+ * gfx950, FP64, with FP32 variants of the orbital passes (the transforms
+ * run on float2 in LDS and T; inputs, the density sum and the outputs stay
+ * FP64).
  *
  * The 3D transform of cr_fft3b / rc_fft3f (fftpack3d/fftpack3d.f:22-96,
  * 107-196) is split into
@@ -20,6 +22,21 @@
  */
 #include "cpsd_internal.h"
 #include "fft_core.h"
+
+/* waves per SIMD (the second __launch_bounds__ argument) the FP32 orbital
+ * kernels are compiled for (the plane kernels at N >= 80 only); their
+ * float2 tiles let LDS hold twice the FP64 kernels' workgroups per CU.  At
+ * N = 96, 6 waves (3 workgroups per CU) speed up the V psi pass by 6%; the
+ * density and z passes spill at 6 and run slower there. */
+#ifndef DENS_WAVES_F32
+#define DENS_WAVES_F32 4
+#endif
+#ifndef VPSI_WAVES_F32
+#define VPSI_WAVES_F32 6
+#endif
+#ifndef ZPASS_WAVES_F32
+#define ZPASS_WAVES_F32 4
+#endif
 
 /* ------------------------------------------------------------ z passes */
 
@@ -41,20 +58,20 @@ __device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*r
  * orbital b+1 is issued into the registers it is loaded into while orbital
  * b is transformed and written, so each workgroup keeps one column block of
  * loads in flight */
-template<int N, int SRC>
-__global__ void __launch_bounds__(GridCfg<N>::ZT, 4) k_zpass_g2r(ZArgs a)
+template<int N, int SRC, typename V>
+__global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F32 : 4) k_zpass_g2r(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;
    constexpr int EPT = (N * KXB + ZT - 1) / ZT;   /* elements per thread */
-   __shared__ double2 sm[N * KXB];
+   __shared__ V sm[N * KXB];
    __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y;
    const int b0 = blockIdx.z * a.bper, b1 = min(a.nbatch, b0 + a.bper);
    const int ncol = min(KXB, a.rows.nhc - kx0);
    /* twiddles are read from LDS (the barrier in stage_rows publishes them) */
-   __shared__ double2 tws[N];
-   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
+   __shared__ V tws[N];
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = cvt<V>(a.tw[m]);
    stage_rows<N>(a.rows, ky, rt);
 
    /* gather positions, the same for every orbital of the row */
@@ -69,21 +86,22 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT, 4) k_zpass_g2r(ZArgs a)
          if (kx >= rs && kx < rs + rl) pe[q] = rt[2][kz] + kx - rs;
       }
    }
-   double2 v[EPT];
+   V v[EPT];
    auto gather = [&](int b) {
       if (SRC == SRC_PACKED) {
          const double2 *src = a.src + (size_t)b * a.sstride;
 #pragma unroll
-         for (int q = 0; q < EPT; ++q) v[q] = pe[q] >= 0 ? src[pe[q]] : make_double2(0.0, 0.0);
+         for (int q = 0; q < EPT; ++q) v[q] = cvt<V>(pe[q] >= 0 ? src[pe[q]] : make_double2(0.0, 0.0));
       } else {
 #pragma unroll
          for (int q = 0; q < EPT; ++q) {
-            v[q] = make_double2(0.0, 0.0);
+            double2 f = make_double2(0.0, 0.0);
             if (pe[q] >= 0) {
                const double2 vl = a.vall[pe[q]], dg = a.dng[pe[q]];
                const double c = a.vc[pe[q]];
-               v[q] = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
+               f = make_double2(vl.x * a.scal2 + dg.x * c, vl.y * a.scal2 + dg.y * c);
             }
+            v[q] = cvt<V>(f);
          }
       }
    };
@@ -97,7 +115,7 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT, 4) k_zpass_g2r(ZArgs a)
       __syncthreads();
       if (SRC == SRC_PACKED && b + 1 < b1) gather(b + 1);   /* the field pass runs one row */
       lds_fft_nat2tr<C::NY1, C::NY2, +1>(sm, ncol, 1, KXB, tws, 1);
-      double2 *T = a.T + (size_t)b * N * N * NH;
+      V *T = reinterpret_cast<V *>(a.T) + (size_t)b * N * N * NH;
       for (int e = threadIdx.x; e < N * KXB; e += blockDim.x) {
          const int s = e / KXB, kxl = e % KXB;
          if (kxl < ncol) {
@@ -109,28 +127,28 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT, 4) k_zpass_g2r(ZArgs a)
    }
 }
 
-template<int N, int DST>
-__global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
+template<int N, int DST, typename V>
+__global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F32 : 1) k_zpass_r2g(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;
    constexpr int EPT = (N * KXB + ZT - 1) / ZT;
-   __shared__ double2 sm[N * KXB];
+   __shared__ V sm[N * KXB];
    __shared__ int rt[3][N];
    const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
    const int ncol = min(KXB, a.rows.nhc - kx0);
-   const double2 *T = a.T + (size_t)b * N * N * NH;
-   __shared__ double2 tws[N];
-   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
+   const V *T = reinterpret_cast<const V *>(a.T) + (size_t)b * N * N * NH;
+   __shared__ V tws[N];
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = cvt<V>(a.tw[m]);
    stage_rows<N>(a.rows, ky, rt);
 
    {
-      double2 v[EPT];
+      V v[EPT];
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
          const int e = threadIdx.x + ZT * q;
          const int s = e / KXB, kxl = e % KXB;
-         v[q] = make_double2(0.0, 0.0);
+         v[q] = cvt<V>(make_double2(0.0, 0.0));
          if (e < N * KXB && kxl < ncol) {
             const int z = tr_index<C::NY1, C::NY2>(s);
             v[q] = T[((size_t)z * N + ky) * NH + kx0 + kxl];
@@ -169,7 +187,7 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
          if (pe[q] >= 0) {
-            const double2 v = sm[threadIdx.x + ZT * q];
+            const double2 v = to_d2(sm[threadIdx.x + ZT * q]);
             const double2 h = make_double2(-a.scal1 * v.x - t[q] * c[q].x, -a.scal1 * v.y - t[q] * c[q].y);
             out[pe[q]] = DST == DST_SD ? make_double2(c[q].x + a.dte * h.x, c[q].y + a.dte * h.y) : h;
          }
@@ -177,7 +195,7 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
    } else {
 #pragma unroll
       for (int q = 0; q < EPT; ++q)
-         if (pe[q] >= 0) out[pe[q]] = sm[threadIdx.x + ZT * q];
+         if (pe[q] >= 0) out[pe[q]] = to_d2(sm[threadIdx.x + ZT * q]);
    }
 }
 
@@ -188,29 +206,30 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT) k_zpass_r2g(ZArgs a)
  * The pairs (k, M-k) with 0 < k < M/2 run in one loop and the self-paired
  * columns k = 0 and k = M/2 in a second one (they touch other elements, so
  * no barrier is needed between them): every wave then takes one path. */
-template<int N>
-__device__ __forceinline__ void c2r_pre(double2 *sm, const double2 *__restrict__ tw)
+template<int N, typename V>
+__device__ __forceinline__ void c2r_pre(V *sm, const V *__restrict__ tw)
 {
+   using R = typename Cx<V>::R;
    constexpr int NH = N / 2 + 1, M = N / 2, KG = M / 2 - 1;
    static_assert(M % 2 == 0, "N/2 must be even");
    for (int it = threadIdx.x; it < N * KG; it += blockDim.x) {
       const int s = it / KG, k = 1 + it % KG;
-      double2 *row = sm + s * NH;
-      const double2 a = row[k], b = row[M - k];
-      const double2 w = tw[k];
-      const double2 wm = make_double2(-w.x, w.y);  /* W^{M-k} = -conj(W^k) */
+      V *row = sm + s * NH;
+      const V a = row[k], b = row[M - k];
+      const V w = tw[k];
+      const V wm = Cx<V>::mk(-w.x, w.y);  /* W^{M-k} = -conj(W^k) */
       row[k] = cadd(cadd(a, cconj(b)), cmuli(cmul(w, csub(a, cconj(b)))));
       row[M - k] = cadd(cadd(b, cconj(a)), cmuli(cmul(wm, csub(b, cconj(a)))));
    }
    for (int it = threadIdx.x; it < 2 * N; it += blockDim.x) {
-      double2 *row = sm + (it >> 1) * NH;
+      V *row = sm + (it >> 1) * NH;
       if (it & 1) {
-         const double2 a = row[M / 2];
-         const double2 w = tw[M / 2];
+         const V a = row[M / 2];
+         const V w = tw[M / 2];
          row[M / 2] = cadd(cadd(a, cconj(a)), cmuli(cmul(w, csub(a, cconj(a)))));
       } else {
-         const double x0 = row[0].x, xm = row[M].x;
-         row[0] = make_double2(x0 + xm, x0 - xm);
+         const R x0 = row[0].x, xm = row[M].x;
+         row[0] = Cx<V>::mk(x0 + xm, x0 - xm);
       }
    }
 }
@@ -218,47 +237,48 @@ __device__ __forceinline__ void c2r_pre(double2 *sm, const double2 *__restrict__
 /* X_k = (Z_k + conj Z_{M-k})/2 - (i/2) W^{-k} (Z_k - conj Z_{M-k}), k = 0..M,
  * with Im X_0 = Im X_M = 0 exactly (fftpack3d.f:160-161); same split of the
  * work as c2r_pre. */
-template<int N>
-__device__ __forceinline__ void r2c_post(double2 *sm, const double2 *__restrict__ tw)
+template<int N, typename V>
+__device__ __forceinline__ void r2c_post(V *sm, const V *__restrict__ tw)
 {
+   using R = typename Cx<V>::R;
    constexpr int NH = N / 2 + 1, M = N / 2, KG = M / 2 - 1;
    for (int it = threadIdx.x; it < N * KG; it += blockDim.x) {
       const int s = it / KG, k = 1 + it % KG;
-      double2 *row = sm + s * NH;
-      const double2 za = row[k], zb = row[M - k];
-      const double2 w = cconj(tw[k]);                        /* W^{-k} */
-      const double2 wm = make_double2(-tw[k].x, -tw[k].y);    /* W^{-(M-k)} = -W^k */
-      const double2 sa = cadd(za, cconj(zb)), da = csub(za, cconj(zb));
-      const double2 sb = cadd(zb, cconj(za)), db = csub(zb, cconj(za));
-      const double2 ta = cmuli(cmul(w, da)), tb = cmuli(cmul(wm, db));
-      row[k] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
-      row[M - k] = make_double2(0.5 * (sb.x - tb.x), 0.5 * (sb.y - tb.y));
+      V *row = sm + s * NH;
+      const V za = row[k], zb = row[M - k];
+      const V w = cconj(tw[k]);                        /* W^{-k} */
+      const V wm = Cx<V>::mk(-tw[k].x, -tw[k].y);      /* W^{-(M-k)} = -W^k */
+      const V sa = cadd(za, cconj(zb)), da = csub(za, cconj(zb));
+      const V sb = cadd(zb, cconj(za)), db = csub(zb, cconj(za));
+      const V ta = cmuli(cmul(w, da)), tb = cmuli(cmul(wm, db));
+      row[k] = Cx<V>::mk(R(0.5) * (sa.x - ta.x), R(0.5) * (sa.y - ta.y));
+      row[M - k] = Cx<V>::mk(R(0.5) * (sb.x - tb.x), R(0.5) * (sb.y - tb.y));
    }
    for (int it = threadIdx.x; it < 2 * N; it += blockDim.x) {
-      double2 *row = sm + (it >> 1) * NH;
+      V *row = sm + (it >> 1) * NH;
       if (it & 1) {
-         const double2 z = row[M / 2];
-         const double2 w = cconj(tw[M / 2]);
-         const double2 sa = cadd(z, cconj(z)), ta = cmuli(cmul(w, csub(z, cconj(z))));
-         row[M / 2] = make_double2(0.5 * (sa.x - ta.x), 0.5 * (sa.y - ta.y));
+         const V z = row[M / 2];
+         const V w = cconj(tw[M / 2]);
+         const V sa = cadd(z, cconj(z)), ta = cmuli(cmul(w, csub(z, cconj(z))));
+         row[M / 2] = Cx<V>::mk(R(0.5) * (sa.x - ta.x), R(0.5) * (sa.y - ta.y));
       } else {
-         const double2 z0 = row[0];
-         row[0] = make_double2(z0.x + z0.y, 0.0);
-         row[M] = make_double2(z0.x - z0.y, 0.0);
+         const V z0 = row[0];
+         row[0] = Cx<V>::mk(z0.x + z0.y, R(0));
+         row[M] = Cx<V>::mk(z0.x - z0.y, R(0));
       }
    }
 }
 
-template<int N>
-__device__ __forceinline__ void load_plane(double2 *sm, const double2 *__restrict__ Tp, int nhload)
+template<int N, typename V>
+__device__ __forceinline__ void load_plane(V *sm, const V *__restrict__ Tp, int nhload)
 {
    constexpr int NH = N / 2 + 1, NT = GridCfg<N>::PLANE_THREADS;
    constexpr int EPT = (N * NH + NT - 1) / NT;
-   double2 v[EPT];
+   V v[EPT];
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
       const int e = threadIdx.x + NT * q;
-      v[q] = (e < N * NH && e % NH < nhload) ? Tp[e] : make_double2(0.0, 0.0);
+      v[q] = (e < N * NH && e % NH < nhload) ? Tp[e] : Cx<V>::mk(0, 0);
    }
 #pragma unroll
    for (int q = 0; q < EPT; ++q) {
@@ -269,8 +289,8 @@ __device__ __forceinline__ void load_plane(double2 *sm, const double2 *__restric
 }
 
 /* y inverse (natural ky -> transposed y), then x complex-to-real */
-template<int N>
-__device__ __forceinline__ void inverse_yx(double2 *sm, const double2 *__restrict__ tw)
+template<int N, typename V>
+__device__ __forceinline__ void inverse_yx(V *sm, const V *__restrict__ tw)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1;
@@ -281,8 +301,8 @@ __device__ __forceinline__ void inverse_yx(double2 *sm, const double2 *__restric
 }
 
 /* x real-to-complex then y forward (transposed y -> natural ky) */
-template<int N>
-__device__ __forceinline__ void forward_xy(double2 *sm, const double2 *__restrict__ tw)
+template<int N, typename V>
+__device__ __forceinline__ void forward_xy(V *sm, const V *__restrict__ tw)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1;
@@ -305,18 +325,33 @@ __device__ __forceinline__ void slot_xy(int s, int m, int &y, int &x0)
 
 /* ------------------------------------------------------- plane kernel */
 
-template<int N, int MODE>
-__global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE == PM_DENSITY) ? 4 : 1) k_plane(PlaneArgs a)
+/* waves per SIMD the plane kernel's registers are sized for: 4 for the
+ * FP64 density pass at N = 96, DENS_WAVES_F32 / VPSI_WAVES_F32 for the FP32
+ * orbital passes at N >= 80 */
+template<int N, int MODE, typename V>
+constexpr int plane_min_waves()
+{
+   return (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
+        : (sizeof(V) == 8 && N >= 80 && MODE == PM_VPSI) ? VPSI_WAVES_F32
+        : (N == 96 && MODE == PM_DENSITY) ? 4 : 1;
+}
+
+extern __shared__ __align__(16) unsigned char plane_lds[];
+
+template<int N, int MODE, typename V>
+__global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N, MODE, V>())) k_plane(PlaneArgs a)
 {
    using C = GridCfg<N>;
+   using R = typename Cx<V>::R;
    constexpr int NH = N / 2 + 1, M = N / 2, NT = C::PLANE_THREADS;
-   extern __shared__ double2 sm[];
+   V *sm = reinterpret_cast<V *>(plane_lds);
    const int z = blockIdx.x;
    const size_t plane = (size_t)N * NH;
    /* twiddles live in LDS after the plane; every mode passes a barrier
     * before its first transform */
-   double2 *tws = sm + plane;
-   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = a.tw[m];
+   V *tws = sm + plane;
+   for (int m = threadIdx.x; m < N; m += blockDim.x) tws[m] = cvt<V>(a.tw[m]);
+   V *Tb = reinterpret_cast<V *>(a.T);
 
    if constexpr (MODE == PM_DENSITY) {
       constexpr int ACC = (N * N + NT - 1) / NT;
@@ -326,9 +361,9 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
       const int bb0 = a.b0 + blockIdx.y * a.nb;
       const int bb1 = min(bb0 + a.nb, a.nbatch);
       for (int b = bb0; b < bb1; ++b) {
-         load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
+         load_plane<N>(sm, Tb + ((size_t)b * N + z) * plane, a.nhload);
          inverse_yx<N>(sm, tws);
-         const double *smr = reinterpret_cast<const double *>(sm);
+         const R *smr = reinterpret_cast<const R *>(sm);
 #pragma unroll
          for (int q = 0; q < ACC; ++q) {
             const int p = threadIdx.x + q * NT;
@@ -354,13 +389,13 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
       }
    } else if constexpr (MODE == PM_VTOT || MODE == PM_C2R_OUT) {
       const int b = blockIdx.y;
-      load_plane<N>(sm, a.T + ((size_t)b * N + z) * plane, a.nhload);
+      load_plane<N>(sm, Tb + ((size_t)b * N + z) * plane, a.nhload);
       inverse_yx<N>(sm, tws);
       for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
          const int s = it / M, m = it % M;
          int y, x0;
          slot_xy<N>(s, m, y, x0);
-         const double2 v = sm[s * NH + m];
+         const double2 v = to_d2(sm[s * NH + m]);
          if (MODE == PM_VTOT) {
             const size_t g = ((size_t)z * N + y) * N + x0;
             a.Vout[g] = v.x + a.xcp[g];
@@ -374,18 +409,18 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
       }
    } else if constexpr (MODE == PM_VPSI) {
       const int b = blockIdx.y;
-      double2 *Tp = a.T + ((size_t)b * N + z) * plane;
+      V *Tp = Tb + ((size_t)b * N + z) * plane;
       /* this plane's potential, fetched before the transforms need it */
       constexpr int EPV = (N * M + NT - 1) / NT;
-      double2 vv[EPV];
+      V vv[EPV];
 #pragma unroll
       for (int q = 0; q < EPV; ++q) {
          const int it = threadIdx.x + NT * q;
-         vv[q] = make_double2(1.0, 1.0);
+         vv[q] = Cx<V>::mk(1, 1);
          if (it < N * M) {
             int y, x0;
             slot_xy<N>(it / M, it % M, y, x0);
-            vv[q] = *reinterpret_cast<const double2 *>(a.V + ((size_t)z * N + y) * N + x0);
+            vv[q] = cvt<V>(*reinterpret_cast<const double2 *>(a.V + ((size_t)z * N + y) * N + x0));
          }
       }
       load_plane<N>(sm, Tp, a.nhload);
@@ -395,7 +430,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
          const int it = threadIdx.x + NT * q;
          if (it < N * M) {
             const int s = it / M, m = it % M;
-            double2 v = sm[s * NH + m];
+            V v = sm[s * NH + m];
             v.x *= vv[q].x;
             v.y *= vv[q].y;
             sm[s * NH + m] = v;
@@ -419,11 +454,11 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (N == 96 && MODE ==
             const double *row = a.rin + (size_t)b * N * N * (N + 2) + ((size_t)z * N + y) * (N + 2);
             v = make_double2(row[x0], row[x0 + 1]);
          }
-         sm[s * NH + m] = v;
+         sm[s * NH + m] = cvt<V>(v);
       }
       __syncthreads();
       forward_xy<N>(sm, tws);
-      double2 *Tp = a.T + ((size_t)b * N + z) * plane;
+      V *Tp = Tb + ((size_t)b * N + z) * plane;
       for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
    }
 }
@@ -453,8 +488,17 @@ static int zg2r(int src, const ZArgs &args, int nbatch, hipStream_t s)
    while (a.bper < 4 && (long)nkb * N * ((nbatch + 2 * a.bper - 1) / (2 * a.bper)) >= 2048) a.bper *= 2;
    dim3 grid(nkb, N, (nbatch + a.bper - 1) / a.bper);
    constexpr int ZT = GridCfg<N>::ZT;
-   if (src == SRC_PACKED) hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED>), grid, dim3(ZT), 0, s, a);
-   else hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG>), grid, dim3(ZT), 0, s, a);
+   if (a.fp32) {
+      if (src != SRC_PACKED) {
+         cpsd_set_error("zpass g2r: FP32 only for packed orbitals");
+         return -1;
+      }
+      hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED, float2>), grid, dim3(ZT), 0, s, a);
+   } else if (src == SRC_PACKED) {
+      hipLaunchKernelGGL((k_zpass_g2r<N, SRC_PACKED, double2>), grid, dim3(ZT), 0, s, a);
+   } else {
+      hipLaunchKernelGGL((k_zpass_g2r<N, SRC_VG, double2>), grid, dim3(ZT), 0, s, a);
+   }
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -465,25 +509,38 @@ static int zr2g(int dst, const ZArgs &a, int nbatch, hipStream_t s)
    constexpr int KXB = GridCfg<N>::KXB;
    dim3 grid((a.rows.nhc + KXB - 1) / KXB, N, nbatch);
    constexpr int ZT = GridCfg<N>::ZT;
-   if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI>), grid, dim3(ZT), 0, s, a);
-   else if (dst == DST_SD) hipLaunchKernelGGL((k_zpass_r2g<N, DST_SD>), grid, dim3(ZT), 0, s, a);
-   else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG>), grid, dim3(ZT), 0, s, a);
-   else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL>), grid, dim3(ZT), 0, s, a);
+   if (a.fp32) {
+      if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI, float2>), grid, dim3(ZT), 0, s, a);
+      else if (dst == DST_SD) hipLaunchKernelGGL((k_zpass_r2g<N, DST_SD, float2>), grid, dim3(ZT), 0, s, a);
+      else {
+         cpsd_set_error("zpass r2g: FP32 only for the orbital outputs");
+         return -1;
+      }
+   } else if (dst == DST_HPSI) hipLaunchKernelGGL((k_zpass_r2g<N, DST_HPSI, double2>), grid, dim3(ZT), 0, s, a);
+   else if (dst == DST_SD) hipLaunchKernelGGL((k_zpass_r2g<N, DST_SD, double2>), grid, dim3(ZT), 0, s, a);
+   else if (dst == DST_DNG) hipLaunchKernelGGL((k_zpass_r2g<N, DST_DNG, double2>), grid, dim3(ZT), 0, s, a);
+   else hipLaunchKernelGGL((k_zpass_r2g<N, DST_FULL, double2>), grid, dim3(ZT), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
 
-template<int N, int MODE>
+template<int N, typename V>
+static constexpr size_t plane_lds_bytes()
+{
+   return ((size_t)N * (N / 2 + 1) + N) * sizeof(V);   /* plane + twiddles */
+}
+
+template<int N, int MODE, typename V>
 static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
 {
-   const size_t lds = ((size_t)N * (N / 2 + 1) + N) * sizeof(double2);   /* plane + twiddles */
+   const size_t lds = plane_lds_bytes<N, V>();
    static bool attr_set = false;
    if (!attr_set) {
-      CPSD_CHECK(hipFuncSetAttribute((const void *)k_plane<N, MODE>,
+      CPSD_CHECK(hipFuncSetAttribute((const void *)k_plane<N, MODE, V>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       attr_set = true;
    }
-   hipLaunchKernelGGL((k_plane<N, MODE>), dim3(N, nblocks), dim3(GridCfg<N>::PLANE_THREADS),
+   hipLaunchKernelGGL((k_plane<N, MODE, V>), dim3(N, nblocks), dim3(GridCfg<N>::PLANE_THREADS),
                       lds, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
@@ -492,13 +549,21 @@ static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
 template<int N>
 static int plane_dispatch(int mode, const PlaneArgs &a, int nblocks, hipStream_t s)
 {
+   if (a.fp32) {
+      switch (mode) {
+      case PM_DENSITY: return plane_launch<N, PM_DENSITY, float2>(a, nblocks, s);
+      case PM_VPSI:    return plane_launch<N, PM_VPSI, float2>(a, nblocks, s);
+      }
+      cpsd_set_error("plane: mode %d has no FP32 form", mode);
+      return -1;
+   }
    switch (mode) {
-   case PM_DENSITY: return plane_launch<N, PM_DENSITY>(a, nblocks, s);
-   case PM_VTOT:    return plane_launch<N, PM_VTOT>(a, nblocks, s);
-   case PM_VPSI:    return plane_launch<N, PM_VPSI>(a, nblocks, s);
-   case PM_C2R_OUT: return plane_launch<N, PM_C2R_OUT>(a, nblocks, s);
-   case PM_R2C_DN:  return plane_launch<N, PM_R2C_DN>(a, nblocks, s);
-   case PM_R2C_IN:  return plane_launch<N, PM_R2C_IN>(a, nblocks, s);
+   case PM_DENSITY: return plane_launch<N, PM_DENSITY, double2>(a, nblocks, s);
+   case PM_VTOT:    return plane_launch<N, PM_VTOT, double2>(a, nblocks, s);
+   case PM_VPSI:    return plane_launch<N, PM_VPSI, double2>(a, nblocks, s);
+   case PM_C2R_OUT: return plane_launch<N, PM_C2R_OUT, double2>(a, nblocks, s);
+   case PM_R2C_DN:  return plane_launch<N, PM_R2C_DN, double2>(a, nblocks, s);
+   case PM_R2C_IN:  return plane_launch<N, PM_R2C_IN, double2>(a, nblocks, s);
    }
    cpsd_set_error("plane: bad mode %d", mode);
    return -1;
@@ -526,11 +591,11 @@ int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s)
    return -1;
 }
 
-template<int N>
+template<int N, typename V>
 static int density_blocks_per_cu()
 {
-   const size_t lds = ((size_t)N * (N / 2 + 1) + N) * sizeof(double2);
-   const void *k = (const void *)k_plane<N, PM_DENSITY>;
+   const size_t lds = plane_lds_bytes<N, V>();
+   const void *k = (const void *)k_plane<N, PM_DENSITY, V>;
    int nb = 0;
    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess ||
        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, GridCfg<N>::PLANE_THREADS, lds) != hipSuccess)
@@ -538,10 +603,10 @@ static int density_blocks_per_cu()
    return nb > 0 ? nb : 1;
 }
 
-int plane_density_occupancy(int n)
+int plane_density_occupancy(int n, int fp32)
 {
    switch (n) {
-#define CASE(nn) case nn: return density_blocks_per_cu<nn>();
+#define CASE(nn) case nn: return fp32 ? density_blocks_per_cu<nn, float2>() : density_blocks_per_cu<nn, double2>();
       CPSD_FOR_EACH_GRID(CASE)
 #undef CASE
    }
