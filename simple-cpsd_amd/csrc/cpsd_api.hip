@@ -99,9 +99,12 @@ struct cpsd_dev {
    double2 *T, *Tv;
    double *dnpart, *dn, *xcp, *xce, *V;
    int ndnblk[2], nob[2], fb;   /* density blocks of the FP64 [0] and FP32 [1] pass */
-   /* precision of the orbital transforms of every step but the last of a
-    * call: 0 FP64, 1 FP32 (the last step, which feeds the energy, is FP64) */
+   /* precision of the orbital transforms and KB products of every step but
+    * the last of a call: 0 FP64, 1 FP32 (the last step, which feeds the
+    * energy, is FP64) */
    int mixed;
+   float *Bf;             /* FP32 copy of the projectors B, [nproj][2 ngp] */
+   int bf_valid;          /* Bf matches B */
    double2 *dng, *s, *vall, *B;
    double *partial; size_t partial_cap;
    double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
@@ -175,7 +178,7 @@ static void ctx_free(cpsd_dev *c)
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
-                   c->fnlpart, c->fnlsum, c->poff};
+                   c->fnlpart, c->fnlsum, c->poff, c->Bf};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
@@ -453,6 +456,20 @@ static int run_structure(cpsd_dev *c)
    a.katm = c->katm; a.lmmax = c->lmmax; a.zv = c->zv; a.vl = c->vl; a.vnl = c->vnl;
    a.s = c->s; a.vall = c->vall; a.B = c->B;
    TRY(sd_structure(a, st));
+   c->bf_valid = 0;
+   return 0;
+}
+
+/* the FP32 projectors of the FP32 KB products, made when first needed
+ * after the projectors change */
+static int ensure_bf(cpsd_dev *c)
+{
+   const size_t n = (size_t)c->nproj * 2 * c->ngp;
+   if (!c->Bf) TRY(dalloc(&c->Bf, n));
+   if (!c->bf_valid) {
+      TRY(launch_to_f32(reinterpret_cast<const double *>(c->B), c->Bf, n, c->stream));
+      c->bf_valid = 1;
+   }
    return 0;
 }
 
@@ -557,8 +574,13 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    if (c->nproj > 0) {
       TRY(ph_begin(c, PH_KBPROJ));
       const int sp = splits_for(c->nproj, ne, K, 0);
-      TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K,
-                                sp, 0, c->partial, st));
+      if (f32) {
+         TRY(ensure_bf(c));
+         TRY(launch_kbproj_f32(c->Bf, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
+      } else {
+         TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K,
+                                   sp, 0, c->partial, st));
+      }
       TRY(ph_end(c, PH_KBPROJ));
       TRY(ph_begin(c, PH_KBRED));
       TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
@@ -587,7 +609,11 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    }
    /* 10 */
    TRY(ph_begin(c, PH_KBAPPLY));
-   if (last || c->nproj > 0) {
+   if (f32 && c->nproj > 0) {
+      /* C2 += dte B X with FP32 products (never the last step) */
+      TRY(launch_kbapply_f32(c->Bf, c->nproj, c->Xkb, 1, c->nproj, ne, K, reinterpret_cast<double *>(C2), K,
+                             dte, st));
+   } else if (last || c->nproj > 0) {
       TallArgs t;
       t.A = reinterpret_cast<const double *>(c->B); t.lda = K;
       t.X = c->Xkb; t.ldx = c->nproj; t.Mr = c->nproj; t.Nc = ne; t.K = K;

@@ -134,6 +134,16 @@ struct TallArgs {
 };
 int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);
 
+/* FP32 forms of the KB projection and apply (the FP32-transform mode):
+ * partial[z][p][n] = sum_{k in split z} w_k Bf[p][k] C[n][k] for `splits`
+ * even K ranges (rows of Bf and C of length K), and
+ * C2[n][k] += dte sum_p X[p xsm + n ldx] Bf[p][k]; Bf = FP32 projectors */
+int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const double *w, size_t K, int splits,
+                      double *partial, size_t partial_cap, hipStream_t s);
+int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,
+                       double *C2, size_t ldc, double dte, hipStream_t s);
+int launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+
 /* ---- epilogue (host) ---------------------------------------------------- */
 /* eigenpairs of the symmetric n x n column-major matrix a (upper triangle
  * read): a <- eigenvectors in columns, eig <- eigenvalues, both in the order

@@ -1,0 +1,304 @@
+/*
+ * gemm_f32.hip -- FP32 MFMA forms of the Kleinman-Bylander projection and
+ * apply (ccpsdv3/cpsd.c:259-306) for the FP32-transform precision mode:
+ *   k_kbproj_f32  : partial[z][p][n] = sum_{k in split z} w_k Bf[p][k] C1[n][k]
+ *                   (the projections sw before their -1/(Omega vnlnrm)
+ *                   scaling), FP32 products summed in FP32 within a split,
+ *                   the splits reduced in FP64 by k_reduce_partials;
+ *   k_kbapply_f32 : C2[n][k] += dte sum_p X[p][n] Bf[p][k] (the nonlocal
+ *                   part of the SD update), the sum in FP32, the update of
+ *                   the FP64 orbitals in FP64.
+ * Bf is an FP32 copy of the projectors, made once per set of ion positions;
+ * the orbitals stay FP64 in HBM and are rounded to FP32 while they are
+ * staged in LDS.
+ *
+ * v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and B[l/16][l%16];
+ * lane l, register r receives D[4(l/16)+r][l%16].  A sum over k may assign
+ * its terms to the instruction's 4 k slots in any order, as long as A and B
+ * agree: a lane that reads 4 consecutive k with one 16-byte LDS read feeds
+ * slot l/16 of 4 successive instructions with them.
+ */
+#include "cpsd_internal.h"
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma16(float a, float b, f4 c)
+{
+   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+/* ------------------------------------------------------ projection */
+
+#define PK 32           /* K step */
+#define PP (PK + 4)     /* row pitch (floats): a 16-byte read of 16 rows at one k hits distinct banks */
+#define PN 128          /* output columns (orbitals) per tile */
+
+/* 8 waves as 2 (rows) x 4 (columns), each (TM/2) x 32 of the TM x 128 tile;
+ * one workgroup per (tile, K split), blocks assigned to the 8 XCDs so that
+ * the workgroups one XCD runs together hold consecutive (tile, split)
+ * pairs: the tiles of one K range then share that XCD's L2 */
+template<int TM>
+__global__ void __launch_bounds__(512, 4)
+k_kbproj_f32(const float *__restrict__ Bf, size_t ldb, int M, const double *__restrict__ C, size_t ldc,
+             int N, const double *__restrict__ w, size_t K, size_t kchunk, int ntm, int ntile, int per,
+             double *__restrict__ partial)
+{
+   constexpr int BI = TM / 32;   /* 16-row blocks per wave, and A loader rows per thread / 32 */
+   __shared__ float As[2][TM][PP];
+   __shared__ float Bs[2][PN][PP];
+   const int bid = blockIdx.x;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   const int t = (int)(g % ntile), kz = (int)(g / ntile);
+   const int tm = t % ntm, tn = t / ntm;
+   const size_t kb = (size_t)kz * kchunk;
+   const size_t ke = min(K, kb + kchunk);
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wm = wave & 1, wn = wave >> 1;
+   const int q = lane >> 4, rr = lane & 15;
+   /* loader: rows lr + 32 h, k pair lq; 16 lanes cover one row's 32 k */
+   const int lr = tid >> 4, lq = (tid & 15) * 2;
+
+   f4 acc[BI][2];
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   float2 ra[BI];
+   double2 rb[4], rw;
+   auto gload = [&](size_t k0) {
+#pragma unroll
+      for (int h = 0; h < BI; ++h) {
+         const int am = tm * TM + lr + 32 * h;
+         ra[h] = am < M ? *reinterpret_cast<const float2 *>(Bf + (size_t)am * ldb + k0 + lq)
+                        : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+         const int bn = tn * PN + lr + 32 * h;
+         rb[h] = bn < N ? *reinterpret_cast<const double2 *>(C + (size_t)bn * ldc + k0 + lq)
+                        : make_double2(0.0, 0.0);
+      }
+      rw = *reinterpret_cast<const double2 *>(w + k0 + lq);
+   };
+   auto lstore = [&](int bf) {
+#pragma unroll
+      for (int h = 0; h < BI; ++h) *reinterpret_cast<float2 *>(&As[bf][lr + 32 * h][lq]) = ra[h];
+      /* the G-space weight (1 or 2, 0 on the padded tail) is exact in FP32 */
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+         *reinterpret_cast<float2 *>(&Bs[bf][lr + 32 * h][lq]) =
+            make_float2((float)(rb[h].x * rw.x), (float)(rb[h].y * rw.y));
+   };
+   auto mma = [&](int bf, int hk) {
+      f4 a[BI], b[2];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) a[i] = *reinterpret_cast<const f4 *>(&As[bf][(TM / 2) * wm + 16 * i + rr][16 * hk + 4 * q]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const f4 *>(&Bs[bf][32 * wn + 16 * j + rr][16 * hk + 4 * q]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+         for (int i = 0; i < BI; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i][c], b[j][c], acc[i][j]);
+   };
+
+   /* K tile k+1 waits in registers while tile k is multiplied and goes to
+    * the other LDS buffer halfway through; one barrier per tile */
+   if (kb < ke) {
+      gload(kb);
+      lstore(0);
+   }
+   __syncthreads();
+   if (kb + PK < ke) gload(kb + PK);
+   int buf = 0;
+   for (size_t k0 = kb; k0 < ke; k0 += PK) {
+      mma(buf, 0);
+      if (k0 + PK < ke) {
+         lstore(buf ^ 1);
+         if (k0 + 2 * PK < ke) gload(k0 + 2 * PK);
+      }
+      mma(buf, 1);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+   /* 16 lanes hold 16 consecutive columns of one row: 128-byte segments */
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+         const int n = tn * PN + 32 * wn + 16 * j + rr;
+#pragma unroll
+         for (int r = 0; r < 4; ++r) {
+            const int p = tm * TM + (TM / 2) * wm + 16 * i + 4 * q + r;
+            if (p < M && n < N) partial[((size_t)kz * M + p) * N + n] = (double)acc[i][j][r];
+         }
+      }
+}
+
+int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const double *w, size_t K, int splits,
+                      double *partial, size_t partial_cap, hipStream_t s)
+{
+   if (K % PK) { cpsd_set_error("kbproj_f32: K=%zu not a multiple of %d", K, PK); return -1; }
+   if (splits < 1 || (size_t)splits * M * N > partial_cap) {
+      cpsd_set_error("kbproj_f32: %d splits of %d x %d do not fit the partial buffer", splits, M, N);
+      return -1;
+   }
+   const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
+   const int tm = pad96 < pad128 ? 96 : 128;
+   const int ntm = (M + tm - 1) / tm, ntile = ntm * ((N + PN - 1) / PN);
+   size_t kchunk = (K + splits - 1) / splits;
+   kchunk = (kchunk + PK - 1) / PK * PK;
+   const long pairs = (long)ntile * splits;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   if (tm == 96)
+      hipLaunchKernelGGL((k_kbproj_f32<96>), dim3((unsigned)pairs), dim3(512), 0, s, Bf, K, M, C, K, N, w, K,
+                         kchunk, ntm, ntile, per, partial);
+   else
+      hipLaunchKernelGGL((k_kbproj_f32<128>), dim3((unsigned)pairs), dim3(512), 0, s, Bf, K, M, C, K, N, w, K,
+                         kchunk, ntm, ntile, per, partial);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* ----------------------------------------------------------- apply */
+
+#define AK 256          /* output reals (k) per workgroup */
+#define AM 16           /* projectors per step */
+#define AN 128          /* output columns (orbitals) per workgroup */
+#define BP (AK + 4)     /* Bs row pitch: the 4 rows 4 apart one read touches land 16 banks apart */
+#define XP (AM + 4)     /* Xs row pitch: a 16-byte read of 16 rows hits distinct banks */
+
+/* 8 waves as 2 (columns) x 4 (k), each 64 columns x 64 k = 4 x 4 blocks */
+__global__ void __launch_bounds__(512, 4)
+k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
+              int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per)
+{
+   __shared__ float Bs[2][AM][BP];
+   __shared__ float Xs[2][AN][XP];
+   const int bid = blockIdx.x;
+   const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
+   const size_t k0 = (size_t)(g / ntn) * AK;
+   const int n0 = (g % ntn) * AN;
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wn = wave & 1, wk = wave >> 1;
+   const int q = lane >> 4, rr = lane & 15;
+   /* loaders: Bs row br, 8 k at bc (two 16-byte pieces; 32 lanes per 1 KB
+    * row); Xs column xn, 4 projectors at xp */
+   const int br = tid >> 5, bc = (tid & 31) * 8;
+   const int xn = tid >> 2, xp = (tid & 3) * 4;
+
+   f4 acc[4][4];   /* [column block][k block] */
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   float4 rb[2];
+   double rx[4];
+   auto gload = [&](int m0) {
+      const int m = m0 + br;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+         const size_t k = k0 + bc + 4 * h;
+         rb[h] = (m < Mr && k < K) ? *reinterpret_cast<const float4 *>(Bf + (size_t)m * ldb + k)
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      const int n = n0 + xn;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+         const int mm = m0 + xp + c;
+         rx[c] = (mm < Mr && n < Nc) ? X[(size_t)mm * xsm + (size_t)n * ldx] : 0.0;
+      }
+   };
+   auto lstore = [&](int bf) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) *reinterpret_cast<float4 *>(&Bs[bf][br][bc + 4 * h]) = rb[h];
+      *reinterpret_cast<float4 *>(&Xs[bf][xn][xp]) =
+         make_float4((float)rx[0], (float)rx[1], (float)rx[2], (float)rx[3]);
+   };
+   /* slot q of instruction c takes projector 4q + c */
+   auto mma = [&](int bf) {
+      f4 x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = *reinterpret_cast<const f4 *>(&Xs[bf][64 * wn + 16 * i + rr][4 * q]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+         float b[4];
+#pragma unroll
+         for (int j = 0; j < 4; ++j) b[j] = Bs[bf][4 * q + c][64 * wk + 16 * j + rr];
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(x[i][c], b[j], acc[i][j]);
+      }
+   };
+
+   gload(0);
+   lstore(0);
+   __syncthreads();
+   if (AM < Mr) gload(AM);
+   int buf = 0;
+   for (int m0 = 0; m0 < Mr; m0 += AM) {
+      if (m0 + AM < Mr) lstore(buf ^ 1);
+      if (m0 + 2 * AM < Mr) gload(m0 + 2 * AM);
+      mma(buf);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+   /* lane (rr, q), register r: column 16 i + 4 q + r, k = 16 j + rr; 16
+    * lanes update 16 consecutive reals of one orbital */
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+         const int n = n0 + 64 * wn + 16 * i + 4 * q + r;
+         if (n >= Nc) continue;
+#pragma unroll
+         for (int j = 0; j < 4; ++j) {
+            const size_t k = k0 + 64 * wk + 16 * j + rr;
+            if (k < K) {
+               double *p = C2 + (size_t)n * ldc + k;
+               *p += dte * (double)acc[i][j][r];
+            }
+         }
+      }
+}
+
+int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,
+                       double *C2, size_t ldc, double dte, hipStream_t s)
+{
+   if (K % 8) { cpsd_set_error("kbapply_f32: K=%zu not a multiple of 8", K); return -1; }
+   const int ntn = (Nc + AN - 1) / AN;
+   const long blocks = (long)((K + AK - 1) / AK) * ntn;
+   const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
+   hipLaunchKernelGGL(k_kbapply_f32, dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X, xsm, ldx, Nc, K,
+                      C2, ldc, dte, ntn, per);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* ----------------------------------------------------- FP32 projectors */
+
+__global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n)
+{
+   const size_t i = ((size_t)blockIdx.x * 256 + threadIdx.x) * 2;
+   if (i + 1 < n) {
+      const double2 v = *reinterpret_cast<const double2 *>(a + i);
+      *reinterpret_cast<float2 *>(b + i) = make_float2((float)v.x, (float)v.y);
+   } else if (i < n) {
+      b[i] = (float)a[i];
+   }
+}
+
+int launch_to_f32(const double *a, float *b, size_t n, hipStream_t s)
+{
+   if (n == 0) return 0;
+   const size_t blocks = (n + 511) / 512;
+   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
