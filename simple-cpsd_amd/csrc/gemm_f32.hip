@@ -176,8 +176,12 @@ __global__ void __launch_bounds__(512, 4)
 k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
               int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per)
 {
-   __shared__ float Bs[2][AM][BP];
-   __shared__ float Xs[2][AN][XP];
+   /* main loop: Bs[2][AM][BP] and Xs[2][AN][XP]; epilogue: T[64][BP], one
+    * 64-column half of the output tile at a time */
+   constexpr int BSZ = 2 * AM * BP, XSZ = 2 * AN * XP, TSZ = 64 * BP;
+   __shared__ float lds[BSZ + XSZ > TSZ ? BSZ + XSZ : TSZ];
+   float (*Bs)[AM][BP] = reinterpret_cast<float (*)[AM][BP]>(lds);
+   float (*Xs)[AN][XP] = reinterpret_cast<float (*)[AN][XP]>(lds + BSZ);
    const int bid = blockIdx.x;
    const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
    const size_t k0 = (size_t)(g / ntn) * AK;
@@ -249,23 +253,37 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
       buf ^= 1;
    }
 
-   /* lane (rr, q), register r: column 16 i + 4 q + r, k = 16 j + rr; 16
-    * lanes update 16 consecutive reals of one orbital */
+   /* epilogue per 64-column half: the half's waves stage their sums in LDS
+    * (lane (rr, q), register r holds column 16 i + 4 q + r, k = 16 j + rr),
+    * then every wave updates whole 2 KB rows of C2, 32 bytes per lane */
+   float (*T)[BP] = reinterpret_cast<float (*)[BP]>(lds);
 #pragma unroll
-   for (int i = 0; i < 4; ++i)
+   for (int half = 0; half < 2; ++half) {
+      if (half > 0) __syncthreads();
+      if (wn == half) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-         const int n = n0 + 64 * wn + 16 * i + 4 * q + r;
-         if (n >= Nc) continue;
+         for (int i = 0; i < 4; ++i)
 #pragma unroll
-         for (int j = 0; j < 4; ++j) {
-            const size_t k = k0 + 64 * wk + 16 * j + rr;
-            if (k < K) {
-               double *p = C2 + (size_t)n * ldc + k;
-               *p += dte * (double)acc[i][j][r];
-            }
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+               for (int r = 0; r < 4; ++r) T[16 * i + 4 * q + r][64 * wk + 16 * j + rr] = acc[i][j][r];
+      }
+      __syncthreads();
+      const size_t k = k0 + 4 * lane;
+#pragma unroll 4
+      for (int nl = wave; nl < 64; nl += 8) {
+         const int n = n0 + 64 * half + nl;
+         if (n < Nc && k < K) {
+            const float4 v = *reinterpret_cast<const float4 *>(&T[nl][4 * lane]);
+            double2 *p = reinterpret_cast<double2 *>(C2 + (size_t)n * ldc + k);
+            double2 a = p[0], b = p[1];
+            a.x += dte * v.x; a.y += dte * v.y;
+            b.x += dte * v.z; b.y += dte * v.w;
+            p[0] = a;
+            p[1] = b;
          }
       }
+   }
 }
 
 int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,

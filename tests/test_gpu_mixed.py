@@ -139,3 +139,36 @@ def test_mixed_h2o32_vs_fp64(gpu):
     assert 2 * o["dn"].sum() * dv == pytest.approx(2 * ne, rel=1e-10)
     for d in devs:
         d.close()
+
+
+@pytest.mark.parametrize("nmol,L,nfft,ne", [(1, 20.0, 128, 4), (4, 14.0, 32, 160), (11, 16.0, 32, 24),
+                                            (20, 16.0, 32, 24)])
+def test_mixed_shapes_vs_fp64(gpu, nmol, L, nfft, ne):
+    """the 128^3 transforms (two-half z passes); 160 orbitals (two column
+    tiles in both FP32 KB products); 66 projectors (one partial 96-row
+    projection tile); 120 projectors (the 128-row projection tile): mixed
+    against FP64 from the same start"""
+    say("mixed vs FP64: %d waters, %d^3, ne=%d" % (nmol, nfft, ne))
+    cell = cpsd_amd.water_cell(nmol, L, nfft, seed=4)
+    psi = cell.random_orbitals(ne, seed=9, decay=0.05)
+    natom = 3 * nmol
+    devs = []
+    for mode in (0, 1):
+        d = cpsd_amd.DeviceSD(cell, ne)
+        d.set_precision(mode)
+        d.set_orbitals(psi)
+        devs.append(d)
+    for _ in range(2):
+        E64, _, _ = devs[0].run(5)
+        E32, _, _ = devs[1].run(5)
+        assert np.max(np.abs(E32 - E64)) < 1e-6 * natom, (E32, E64)
+    o64 = devs[0].get(c2=True)["c2"]
+    o32 = devs[1].get(c2=True, dn=True)
+    assert np.max(np.abs(o32["c2"] - o64)) < 1e-6
+    assert not np.array_equal(o32["c2"], o64)
+    S = gram(o32["c2"], ne, nfft)
+    assert np.max(np.abs(S - np.eye(ne))) < 1e-11
+    dv = cell.omega / nfft ** 3
+    assert 2 * o32["dn"].sum() * dv == pytest.approx(2 * ne, rel=1e-10)
+    for d in devs:
+        d.close()
