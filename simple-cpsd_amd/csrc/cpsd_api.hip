@@ -105,6 +105,11 @@ struct cpsd_dev {
    int mixed;
    float *Bf;             /* FP32 copy of the projectors B, [nproj][2 ngp] */
    int bf_valid;          /* Bf matches B */
+   /* <c1|c1> of the next step, carried from this step's overlaps and lmd
+    * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
+   double *g11c[2], *g11w;
+   int g11_idx, g11_valid;
+   long gram_wgs;         /* workgroups of the full Gram launch */
    double2 *dng, *s, *vall, *B;
    double *partial; size_t partial_cap;
    double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
@@ -178,7 +183,7 @@ static void ctx_free(cpsd_dev *c)
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
-                   c->fnlpart, c->fnlsum, c->poff, c->Bf};
+                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
@@ -406,13 +411,26 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
       if ((size_t)(so > sd ? so : sd) * ne * ne > cap) cap = (size_t)(so > sd ? so : sd) * ne * ne;
       sp = splits_for(nproj, ne, K, 0);
       if ((size_t)sp * nproj * ne > cap) cap = (size_t)sp * nproj * ne;
+      /* the Gram without its C1 x C1 block (ne a multiple of 128): the same
+       * number of workgroups over the tiles that touch C2 */
+      c->gram_wgs = (long)gemm_tn_tiles(2 * ne, 2 * ne, 1) * splits_for(2 * ne, 2 * ne, K, 1);
+      if (ne % 128 == 0 && ne <= 512) {
+         const int q = ne / 128, nt = 2 * q;
+         int so2, sd2;
+         gemm_tn_list_splits(nt * (nt - 1) / 2 - q * (q - 1) / 2, q, K, c->gram_wgs, &so2, &sd2);
+         const size_t z = (size_t)(so2 > 2 * sd2 ? so2 : 2 * sd2);
+         if (z * 4 * ne * ne > cap) cap = z * 4 * ne * ne;
+      }
       sp = splits_for(ne, ne, K, 0);
       if ((size_t)sp * ne * ne > cap) cap = (size_t)sp * ne * ne;
       c->partial_cap = cap;
       TRY(dalloc(&c->partial, cap));
    }
    const size_t ee = (size_t)ne * ne;
-   TRY(dalloc(&c->Gm, 4 * ee)); TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
+   TRY(dalloc(&c->Gm, 4 * ee));
+   if (ne % 128 == 0 && ne <= 512) {
+      TRY(dalloc(&c->g11c[0], ee)); TRY(dalloc(&c->g11c[1], ee)); TRY(dalloc(&c->g11w, 2 * ee));
+   } TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->Xkb, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->s11, ee)); TRY(dalloc(&c->s12, ee)); TRY(dalloc(&c->s21, ee)); TRY(dalloc(&c->s22, ee));
    TRY(dalloc(&c->LX, 2 * ee)); TRY(dalloc(&c->st1, ee)); TRY(dalloc(&c->lmd, ee));
@@ -624,16 +642,28 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    }
    TRY(ph_end(c, PH_KBAPPLY));
    /* 11 */
+   /* with <c1|c1> carried from the previous step, the Gram skips its C1 x C1
+    * block (whose reduced entries are then not read) */
+   const int carry = c->g11_valid;
    TRY(ph_begin(c, PH_GRAM));
    {
       const int sp = splits_for(2 * ne, 2 * ne, K, 1);
       const double *Cb = reinterpret_cast<const double *>(c->Cbase);
-      TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
+      int so = 0, sdd = 0;
+      if (carry) {
+         TRY(launch_gram_c2(Cb, K, 2 * ne, (c->cur ^ 1) * ne, c->w, K, c->gram_wgs, c->partial, c->partial_cap,
+                            &so, &sdd, st));
+      } else {
+         TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
+      }
       TRY(ph_end(c, PH_GRAM));
       TRY(ph_begin(c, PH_GRAMRED));
       /* diagonal tiles hold their upper 16 x 16 blocks only */
-      TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
-      TRY(sd_smat(c->Gm, ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, st));
+      if (carry) TRY(launch_reduce_partials_split(c->partial, so, sdd, 2 * ne, c->cur * ne, (c->cur + 1) * ne,
+                                                  c->Gm, st));
+      else TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
+      TRY(sd_smat(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->s11, c->s12, c->s21,
+                  c->s22, st));
       TRY(ph_end(c, PH_GRAMRED));
    }
    /* 12 */
@@ -646,6 +676,15 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(sd_lagrange(l, st));
    }
    TRY(ph_end(c, PH_LAGRANGE));
+   /* <c2f|c2f> of the constrained orbitals, the next step's <c1|c1> */
+   if (c->g11c[0]) {
+      TRY(ph_begin(c, PH_GRAMRED));
+      TRY(sd_gram_next(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, c->lmd, dte, c->g11w,
+                       c->g11c[c->g11_idx ^ 1], st));
+      c->g11_idx ^= 1;
+      c->g11_valid = 1;
+      TRY(ph_end(c, PH_GRAMRED));
+   }
    /* 13 */
    TRY(ph_begin(c, PH_CONSTRAINT));
    {
@@ -795,6 +834,7 @@ static int ctx_set_orbitals(cpsd_dev *c, const double *cin)
    CPSD_CHECK(hipMemsetAsync(c->flag, 0, sizeof(int), st));
    TRY(sd_check_unmasked(c->fullbuf, fstride, c->pidx, c->nfft3d, c->ne, c->flag, st));
    c->cur = 0;
+   c->g11_valid = 0;
    TRY(sd_pack(c->fullbuf, fstride, c->gidx, c->ng, c->ngp, c->Cbase, c->ngp, c->ne, st));
    int flag = 0;
    CPSD_CHECK(hipMemcpyAsync(&flag, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -844,6 +884,7 @@ static int ctx_get(cpsd_dev *c, double *c1, double *c2, double *cpsi, double *dn
 static int ctx_epilogue(cpsd_dev *c, double *ortho, double *eig, double *V)
 {
    if (!c->have_run) { cpsd_set_error("epilogue: no SD step has run on this context"); return -1; }
+   c->g11_valid = 0;
    const int ne = c->ne;
    const size_t K = 2 * (size_t)c->ngp, orb = (size_t)ne * c->ngp, ee = (size_t)ne * ne;
    hipStream_t st = c->stream;

@@ -101,6 +101,11 @@ int plane_density_occupancy(int n, int fp32);
  * (A: M rows of stride lda, B: N rows of stride ldb; K multiple of 32).
  * symm (A == B, M == N): only the 16 x 16 blocks on or above the diagonal
  * are formed.                                                             */
+/* tile list of a symmetric product restricted to some tiles (k_gemm_tn_splitk) */
+struct GramTiles {
+   int on, noff, nd;
+   unsigned char otm[32], otn[32], dt[8];
+};
 int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, size_t ldb,
                           int N, const double *w, size_t K, int splits, int symm,
                           double *partial, hipStream_t s);
@@ -115,6 +120,17 @@ int launch_reduce_partials_symm(const double *partial, int M, size_t K, int spli
                                 hipStream_t s);
 /* split counts a symmetric product uses: off-diagonal tiles and diagonal pairs */
 void gemm_tn_symm_splits(int M, size_t K, int splits, int *sp_off, int *sp_dd);
+/* the same for a product restricted to noff off-diagonal and nd diagonal
+ * tiles over about wgs workgroups (diagonal tiles as two half-range splits) */
+void gemm_tn_list_splits(int noff, int nd, size_t K, long wgs, int *sp_off, int *sp_dd);
+/* the Gram of [C1 C2] (M = 2 ne rows, ne a multiple of 128, C2 from row
+ * c2row) without its C1 x C1 block; returns the split counts of the
+ * off-diagonal and diagonal tiles for launch_reduce_partials_split */
+int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *w, size_t K, long wgs,
+                   double *partial, size_t partial_cap, int *sp_off, int *sp_dd, hipStream_t s);
+/* rows and columns both in [skip0, skip1) are left as they are */
+int launch_reduce_partials_split(const double *partial, int sp_off, int sp_dd, int M, int skip0, int skip1,
+                                 double *out, hipStream_t s);
 
 /* EPI_STORE: C2[n][k] = result (the rotation c1 = c2 V of the epilogue) */
 enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2, EPI_STORE = 3 };

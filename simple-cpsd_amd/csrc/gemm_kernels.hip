@@ -29,6 +29,7 @@
  */
 #include "cpsd_internal.h"
 #include <type_traits>
+#include <cstring>
 
 __device__ __forceinline__ double mfma4(double a, double b, double c)
 {
@@ -100,20 +101,39 @@ __device__ constexpr int dg_bj(int dm, int b)
  * 256 rows as the off-diagonal tile (2d, 2d+1) and forms the 2 x 36 blocks on
  * or above the diagonals of tiles 2d and 2d+1 (waves 0-3 the first, 4-7 the
  * second), 18 blocks per SIMD against 16 for an off-diagonal tile, so the
- * workgroups of a launch take about the same time. */
+ * workgroups of a launch take about the same time.
+ *
+ * With a tile list (tl.on), a symmetric product forms only the listed
+ * off-diagonal tiles and the listed diagonal tiles, each diagonal tile by a
+ * diagonal-pair workgroup whose two halves take the two halves of its K
+ * range (rows of the same tile at k and at k + half) and write them as two
+ * K splits, 2z and 2z + 1. */
 template<int TMR>
 __global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
                  const double *__restrict__ B, size_t ldb, int N,
                  const double *__restrict__ w, size_t K, size_t kchunk, size_t kchunk_dd, int symm,
-                 int ntm, int ntile, long poff, int per, double *__restrict__ partial)
+                 int ntm, int ntile, long poff, int per, GramTiles tl, double *__restrict__ partial)
 {
    const int bid = blockIdx.x;
    const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
    int tm, tn, kz;
-   bool dd = false;
+   bool dd = false, selfdd = false;
    size_t kc = kchunk;
-   if (symm) {
+   if (symm && tl.on) {
+      if (g < poff) {
+         const int t = (int)(g % tl.noff);
+         kz = (int)(g / tl.noff);
+         tm = tl.otm[t];
+         tn = tl.otn[t];
+      } else {
+         const long h = g - poff;
+         tm = tn = tl.dt[h % tl.nd];
+         kz = (int)(h / tl.nd);
+         dd = selfdd = true;
+         kc = kchunk_dd;
+      }
+   } else if (symm) {
       const int noff = ntm * (ntm - 1) / 2, ndd = (ntm + 1) / 2;
       if (g < poff) {   /* tiles strictly above the diagonal, row by row */
          int rem = (int)(g % noff);
@@ -137,12 +157,15 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    }
    const size_t kb = (size_t)kz * kc;
    const size_t ke = min(K, kb + kc);
+   /* a single diagonal tile's second half starts khalf after the first */
+   const size_t khalf = (selfdd && ke > kb) ? ((ke - kb) / KT + 1) / 2 * KT : 0;
+   const size_t kend = selfdd ? min(ke, kb + khalf) : ke;
    /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
     * stages the output tile for the epilogue.  Row pitch KT + 2: the 16 rows
     * x 2 k of a half-wave's operand read fall in distinct bank pairs, and
     * rows stay 16-byte aligned for the stores. */
    __shared__ double S[2][2][TT][KT + 2];
-   __shared__ double Ws[2][KT];
+   __shared__ double Ws[2][2][KT];   /* weights of the A rows ([0]) and B rows ([1]) */
    static_assert(sizeof(S) >= sizeof(double) * TT * TP, "staging tile does not fit");
    /* the wave index is uniform: read it as a scalar */
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -173,16 +196,23 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
     * k+2 is requested right after, so loads, LDS stores and MFMAs overlap
     * with one barrier per tile */
    double2 ra[4], rb[4], rw;
+   /* the B rows sit khalf further along K (0 except for a single diagonal
+    * tile), and are zero past the workgroup's K range */
    auto gload = [&](size_t k0) {
+      const bool bin = k0 + khalf < ke;
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
          const int r = lr + 32 * h, am = tm * TMR + r, bn = tn * TT + r;
          ra[h] = make_double2(0.0, 0.0);
          rb[h] = make_double2(0.0, 0.0);
          if (r < TMR && am < M) ra[h] = *reinterpret_cast<const double2 *>(A + (size_t)am * lda + k0 + lq);
-         if (bn < N) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + lq);
+         if (bn < N && bin) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + khalf + lq);
       }
-      if (tid < KT / 2) rw = *reinterpret_cast<const double2 *>(w + k0 + 2 * tid);
+      if (tid < KT) {
+         const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
+         rw = make_double2(0.0, 0.0);
+         if (hb == 0 || bin) rw = *reinterpret_cast<const double2 *>(w + k0 + (hb ? khalf : 0) + kq);
+      }
    };
    auto lstore = [&](int bf) {
 #pragma unroll
@@ -190,7 +220,11 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
          *reinterpret_cast<double2 *>(&S[bf][0][lr + 32 * h][lq]) = ra[h];
          *reinterpret_cast<double2 *>(&S[bf][1][lr + 32 * h][lq]) = rb[h];
       }
-      if (tid < KT / 2) { Ws[bf][2 * tid] = rw.x; Ws[bf][2 * tid + 1] = rw.y; }
+      if (tid < KT) {
+         const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
+         Ws[bf][hb][kq] = rw.x;
+         Ws[bf][hb][kq + 1] = rw.y;
+      }
    };
 
    /* diagonal pair: wave w works on tile half hs = w / 4 (operand S[.][hs]);
@@ -205,7 +239,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto mma_diag = [&](auto role, int bf, int ks) {
       constexpr int DM = decltype(role)::value;
       const int kk = 4 * ks + (lane >> 4);
-      const double wk = Ws[bf][kk];
+      const double wk = Ws[bf][hs][kk];
       double a[4], bv[4][4];
 #pragma unroll
       for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kk] * wk;
@@ -218,7 +252,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    };
    auto mma_full = [&](int bf, int ks) {
       const int kk = 4 * ks + (lane >> 4);
-      const double wk = Ws[bf][kk];
+      const double wk = Ws[bf][0][kk];
       double a[BI];
 #pragma unroll
       for (int i = 0; i < BI; ++i) a[i] = S[bf][0][(TMR / 2) * wm + 16 * i + (lane & 15)][kk] * wk;
@@ -237,19 +271,19 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
     * the unrolled MFMA steps */
    auto kloop = [&](auto role) {
       constexpr int DM = decltype(role)::value;
-      if (kb < ke) {
+      if (kb < kend) {
          gload(kb);
          lstore(0);
       }
       __syncthreads();
-      if (kb + KT < ke) gload(kb + KT);
+      if (kb + KT < kend) gload(kb + KT);
       int buf = 0;
-      for (size_t k0 = kb; k0 < ke; k0 += KT) {
+      for (size_t k0 = kb; k0 < kend; k0 += KT) {
 #pragma unroll
          for (int ks = 0; ks < KT / 8; ++ks) DM ? mma_diag(role, buf, ks) : mma_full(buf, ks);
-         if (k0 + KT < ke) {
+         if (k0 + KT < kend) {
             lstore(buf ^ 1);
-            if (k0 + 2 * KT < ke) gload(k0 + 2 * KT);
+            if (k0 + 2 * KT < kend) gload(k0 + 2 * KT);
          }
 #pragma unroll
          for (int ks = KT / 8; ks < KT / 4; ++ks) DM ? mma_diag(role, buf, ks) : mma_full(buf, ks);
@@ -264,8 +298,8 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    /* stage the tile in LDS, then write whole 1 KB rows of the partial */
    double (*T)[TP] = reinterpret_cast<double (*)[TP]>(&S[0][0][0][0]);
    const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
-   auto put = [&](int m, int n, int ml) {
-      double *dst = partial + ((size_t)kz * M + m) * N + n;
+   auto put = [&](int m, int n, int ml, int zz) {
+      double *dst = partial + ((size_t)zz * M + m) * N + n;
       if (n + 1 < N && !(N & 1)) {
          *reinterpret_cast<double2 *>(dst) = make_double2(T[ml][2 * lane], T[ml][2 * lane + 1]);
       } else {
@@ -291,13 +325,13 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
             else stage(std::integral_constant<int, 2>{});
          }
          __syncthreads();
-         const int t0 = (tm + h) * TT, n = t0 + 2 * lane;
+         const int t0 = (tm + (selfdd ? 0 : h)) * TT, n = t0 + 2 * lane;
          if (n < N) {
             for (int ml = wave; ml < TT; ml += 8) {
                const int m = t0 + ml;
                if (m >= M) break;
                if (2 * lane < 16 * (ml >> 4)) continue;
-               put(m, n, ml);
+               put(m, n, ml, selfdd ? 2 * kz + h : kz);
             }
          }
          __syncthreads();
@@ -319,7 +353,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       for (int ml = wave; ml < TMR; ml += 8) {
          const int m = tm * TMR + ml;
          if (m >= M) break;
-         put(m, n, ml);
+         put(m, n, ml, kz);
       }
    }
 }
@@ -392,13 +426,78 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
       kchunk = chunk(splits);
    }
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   GramTiles tl;
+   memset(&tl, 0, sizeof tl);
    if (tmr == 96)
       hipLaunchKernelGGL((k_gemm_tn_splitk<96>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
-                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, partial);
+                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, tl, partial);
    else
       hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
-                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, partial);
+                         ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, tl, partial);
    CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* K splits of a symmetric product restricted to the tiles of a list: the
+ * off-diagonal tiles take sp_off splits and the diagonal tiles sp_dd, each of
+ * the latter formed as two half-range splits (2 sp_dd partials).  A diagonal
+ * workgroup does 18 blocks per SIMD over half its range against 16 over the
+ * whole range for an off-diagonal one, so sp_dd = 9/16 sp_off balances them */
+void gemm_tn_list_splits(int noff, int nd, size_t K, long wgs, int *sp_off, int *sp_dd)
+{
+   const int kmax = (int)(K / KT) > 0 ? (int)(K / KT) : 1;
+   const int kmax2 = kmax / 2 > 0 ? kmax / 2 : 1;
+   int so = noff > 0 ? (int)(16 * wgs / (16L * noff + 9L * nd)) : 0;
+   if (noff > 0 && so < 1) so = 1;
+   if (so > kmax) so = kmax;
+   int sd = nd > 0 ? (int)((wgs - (long)noff * so + nd - 1) / nd) : 0;
+   if (nd > 0 && sd < 1) sd = 1;
+   if (sd > kmax2) sd = kmax2;
+   *sp_off = so;
+   *sp_dd = sd;
+}
+
+/* the Gram of [C1 C2] (M = 2 ne rows of stride lda, ne a multiple of 128)
+ * without the C1 x C1 block: every 128 x 128 tile with a row or column in the
+ * C2 block (rows [c2row, c2row + ne)) */
+int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *w, size_t K, long wgs,
+                   double *partial, size_t partial_cap, int *sp_off, int *sp_dd, hipStream_t s)
+{
+   if (K % KT) { cpsd_set_error("gram: K=%zu not a multiple of %d", K, KT); return -1; }
+   if (M % TT || c2row % TT || M > 8 * TT) {
+      cpsd_set_error("gram: %d rows (C2 from row %d) are not whole 128-row tiles (at most 8)", M, c2row);
+      return -1;
+   }
+   const int ntm = M / TT, c0 = c2row / TT, c1 = c0 + M / 2 / TT;
+   auto in_c2 = [&](int t) { return t >= c0 && t < c1; };
+   GramTiles tl;
+   memset(&tl, 0, sizeof tl);
+   tl.on = 1;
+   for (int i = 0; i < ntm; ++i)
+      for (int j = i + 1; j < ntm; ++j)
+         if (in_c2(i) || in_c2(j)) { tl.otm[tl.noff] = (unsigned char)i; tl.otn[tl.noff] = (unsigned char)j; ++tl.noff; }
+   for (int i = 0; i < ntm; ++i)
+      if (in_c2(i)) tl.dt[tl.nd++] = (unsigned char)i;
+   int so, sd;
+   gemm_tn_list_splits(tl.noff, tl.nd, K, wgs, &so, &sd);
+   const int zmax = so > 2 * sd ? so : 2 * sd;
+   if ((size_t)zmax * M * M > partial_cap) {
+      cpsd_set_error("gram: %d splits of %d x %d do not fit the partial buffer", zmax, M, M);
+      return -1;
+   }
+   auto chunk = [&](int sp) {
+      size_t kc = (K + sp - 1) / sp;
+      return (kc + KT - 1) / KT * KT;
+   };
+   const long poff = (long)tl.noff * so;
+   const long pairs = poff + (long)tl.nd * sd;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   const size_t kchunk = so > 0 ? chunk(so) : 0, kchunk_dd = chunk(sd);
+   hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, A, lda, M, w,
+                      K, kchunk, kchunk_dd, 1, ntm, 0, poff, per, tl, partial);
+   CPSD_CHECK(hipGetLastError());
+   *sp_off = so;
+   *sp_dd = 2 * sd;
    return 0;
 }
 
@@ -410,15 +509,17 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
  * sums the splits z = g mod 4 in two interleaved chains, and the four group
  * sums are added in g order, so the result does not depend on scheduling. */
 __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restrict__ partial, int splits,
-                                                         int sp_dd, int M, int N, int symm,
-                                                         double *__restrict__ out)
+                                                         int sp_dd, int M, int N, int symm, int skip0,
+                                                         int skip1, double *__restrict__ out)
 {
    __shared__ double red[4][64];
    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
    const size_t idx = (size_t)blockIdx.x * 64 + l;
    const int m = (int)(idx / N), n = (int)(idx % N);
-   /* symmetric: the blocks below the diagonal are written from their mirror */
-   const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm));
+   /* symmetric: the blocks below the diagonal are written from their mirror;
+    * rows and columns both in [skip0, skip1) were not formed */
+   const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm)) &&
+                      !(m >= skip0 && m < skip1 && n >= skip0 && n < skip1);
    const int nz = (symm && m / TT == n / TT) ? sp_dd : splits;
    double s0 = 0.0, s1 = 0.0;
    if (valid) {
@@ -445,7 +546,17 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
 {
    const size_t tot = (size_t)M * N;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
-                      splits, splits, M, N, symm, out);
+                      splits, splits, M, N, symm, 0, 0, out);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_reduce_partials_split(const double *partial, int sp_off, int sp_dd, int M, int skip0, int skip1,
+                                 double *out, hipStream_t s)
+{
+   const size_t tot = (size_t)M * M;
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
+                      sp_off, sp_dd, M, M, 16, skip0, skip1, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -457,7 +568,7 @@ int launch_reduce_partials_symm(const double *partial, int M, size_t K, int spli
    gemm_tn_symm_splits(M, K, splits, &so, &sd);
    const size_t tot = (size_t)M * M;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
-                      so, sd, M, M, 16, out);
+                      so, sd, M, M, 16, 0, 0, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

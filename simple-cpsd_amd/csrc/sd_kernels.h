@@ -59,8 +59,17 @@ int sd_orbsums(const double2 *c1, const double2 *c2, double2 *cp, const double *
 int sd_ewald_real(const double *r1, const int *katm, const double *zv, int nion, const double *rcell,
                   int nsh, double rcut, double *part, hipStream_t s);
 int sd_erfc_as(const double *w, int n, double *out, hipStream_t s);
-int sd_smat(const double *Gm, int ne, int c1_second, double dte, double *s11, double *s12, double *s21,
-            double *s22, hipStream_t s);
+/* s11, s12, s21, s22 of ccpsdv3/cpsd.c:415-437 from the Gram Gm of [C1 C2]
+ * (or of [C2 C1], c1_second); with g11 non-null, <c1_i|c1_j> is read from
+ * the dense ne x ne g11 instead of Gm */
+int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *s11, double *s12,
+            double *s21, double *s22, hipStream_t s);
+/* <c2f_a|c2f_b> of the orbitals after the constraint c2f = c2 + dte c1 lmd,
+ * from the step's Gram (G11 from g11 when non-null) and lmd[i + j ne]:
+ * G22 + dte (G21 lmd + lmd^T G12) + dte^2 lmd^T G11 lmd, into out (ne x ne);
+ * work holds 2 ne x ne */
+int sd_gram_next(const double *Gm, const double *g11, int ne, int c1_second, const double *lmd, double dte,
+                 double *work, double *out, hipStream_t s);
 int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double scal2, double *X, hipStream_t s);
 int sd_lagrange(const LagArgs &a, hipStream_t s);
 int sd_energy(const EnergyArgs &a, hipStream_t s);

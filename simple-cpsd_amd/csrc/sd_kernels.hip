@@ -318,8 +318,9 @@ __global__ void k_erfc_as(const double *__restrict__ w, int n, double *__restric
  * unless c1_second) -> the four ne x ne
  * matrices of ccpsdv3/cpsd.c:415-437 (column-major).  s21 and s12 are each
  * filled from one triangle and mirrored, as the reference does. */
-__global__ void k_smat(const double *__restrict__ Gm, int ne, int c1_second, double dte, double *__restrict__ s11,
-                       double *__restrict__ s12, double *__restrict__ s21, double *__restrict__ s22)
+__global__ void k_smat(const double *__restrict__ Gm, const double *__restrict__ g11c, int ne, int c1_second,
+                       double dte, double *__restrict__ s11, double *__restrict__ s12, double *__restrict__ s21,
+                       double *__restrict__ s22)
 {
    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
    if (idx >= ne * ne) return;
@@ -327,7 +328,7 @@ __global__ void k_smat(const double *__restrict__ Gm, int ne, int c1_second, dou
    const int i = min(r, c), j = max(r, c);
    const int n2 = 2 * ne;
    const int o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;   /* row offsets of C1, C2 */
-   const double g11 = Gm[(size_t)(o1 + i) * n2 + o1 + j];
+   const double g11 = g11c ? g11c[(size_t)i * ne + j] : Gm[(size_t)(o1 + i) * n2 + o1 + j];
    const double g22 = Gm[(size_t)(o2 + i) * n2 + o2 + j];
    const double g21 = Gm[(size_t)(o2 + i) * n2 + o1 + j];      /* <c2_i|c1_j> */
    const double g12 = Gm[(size_t)(o1 + i) * n2 + o2 + j];      /* <c1_i|c2_j> */
@@ -341,6 +342,76 @@ __global__ void k_smat(const double *__restrict__ Gm, int ne, int c1_second, dou
       s21[idx] = -g21 * 0.5;
       s12[idx] = -g12 * 0.5;
       s11[idx] = -g11 * 0.5 * dte;
+   }
+}
+
+/* W = G11 lmd and T1 = G21 lmd (G21[i][j] = <c2_i|c1_j>), both stored as
+ * [i][b].  A workgroup forms 64 outputs (i, b), i fastest; its four thread
+ * groups sum over the four quarters of j and the quarters are added in a
+ * fixed order.  G11[j][i] (symmetric) and the mirrored Gram entry
+ * <c1_j|c2_i> make a wave read consecutive entries; lmd(j, b) = lmd[j + b ne]
+ * is the same for the whole wave. */
+__global__ void __launch_bounds__(256) k_gram_next_w(const double *__restrict__ Gm, const double *__restrict__ g11c,
+                                                     int ne, int c1_second, const double *__restrict__ lmd,
+                                                     double *__restrict__ W, double *__restrict__ T1)
+{
+   __shared__ double red[2][4][64];
+   const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
+   const int idx = blockIdx.x * 64 + l;
+   const bool valid = idx < ne * ne;
+   const int i = idx % ne, b = idx / ne;
+   const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
+   double w = 0.0, t = 0.0;
+   if (valid) {
+      const int q = ne / 4, j0 = g * q;
+      const double *gp = g11c ? g11c + i : Gm + (size_t)o1 * n2 + o1 + i;
+      const size_t gs = g11c ? (size_t)ne : (size_t)n2;
+      const double *tp = Gm + (size_t)o1 * n2 + o2 + i;
+      const double *lp = lmd + (size_t)b * ne;
+      /* unrolled so that a batch of independent loads is in flight at once */
+#pragma unroll 8
+      for (int j = j0; j < j0 + q; ++j) {
+         const double lv = lp[j];
+         w += gp[j * gs] * lv;
+         t += tp[(size_t)j * n2] * lv;
+      }
+   }
+   red[0][g][l] = w;
+   red[1][g][l] = t;
+   __syncthreads();
+   if (g == 0 && valid) {
+      W[(size_t)i * ne + b] = (red[0][0][l] + red[0][1][l]) + (red[0][2][l] + red[0][3][l]);
+      T1[(size_t)i * ne + b] = (red[1][0][l] + red[1][1][l]) + (red[1][2][l] + red[1][3][l]);
+   }
+}
+
+/* out[a][b] = G22[a][b] + dte (T1[a][b] + T1[b][a]) + dte^2 sum_i lmd(i,a) W[i][b]
+ * (T1[b][a] = sum_i lmd(i,a) G21[b][i]); 64 outputs per workgroup, b
+ * fastest, the sum over i in four fixed quarters */
+__global__ void __launch_bounds__(256) k_gram_next(const double *__restrict__ Gm, int ne, int c1_second,
+                                                   const double *__restrict__ lmd, double dte,
+                                                   const double *__restrict__ W, const double *__restrict__ T1,
+                                                   double *__restrict__ out)
+{
+   __shared__ double red[4][64];
+   const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
+   const int idx = blockIdx.x * 64 + l;
+   const bool valid = idx < ne * ne;
+   const int b = idx % ne, a = idx / ne;
+   double t3 = 0.0;
+   if (valid) {
+      const int q = ne / 4, i0 = g * q;
+      const double *la = lmd + (size_t)a * ne;
+#pragma unroll 8
+      for (int i = i0; i < i0 + q; ++i) t3 += la[i] * W[(size_t)i * ne + b];
+   }
+   red[g][l] = t3;
+   __syncthreads();
+   if (g == 0 && valid) {
+      const int n2 = 2 * ne, o2 = c1_second ? 0 : ne;
+      const double s3 = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+      out[(size_t)a * ne + b] = Gm[(size_t)(o2 + a) * n2 + o2 + b] +
+                                dte * (T1[(size_t)a * ne + b] + T1[(size_t)b * ne + a]) + dte * dte * s3;
    }
 }
 
@@ -780,10 +851,24 @@ int sd_erfc_as(const double *w, int n, double *out, hipStream_t s)
    return 0;
 }
 
-int sd_smat(const double *Gm, int ne, int c1_second, double dte, double *s11, double *s12, double *s21,
-            double *s22, hipStream_t s)
+int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *s11, double *s12,
+            double *s21, double *s22, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_smat, g1((size_t)ne * ne), dim3(256), 0, s, Gm, ne, c1_second, dte, s11, s12, s21, s22);
+   hipLaunchKernelGGL(k_smat, g1((size_t)ne * ne), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, s11, s12, s21,
+                      s22);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_gram_next(const double *Gm, const double *g11, int ne, int c1_second, const double *lmd, double dte,
+                 double *work, double *out, hipStream_t s)
+{
+   double *W = work, *T1 = work + (size_t)ne * ne;
+   if (ne % 4) { cpsd_set_error("gram_next: ne=%d is not a multiple of 4", ne); return -1; }
+   const dim3 grid((unsigned)(((size_t)ne * ne + 63) / 64));
+   hipLaunchKernelGGL(k_gram_next_w, grid, dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W, T1);
+   CPSD_CHECK(hipGetLastError());
+   hipLaunchKernelGGL(k_gram_next, grid, dim3(256), 0, s, Gm, ne, c1_second, lmd, dte, W, T1, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

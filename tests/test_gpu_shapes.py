@@ -56,6 +56,16 @@ def test_ne160_vs_oracle(gpu):
     run_pair(4, 14.0, 32, 160, steps=1, inner=1)
 
 
+@pytest.mark.parametrize("ne,steps", [(128, 2), (256, 1)])
+def test_carried_overlap_vs_oracle(gpu, ne, steps):
+    """ne a multiple of 128 over several steps per call: from the second step
+    on, <c1|c1> comes from the previous step's overlaps and Lagrange
+    multipliers (<c2f|c2f> of the constrained orbitals) and the Gram skips its
+    C1 x C1 block; calls continue the carried value"""
+    say("carried <c1|c1>, ne=%d at 24^3" % ne)
+    run_pair(2, 12.0, 24, ne, steps=steps, inner=3, seed=7)
+
+
 def test_grid96_step_vs_oracle(gpu):
     """the 32-water cell of the h2o-32 workload (96^3, 192 projectors) with
     8 orbitals"""
