@@ -107,8 +107,8 @@ __device__ constexpr int dg_bj(int dm, int b)
  * off-diagonal tiles and the listed diagonal tiles, each diagonal tile by a
  * diagonal-pair workgroup whose two halves take the two halves of its K
  * range (rows of the same tile at k and at k + half) and write them as two
- * K splits, 2z and 2z + 1. */
-template<int TMR>
+ * K splits, 2z and 2z + 1 (SELF instances only). */
+template<int TMR, bool SELF>
 __global__ void __launch_bounds__(512)
 k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
                  const double *__restrict__ B, size_t ldb, int N,
@@ -120,7 +120,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    int tm, tn, kz;
    bool dd = false, selfdd = false;
    size_t kc = kchunk;
-   if (symm && tl.on) {
+   if (SELF && symm && tl.on) {
       if (g < poff) {
          const int t = (int)(g % tl.noff);
          kz = (int)(g / tl.noff);
@@ -158,14 +158,14 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    const size_t kb = (size_t)kz * kc;
    const size_t ke = min(K, kb + kc);
    /* a single diagonal tile's second half starts khalf after the first */
-   const size_t khalf = (selfdd && ke > kb) ? ((ke - kb) / KT + 1) / 2 * KT : 0;
-   const size_t kend = selfdd ? min(ke, kb + khalf) : ke;
+   const size_t khalf = (SELF && selfdd && ke > kb) ? ((ke - kb) / KT + 1) / 2 * KT : 0;
+   const size_t kend = (SELF && selfdd) ? min(ke, kb + khalf) : ke;
    /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
     * stages the output tile for the epilogue.  Row pitch KT + 2: the 16 rows
     * x 2 k of a half-wave's operand read fall in distinct bank pairs, and
     * rows stay 16-byte aligned for the stores. */
    __shared__ double S[2][2][TT][KT + 2];
-   __shared__ double Ws[2][2][KT];   /* weights of the A rows ([0]) and B rows ([1]) */
+   __shared__ double Ws[2][SELF ? 2 : 1][KT];   /* weights of the A rows ([0]) and B rows ([1]) */
    static_assert(sizeof(S) >= sizeof(double) * TT * TP, "staging tile does not fit");
    /* the wave index is uniform: read it as a scalar */
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -199,19 +199,31 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    /* the B rows sit khalf further along K (0 except for a single diagonal
     * tile), and are zero past the workgroup's K range */
    auto gload = [&](size_t k0) {
-      const bool bin = k0 + khalf < ke;
+      if constexpr (SELF) {
+         const bool bin = k0 + khalf < ke;
 #pragma unroll
-      for (int h = 0; h < 4; ++h) {
-         const int r = lr + 32 * h, am = tm * TMR + r, bn = tn * TT + r;
-         ra[h] = make_double2(0.0, 0.0);
-         rb[h] = make_double2(0.0, 0.0);
-         if (r < TMR && am < M) ra[h] = *reinterpret_cast<const double2 *>(A + (size_t)am * lda + k0 + lq);
-         if (bn < N && bin) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + khalf + lq);
-      }
-      if (tid < KT) {
-         const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
-         rw = make_double2(0.0, 0.0);
-         if (hb == 0 || bin) rw = *reinterpret_cast<const double2 *>(w + k0 + (hb ? khalf : 0) + kq);
+         for (int h = 0; h < 4; ++h) {
+            const int r = lr + 32 * h, am = tm * TMR + r, bn = tn * TT + r;
+            ra[h] = make_double2(0.0, 0.0);
+            rb[h] = make_double2(0.0, 0.0);
+            if (r < TMR && am < M) ra[h] = *reinterpret_cast<const double2 *>(A + (size_t)am * lda + k0 + lq);
+            if (bn < N && bin) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + khalf + lq);
+         }
+         if (tid < KT) {
+            const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
+            rw = make_double2(0.0, 0.0);
+            if (hb == 0 || bin) rw = *reinterpret_cast<const double2 *>(w + k0 + (hb ? khalf : 0) + kq);
+         }
+      } else {
+#pragma unroll
+         for (int h = 0; h < 4; ++h) {
+            const int r = lr + 32 * h, am = tm * TMR + r, bn = tn * TT + r;
+            ra[h] = make_double2(0.0, 0.0);
+            rb[h] = make_double2(0.0, 0.0);
+            if (r < TMR && am < M) ra[h] = *reinterpret_cast<const double2 *>(A + (size_t)am * lda + k0 + lq);
+            if (bn < N) rb[h] = *reinterpret_cast<const double2 *>(B + (size_t)bn * ldb + k0 + lq);
+         }
+         if (tid < KT / 2) rw = *reinterpret_cast<const double2 *>(w + k0 + 2 * tid);
       }
    };
    auto lstore = [&](int bf) {
@@ -220,10 +232,14 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
          *reinterpret_cast<double2 *>(&S[bf][0][lr + 32 * h][lq]) = ra[h];
          *reinterpret_cast<double2 *>(&S[bf][1][lr + 32 * h][lq]) = rb[h];
       }
-      if (tid < KT) {
-         const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
-         Ws[bf][hb][kq] = rw.x;
-         Ws[bf][hb][kq + 1] = rw.y;
+      if constexpr (SELF) {
+         if (tid < KT) {
+            const int hb = tid / (KT / 2), kq = 2 * (tid % (KT / 2));
+            Ws[bf][hb][kq] = rw.x;
+            Ws[bf][hb][kq + 1] = rw.y;
+         }
+      } else {
+         if (tid < KT / 2) { Ws[bf][0][2 * tid] = rw.x; Ws[bf][0][2 * tid + 1] = rw.y; }
       }
    };
 
@@ -239,7 +255,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto mma_diag = [&](auto role, int bf, int ks) {
       constexpr int DM = decltype(role)::value;
       const int kk = 4 * ks + (lane >> 4);
-      const double wk = Ws[bf][hs][kk];
+      const double wk = Ws[bf][SELF ? hs : 0][kk];
       double a[4], bv[4][4];
 #pragma unroll
       for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kk] * wk;
@@ -325,13 +341,13 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
             else stage(std::integral_constant<int, 2>{});
          }
          __syncthreads();
-         const int t0 = (tm + (selfdd ? 0 : h)) * TT, n = t0 + 2 * lane;
+         const int t0 = (tm + ((SELF && selfdd) ? 0 : h)) * TT, n = t0 + 2 * lane;
          if (n < N) {
             for (int ml = wave; ml < TT; ml += 8) {
                const int m = t0 + ml;
                if (m >= M) break;
                if (2 * lane < 16 * (ml >> 4)) continue;
-               put(m, n, ml, selfdd ? 2 * kz + h : kz);
+               put(m, n, ml, (SELF && selfdd) ? 2 * kz + h : kz);
             }
          }
          __syncthreads();
@@ -429,10 +445,10 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
    GramTiles tl;
    memset(&tl, 0, sizeof tl);
    if (tmr == 96)
-      hipLaunchKernelGGL((k_gemm_tn_splitk<96>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
+      hipLaunchKernelGGL((k_gemm_tn_splitk<96, false>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
                          ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, tl, partial);
    else
-      hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
+      hipLaunchKernelGGL((k_gemm_tn_splitk<TT, false>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, B,
                          ldb, N, w, K, kchunk, kchunk_dd, symm, ntm, ntile, poff, per, tl, partial);
    CPSD_CHECK(hipGetLastError());
    return 0;
@@ -493,8 +509,8 @@ int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *
    const long pairs = poff + (long)tl.nd * sd;
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
    const size_t kchunk = so > 0 ? chunk(so) : 0, kchunk_dd = chunk(sd);
-   hipLaunchKernelGGL((k_gemm_tn_splitk<TT>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, A, lda, M, w,
-                      K, kchunk, kchunk_dd, 1, ntm, 0, poff, per, tl, partial);
+   hipLaunchKernelGGL((k_gemm_tn_splitk<TT, true>), dim3((unsigned)pairs), dim3(512), 0, s, A, lda, M, A, lda, M,
+                      w, K, kchunk, kchunk_dd, 1, ntm, 0, poff, per, tl, partial);
    CPSD_CHECK(hipGetLastError());
    *sp_off = so;
    *sp_dd = 2 * sd;
