@@ -66,6 +66,42 @@ def test_carried_overlap_vs_oracle(gpu, ne, steps):
     run_pair(2, 12.0, 24, ne, steps=steps, inner=3, seed=7)
 
 
+def test_carried_overlap_reset_by_new_orbitals(gpu):
+    """new orbitals drop the carried <c1|c1>: a context that ran, then takes
+    new orbitals, gives the bits of a fresh context; the epilogue in between
+    runs does not disturb the next run (in both precision modes)"""
+    say("carried <c1|c1> reset, ne=128 at 24^3")
+    ne = 128
+    cell = cpsd_amd.water_cell(2, 12.0, 24, seed=7)
+    psi_a = cell.random_orbitals(ne, seed=1, decay=0.05)
+    psi_b = cell.random_orbitals(ne, seed=2, decay=0.05)
+    for mode in (0, 1):
+        dev = cpsd_amd.DeviceSD(cell, ne)
+        dev.set_precision(mode)
+        dev.set_orbitals(psi_a)
+        dev.run(4)
+        dev.set_orbitals(psi_b)
+        E1, _, _ = dev.run(4)
+        E2, _, _ = dev.run(4)
+        c_used = dev.get(c2=True)["c2"]
+        fresh = cpsd_amd.DeviceSD(cell, ne)
+        fresh.set_precision(mode)
+        fresh.set_orbitals(psi_b)
+        F1, _, _ = fresh.run(4)
+        F2, _, _ = fresh.run(4)
+        assert np.array_equal(E1, F1) and np.array_equal(E2, F2)
+        assert np.array_equal(c_used, fresh.get(c2=True)["c2"])
+        # an epilogue between runs leaves the following run as it would be
+        # without it (the next step's c1 is the unrotated current set); its
+        # first step forms the whole Gram again, so agreement is to rounding
+        fresh.epilogue()
+        G3, _, _ = fresh.run(4)
+        H3, _, _ = dev.run(4)
+        assert np.max(np.abs(G3 - H3)) < 1e-10
+        dev.close()
+        fresh.close()
+
+
 def test_grid96_step_vs_oracle(gpu):
     """the 32-water cell of the h2o-32 workload (96^3, 192 projectors) with
     8 orbitals"""
