@@ -100,6 +100,7 @@ def run(args):
     t_setup = time.perf_counter() - t0
 
     dev = cpsd_amd.DeviceSD(cell, w["ne"])
+    dev.set_graphs(args.graphs)
     dev.set_orbitals(psi)
     sizes = dev.sizes()
     inner = 10
@@ -268,7 +269,8 @@ def run(args):
         "config": {"workload": args.workload, "waters": w["nmol"], "atoms": 3 * w["nmol"],
                    "cell_bohr": w["L"], "fft_grid": w["nfft"], "orbitals": w["ne"],
                    "projectors": sizes["nproj"], "masked_G": sizes["ng"], "inner": inner,
-                   "precision": "fp64", "start": "gaussian", "setup_s": round(t_setup, 2)},
+                   "precision": "fp64", "start": "gaussian", "graphs": bool(args.graphs),
+                   "setup_s": round(t_setup, 2)},
         "time_to_converge": conv,
         "mixed": mixed,
         "profiled_pass": {"steps": prof_steps, "ms_per_step": round(elapsed_prof * 1000.0 / prof_steps, 4)},
@@ -292,6 +294,8 @@ def main():
     p.add_argument("--converge-steps", type=int, default=10000)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     p.add_argument("--no-mixed", dest="mixed", action="store_false")
+    p.add_argument("--graphs", action="store_true",
+                   help="replay each 10-step call from a captured HIP graph")
     p.add_argument("--cpu-sample", type=int, default=16)
     args = p.parse_args()
     if args.gpus != 1:

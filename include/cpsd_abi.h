@@ -121,6 +121,14 @@ int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
  * reference has no such mode; its all-FP32 build (float_single.h with
  * sfftpack3d) is the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
+/* enable != 0: each cpsd_dev_run() call (its inner steps and the energy
+ * evaluation) is captured once into a HIP graph and replayed on later calls
+ * with the same inner, dt, fmass, precision mode and state parity, which
+ * saves the per-kernel launch cost of small cells; results are those of the
+ * uncaptured calls, bit for bit.  Calls with moving ions or profiling, and
+ * the first call in each precision mode, run uncaptured.  No counterpart in
+ * the reference. */
+int cpsd_dev_set_graphs(cpsd_dev *h, int enable);
 /* last step's Lagrange iteration count and final max|dX| */
 int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff);
 /* sizes: [0] ng (masked G vectors), [1] ngp, [2] nproj */

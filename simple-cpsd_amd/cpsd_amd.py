@@ -36,7 +36,7 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
-               "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision")
+               "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision", "cpsd_dev_set_graphs")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
                 "cpsdh_write_ionout")
@@ -72,6 +72,7 @@ def hip_lib():
         L.cpsd_dev_get.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.cpsd_dev_lagrange_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DPTR]
         L.cpsd_dev_set_precision.argtypes = [C.c_void_p, C.c_int]
+        L.cpsd_dev_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_sizes.argtypes = [C.c_void_p, _ip]
         L.cpsd_dev_profile.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _ip, C.c_int]
@@ -395,6 +396,11 @@ class DeviceSD:
         """0: FP64 throughout; 1: FP32 orbital FFTs in every step of a run()
         but its last (energies stay FP64 values of the current orbitals)"""
         _check(hip_lib().cpsd_dev_set_precision(self.h, int(mode)), "cpsd_dev_set_precision")
+
+    def set_graphs(self, enable=True):
+        """capture each run() into a HIP graph and replay it on later calls
+        with the same arguments and state parity"""
+        _check(hip_lib().cpsd_dev_set_graphs(self.h, 1 if enable else 0), "cpsd_dev_set_graphs")
 
     def lagrange_info(self):
         it, ad = C.c_int(), C.c_double()

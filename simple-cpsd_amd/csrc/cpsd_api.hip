@@ -110,6 +110,17 @@ struct cpsd_dev {
    double *g11c[2], *g11w;
    int g11_idx, g11_valid;
    long gram_wgs;         /* workgroups of the full Gram launch */
+   /* HIP graphs of whole calls (inner steps + energy tail), keyed on what
+    * the captured launches bake in; see ctx_run */
+   int graphs;
+   int ran_mode[2];       /* an eager call has run in this precision mode */
+   struct GraphEntry {
+      int inner, mixed, cur0, g11i0, g11v0;
+      double dt, dte;
+      int cur1, g11i1, g11v1;
+      hipGraphExec_t exec;
+   } gcache[8];
+   int ngc;
    double2 *dng, *s, *vall, *B;
    double *partial; size_t partial_cap;
    double *Gm, *P, *Xkb, *s11, *s12, *s21, *s22, *LX, *st1, *lmd, *wgmax, *info, *hml, *Mh;
@@ -189,6 +200,7 @@ static void ctx_free(cpsd_dev *c)
    free(c->katm_h);
    free(c->amass_h);
    free(c->Gp_h);
+   for (int i = 0; i < c->ngc; ++i) (void)hipGraphExecDestroy(c->gcache[i].exec);
    if (c->pool) {
       for (int i = 0; i < 2 * c->pool_cap; ++i) (void)hipEventDestroy(c->pool[i]);
       free(c->pool);
@@ -753,9 +765,54 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
       TRY(run_structure(c));
       c->struct_valid = 1;
    }
-   for (int it = 0; it < inner; ++it) TRY(sd_step(c, dte, it == inner - 1));
+   /* graph replay: the steps and the energy tail are device work only (no
+    * host synchronisation, no memset), and every buffer is fixed once the
+    * context is built, so a captured call replays as it ran; the host state
+    * the steps advance (orbital block, carried overlap) is advanced as the
+    * captured run advanced it.  Moving ions, profiling and a precision
+    * mode's first call (one-time kernel attributes, FP32 projectors) run
+    * eagerly. */
+   const bool use_graph = c->graphs && !c->move && !c->prof && c->ran_mode[c->mixed] &&
+                          (!c->mixed || c->bf_valid);
+   cpsd_dev::GraphEntry *ge = nullptr;
+   if (use_graph) {
+      for (int i = 0; i < c->ngc; ++i) {
+         cpsd_dev::GraphEntry &e = c->gcache[i];
+         if (e.inner == inner && e.mixed == c->mixed && e.cur0 == c->cur && e.g11i0 == c->g11_idx &&
+             e.g11v0 == c->g11_valid && e.dt == dt && e.dte == dte) { ge = &e; break; }
+      }
+      if (!ge && c->ngc < 8) {
+         cpsd_dev::GraphEntry e;
+         e.inner = inner; e.mixed = c->mixed; e.cur0 = c->cur; e.g11i0 = c->g11_idx; e.g11v0 = c->g11_valid;
+         e.dt = dt; e.dte = dte;
+         hipGraph_t g = nullptr;
+         CPSD_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+         int rc = 0;
+         for (int it = 0; it < inner && rc == 0; ++it) rc = sd_step(c, dte, it == inner - 1);
+         if (rc == 0) rc = energy_tail(c, inner, dt, dte);
+         const hipError_t ec = hipStreamEndCapture(st, &g);
+         if (rc != 0) { if (g) (void)hipGraphDestroy(g); return rc; }
+         CPSD_CHECK(ec);
+         const hipError_t ei = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+         (void)hipGraphDestroy(g);
+         CPSD_CHECK(ei);
+         e.cur1 = c->cur; e.g11i1 = c->g11_idx; e.g11v1 = c->g11_valid;
+         /* the capture ran the host side of the steps: return to the state
+          * the launch starts from */
+         c->cur = e.cur0; c->g11_idx = e.g11i0; c->g11_valid = e.g11v0;
+         c->gcache[c->ngc++] = e;
+         ge = &c->gcache[c->ngc - 1];
+      }
+   }
+   if (ge) {
+      CPSD_CHECK(hipGraphLaunch(ge->exec, st));
+      c->cur = ge->cur1; c->g11_idx = ge->g11i1; c->g11_valid = ge->g11v1;
+   } else {
+      for (int it = 0; it < inner; ++it) TRY(sd_step(c, dte, it == inner - 1));
+      TRY(energy_tail(c, inner, dt, dte));
+      c->ran_mode[c->mixed] = 1;
+   }
    if (c->move) c->struct_valid = 0;
-   TRY(energy_tail(c, inner, dt, dte));
    double out[12], info[2];
    int err = 0;
    CPSD_CHECK(hipMemcpyAsync(out, c->E, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -982,6 +1039,13 @@ extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
       return -1;
    }
    h->mixed = mode;
+   return 0;
+}
+
+extern "C" int cpsd_dev_set_graphs(cpsd_dev *h, int enable)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   h->graphs = enable ? 1 : 0;
    return 0;
 }
 
