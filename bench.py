@@ -183,6 +183,8 @@ def run(args):
                      ms_per_step=round(el_m * 1000.0 / args.steps, 4),
                      speedup_vs_fp64=round(elapsed / el_m, 4),
                      fp32_steps_per_call=inner - 1, E_last=float(Em[0]),
+                     # both legs ran warmup + steps steps from the same start
+                     abs_E_diff_vs_fp64_per_atom_at_E_last=abs(float(Em[0]) - float(E[0])) / (3 * w["nmol"]),
                      phases={k: round(t / prof_steps, 4) for k, (t, c) in prof_m.items() if c > 0})
         if args.converge_steps > 0:
             cm = converge(1)
