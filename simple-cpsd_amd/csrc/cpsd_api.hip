@@ -604,8 +604,12 @@ static int sd_step(cpsd_dev *c, double dte, int last)
    if (c->nproj > 0) {
       TRY(ph_begin(c, PH_KBPROJ));
       const int sp = splits_for(c->nproj, ne, K, 0);
-      if (f32) {
-         TRY(ensure_bf(c));
+      const int small = gemm_tn_small_ok(c->nproj, ne);
+      if (f32) TRY(ensure_bf(c));
+      if (small) {
+         TRY(launch_gemm_tn_small(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K, 0,
+                                  c->P, st));
+      } else if (f32) {
          TRY(launch_kbproj_f32(c->Bf, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
       } else {
          TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K,
@@ -613,7 +617,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       }
       TRY(ph_end(c, PH_KBPROJ));
       TRY(ph_begin(c, PH_KBRED));
-      TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
+      if (!small) TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
       TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
       TRY(ph_end(c, PH_KBRED));
    }
@@ -662,9 +666,12 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       const int sp = splits_for(2 * ne, 2 * ne, K, 1);
       const double *Cb = reinterpret_cast<const double *>(c->Cbase);
       int so = 0, sdd = 0;
+      const int small = !carry && gemm_tn_small_ok(2 * ne, 2 * ne);
       if (carry) {
          TRY(launch_gram_c2(Cb, K, 2 * ne, (c->cur ^ 1) * ne, c->w, K, c->gram_wgs, c->partial, c->partial_cap,
                             &so, &sdd, st));
+      } else if (small) {
+         TRY(launch_gemm_tn_small(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, 1, c->Gm, st));
       } else {
          TRY(launch_gemm_tn_splitk(Cb, K, 2 * ne, Cb, K, 2 * ne, c->w, K, sp, 1, c->partial, st));
       }
@@ -673,7 +680,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       /* diagonal tiles hold their upper 16 x 16 blocks only */
       if (carry) TRY(launch_reduce_partials_split(c->partial, so, sdd, 2 * ne, c->cur * ne, (c->cur + 1) * ne,
                                                   c->Gm, st));
-      else TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
+      else if (!small) TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
       TRY(sd_smat(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->s11, c->s12, c->s21,
                   c->s22, st));
       TRY(ph_end(c, PH_GRAMRED));
@@ -721,10 +728,15 @@ static int energy_tail(cpsd_dev *c, int inner, double dt, double dte)
    TRY(ph_begin(c, PH_ENERGY));
    {
       const int sp = splits_for(ne, ne, K, 0);
-      TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(C1), K, ne,
-                                reinterpret_cast<const double *>(c->CP), K, ne, c->w, K, sp, 0,
-                                c->partial, st));
-      TRY(launch_reduce_partials(c->partial, sp, ne, ne, c->Mh, 0, st));
+      if (gemm_tn_small_ok(ne, ne)) {
+         TRY(launch_gemm_tn_small(reinterpret_cast<const double *>(C1), K, ne,
+                                  reinterpret_cast<const double *>(c->CP), K, ne, c->w, K, 0, c->Mh, st));
+      } else {
+         TRY(launch_gemm_tn_splitk(reinterpret_cast<const double *>(C1), K, ne,
+                                   reinterpret_cast<const double *>(c->CP), K, ne, c->w, K, sp, 0,
+                                   c->partial, st));
+         TRY(launch_reduce_partials(c->partial, sp, ne, ne, c->Mh, 0, st));
+      }
       TRY(sd_hml(c->Mh, ne, c->hml, st));
    }
    TRY(sd_gsums(c->s, c->dng, c->vall, c->vg, c->vc, c->w, ngp, c->gsum_part, c->ngsum_parts, st));
@@ -956,8 +968,12 @@ static int ctx_epilogue(cpsd_dev *c, double *ortho, double *eig, double *V)
          return -1;
       }
       const double *c1r = reinterpret_cast<const double *>(C1);
-      TRY(launch_gemm_tn_splitk(c1r, K, ne, c1r, K, ne, c->w, K, sp, 1, c->partial, st));
-      TRY(launch_reduce_partials_symm(c->partial, ne, K, sp, c->Gm, st));
+      if (gemm_tn_small_ok(ne, ne)) {
+         TRY(launch_gemm_tn_small(c1r, K, ne, c1r, K, ne, c->w, K, 1, c->Gm, st));
+      } else {
+         TRY(launch_gemm_tn_splitk(c1r, K, ne, c1r, K, ne, c->w, K, sp, 1, c->partial, st));
+         TRY(launch_reduce_partials_symm(c->partial, ne, K, sp, c->Gm, st));
+      }
       CPSD_CHECK(hipMemcpyAsync(ortho, c->Gm, ee * sizeof(double), hipMemcpyDeviceToHost, st));
    }
    std::vector<double> a(ee), w(ne);

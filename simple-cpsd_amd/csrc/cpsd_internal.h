@@ -101,6 +101,11 @@ int plane_density_occupancy(int n, int fp32);
  * (A: M rows of stride lda, B: N rows of stride ldb; K multiple of 32).
  * symm (A == B, M == N): only the 16 x 16 blocks on or above the diagonal
  * are formed.                                                             */
+/* out[m][n] = sum_k w[k] A[m][k] B[n][k] directly (no partials) for the
+ * small products gemm_tn_small_ok() admits; symm: out is written in full */
+int gemm_tn_small_ok(int M, int N);
+int launch_gemm_tn_small(const double *A, size_t lda, int M, const double *B, size_t ldb, int N,
+                         const double *w, size_t K, int symm, double *out, hipStream_t s);
 /* tile list of a symmetric product restricted to some tiles (k_gemm_tn_splitk) */
 struct GramTiles {
    int on, noff, nd;

@@ -589,10 +589,7 @@ int launch_reduce_partials_symm(const double *partial, int M, size_t K, int spli
    return 0;
 }
 
-/* ---------------------------------------------------------- tall NN */
-
-#define TK 128  /* output rows (K) per workgroup */
-#define MC 16   /* reduction step */
+/* ------------------------------------------------- small M x N (TN) */
 
 /* sum over the 64 lanes of a wave, same tree in every wave */
 __device__ __forceinline__ double wave_sum(double v)
@@ -601,6 +598,69 @@ __device__ __forceinline__ double wave_sum(double v)
    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
    return v;
 }
+
+/* out[m][n] = sum_k w[k] A[m][k] B[n][k] for small M, N: one 1024-thread
+ * workgroup per output (per upper-triangle pair m <= n when symm, mirrored),
+ * its 16 waves taking interleaved 16-byte pieces of K so that many loads are
+ * in flight; wave sums by a fixed shuffle tree, then the 16 wave sums in
+ * order: no split-K partials and no reduction launch */
+__global__ void __launch_bounds__(1024) k_gemm_tn_small(const double *__restrict__ A, size_t lda, int M,
+                                                        const double *__restrict__ B, size_t ldb, int N,
+                                                        const double *__restrict__ w, size_t K, int symm,
+                                                        double *__restrict__ out)
+{
+   __shared__ double ws[16];
+   const int o = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+   int m, n;
+   if (symm) {
+      int r = o;
+      m = 0;
+      while (r >= M - m) { r -= M - m; ++m; }
+      n = m + r;
+   } else {
+      m = o / N;
+      n = o % N;
+   }
+   const double *a = A + (size_t)m * lda, *b = B + (size_t)n * ldb;
+   double s0 = 0.0, s1 = 0.0;
+#pragma unroll 2
+   for (size_t k = 2 * (size_t)t; k < K; k += 2048) {
+      const double2 av = *reinterpret_cast<const double2 *>(a + k);
+      const double2 bv = *reinterpret_cast<const double2 *>(b + k);
+      const double2 wk = *reinterpret_cast<const double2 *>(w + k);
+      s0 += (wk.x * av.x) * bv.x;
+      s1 += (wk.y * av.y) * bv.y;
+   }
+   const double v = wave_sum(s0 + s1);
+   if (lane == 0) ws[wv] = v;
+   __syncthreads();
+   if (t == 0) {
+      double tot = 0.0;
+      for (int i = 0; i < 16; ++i) tot += ws[i];
+      out[(size_t)m * N + n] = tot;
+      if (symm) out[(size_t)n * N + m] = tot;
+   }
+}
+
+int gemm_tn_small_ok(int M, int N) { return M <= 16 && N <= 16; }
+
+int launch_gemm_tn_small(const double *A, size_t lda, int M, const double *B, size_t ldb, int N,
+                         const double *w, size_t K, int symm, double *out, hipStream_t s)
+{
+   if (K % 2) { cpsd_set_error("gemm_tn_small: K=%zu is odd", K); return -1; }
+   if (symm && M != N) { cpsd_set_error("gemm_tn_small: symmetric product needs M == N"); return -1; }
+   const long outs = symm ? (long)M * (M + 1) / 2 : (long)M * N;
+   hipLaunchKernelGGL(k_gemm_tn_small, dim3((unsigned)outs), dim3(1024), 0, s, A, lda, M, B, ldb, N, w, K, symm,
+                      out);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* ---------------------------------------------------------- tall NN */
+
+#define TK 128  /* output rows (K) per workgroup */
+#define MC 16   /* reduction step */
+
 
 /* waves arranged WK (along K) x WN (along the output columns) = 8; at most
  * 128 VGPRs and 68 KB of LDS, so that two workgroups share a CU and one's

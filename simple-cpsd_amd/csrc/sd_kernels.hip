@@ -445,6 +445,14 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int
    __syncthreads();
 }
 
+/* a grid barrier, or a workgroup barrier when the grid is one workgroup
+ * (global writes of the workgroup are then visible after __syncthreads) */
+__device__ __forceinline__ void lag_barrier(unsigned *bar, unsigned nbar, int nwg, int *err)
+{
+   if (nwg == 1) __syncthreads();
+   else grid_barrier(bar, nbar * nwg, err);
+}
+
 /* C_tile += A * B over k (column-major ne x ne), TS x TS tile at (ti, tj);
  * one or more outputs per thread accumulated in acc[]. */
 template<int TS, int PER>
@@ -500,7 +508,7 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
       const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
       if (r < a.ne && c < a.ne) a.X[r + (size_t)c * a.ne] = a.s22[r + (size_t)c * a.ne];
    }
-   grid_barrier(a.bar, (++nbar) * nwg, a.err);
+   lag_barrier(a.bar, ++nbar, nwg, a.err);
    while (it < 30) {
       const double *X = a.X + (size_t)(it & 1) * nn;
       double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
@@ -512,7 +520,7 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
          const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
          if (r < a.ne && c < a.ne) a.st1[r + (size_t)c * a.ne] = acc[q];
       }
-      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
       for (int q = 0; q < PER; ++q) acc[q] = 0.0;
       tile_mm<TS, PER>(a.s21, X, a.ne, ti, tj, acc, sa, sb);
       tile_mm<TS, PER>(X, a.s12, a.ne, ti, tj, acc, sa, sb);
@@ -534,7 +542,7 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
          __syncthreads();
       }
       if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
-      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
       gmax = 0.0;
       for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
       if (gmax < 1e-15) break;
@@ -607,7 +615,7 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
       const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
       if (r < ne && c < ne) a.X[r + (size_t)c * ne] = a.s22[r + (size_t)c * ne];
    }
-   grid_barrier(a.bar, (++nbar) * nwg, a.err);
+   lag_barrier(a.bar, ++nbar, nwg, a.err);
    while (it < 30) {
       const double *X = a.X + (size_t)(it & 1) * nn;
       double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
@@ -621,7 +629,7 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
          const int r = ti * TS + rl, c = tj * TS + cl;
          if (r < ne && c < ne) a.st1[r + (size_t)c * ne] = dot(s11r, xc, rl, cl);
       }
-      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
       load_cols(a.st1, stc);
       __syncthreads();
       /* X_new = s22 + s21 X + X s12 + X st1 */
@@ -643,7 +651,7 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
          __syncthreads();
       }
       if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
-      grid_barrier(a.bar, (++nbar) * nwg, a.err);
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
       gmax = 0.0;
       for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
       if (gmax < 1e-15) break;
