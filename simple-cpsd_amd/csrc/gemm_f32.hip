@@ -7,7 +7,8 @@
  *                   the splits reduced in FP64 by k_reduce_partials;
  *   k_kbapply_f32 : C2[n][k] += dte sum_p X[p][n] Bf[p][k] (the nonlocal
  *                   part of the SD update), the sum in FP32, the update of
- *                   the FP64 orbitals in FP64.
+ *                   the FP64 orbitals in FP64; 128 reals x 128 orbitals per
+ *                   workgroup.
  * Bf is an FP32 copy of the projectors, made once per set of ion positions;
  * the orbitals stay FP64 in HBM and are rounded to FP32 while they are
  * staged in LDS.
@@ -165,17 +166,20 @@ int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const doub
 
 /* ----------------------------------------------------------- apply */
 
-#define AK 256          /* output reals (k) per workgroup */
 #define AM 16           /* projectors per step */
 #define AN 128          /* output columns (orbitals) per workgroup */
-#define BP (AK + 4)     /* Bs row pitch: the 4 rows 4 apart one read touches land 16 banks apart */
 #define XP (AM + 4)     /* Xs row pitch: a 16-byte read of 16 rows hits distinct banks */
 
-/* 8 waves as 2 (columns) x 4 (k), each 64 columns x 64 k = 4 x 4 blocks */
-__global__ void __launch_bounds__(512, 4)
+/* AKT output reals (k) per workgroup, 8 waves as 2 (columns) x 4 (k), each
+ * 64 columns x AKT/4 k = 4 x AKT/64 blocks; the Bs row pitch AKT + 4 puts the
+ * 4 rows 4 apart that one read touches 16 banks apart.  WAVES: waves per
+ * SIMD the registers are sized for. */
+template<int AKT, int WAVES>
+__global__ void __launch_bounds__(512, WAVES)
 k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
               int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per)
 {
+   constexpr int BP = AKT + 4, JB = AKT / 64, WKE = AKT / 4, BH = AKT / 128;
    /* main loop: Bs[2][AM][BP] and Xs[2][AN][XP]; epilogue: T[64][BP], one
     * 64-column half of the output tile at a time */
    constexpr int BSZ = 2 * AM * BP, XSZ = 2 * AN * XP, TSZ = 64 * BP;
@@ -184,28 +188,28 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
    float (*Xs)[AN][XP] = reinterpret_cast<float (*)[AN][XP]>(lds + BSZ);
    const int bid = blockIdx.x;
    const int g = per > 0 ? (bid & 7) * per + (bid >> 3) : bid;
-   const size_t k0 = (size_t)(g / ntn) * AK;
+   const size_t k0 = (size_t)(g / ntn) * AKT;
    const int n0 = (g % ntn) * AN;
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wn = wave & 1, wk = wave >> 1;
    const int q = lane >> 4, rr = lane & 15;
-   /* loaders: Bs row br, 8 k at bc (two 16-byte pieces; 32 lanes per 1 KB
-    * row); Xs column xn, 4 projectors at xp */
-   const int br = tid >> 5, bc = (tid & 31) * 8;
+   /* loaders: Bs row br, BH 16-byte pieces from bc (32 lanes per row); Xs
+    * column xn, 4 projectors at xp */
+   const int br = tid >> 5, bc = (tid & 31) * 4 * BH;
    const int xn = tid >> 2, xp = (tid & 3) * 4;
 
-   f4 acc[4][4];   /* [column block][k block] */
+   f4 acc[4][JB];   /* [column block][k block] */
 #pragma unroll
    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int j = 0; j < JB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-   float4 rb[2];
+   float4 rb[BH];
    double rx[4];
    auto gload = [&](int m0) {
       const int m = m0 + br;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < BH; ++h) {
          const size_t k = k0 + bc + 4 * h;
          rb[h] = (m < Mr && k < K) ? *reinterpret_cast<const float4 *>(Bf + (size_t)m * ldb + k)
                                    : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -219,7 +223,7 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
    };
    auto lstore = [&](int bf) {
 #pragma unroll
-      for (int h = 0; h < 2; ++h) *reinterpret_cast<float4 *>(&Bs[bf][br][bc + 4 * h]) = rb[h];
+      for (int h = 0; h < BH; ++h) *reinterpret_cast<float4 *>(&Bs[bf][br][bc + 4 * h]) = rb[h];
       *reinterpret_cast<float4 *>(&Xs[bf][xn][xp]) =
          make_float4((float)rx[0], (float)rx[1], (float)rx[2], (float)rx[3]);
    };
@@ -230,13 +234,13 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
       for (int i = 0; i < 4; ++i) x[i] = *reinterpret_cast<const f4 *>(&Xs[bf][64 * wn + 16 * i + rr][4 * q]);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-         float b[4];
+         float b[JB];
 #pragma unroll
-         for (int j = 0; j < 4; ++j) b[j] = Bs[bf][4 * q + c][64 * wk + 16 * j + rr];
+         for (int j = 0; j < JB; ++j) b[j] = Bs[bf][4 * q + c][WKE * wk + 16 * j + rr];
 #pragma unroll
          for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(x[i][c], b[j], acc[i][j]);
+            for (int j = 0; j < JB; ++j) acc[i][j] = mfma16(x[i][c], b[j], acc[i][j]);
       }
    };
 
@@ -254,8 +258,9 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
    }
 
    /* epilogue per 64-column half: the half's waves stage their sums in LDS
-    * (lane (rr, q), register r holds column 16 i + 4 q + r, k = 16 j + rr),
-    * then every wave updates whole 2 KB rows of C2, 32 bytes per lane */
+    * (lane (rr, q), register r holds column 16 i + 4 q + r, k = 16 j + rr
+    * of the wave's range), then every wave updates whole rows of C2, AKT/64
+    * reals per lane */
    float (*T)[BP] = reinterpret_cast<float (*)[BP]>(lds);
 #pragma unroll
    for (int half = 0; half < 2; ++half) {
@@ -264,37 +269,49 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
 #pragma unroll
          for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < JB; ++j)
 #pragma unroll
-               for (int r = 0; r < 4; ++r) T[16 * i + 4 * q + r][64 * wk + 16 * j + rr] = acc[i][j][r];
+               for (int r = 0; r < 4; ++r) T[16 * i + 4 * q + r][WKE * wk + 16 * j + rr] = acc[i][j][r];
       }
       __syncthreads();
-      const size_t k = k0 + 4 * lane;
+      const size_t k = k0 + JB * lane;
 #pragma unroll 4
       for (int nl = wave; nl < 64; nl += 8) {
          const int n = n0 + 64 * half + nl;
          if (n < Nc && k < K) {
-            const float4 v = *reinterpret_cast<const float4 *>(&T[nl][4 * lane]);
             double2 *p = reinterpret_cast<double2 *>(C2 + (size_t)n * ldc + k);
-            double2 a = p[0], b = p[1];
-            a.x += dte * v.x; a.y += dte * v.y;
-            b.x += dte * v.z; b.y += dte * v.w;
-            p[0] = a;
-            p[1] = b;
+            if constexpr (JB == 4) {
+               const float4 v = *reinterpret_cast<const float4 *>(&T[nl][4 * lane]);
+               double2 a = p[0], b = p[1];
+               a.x += dte * v.x; a.y += dte * v.y;
+               b.x += dte * v.z; b.y += dte * v.w;
+               p[0] = a;
+               p[1] = b;
+            } else {
+               const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
+               double2 a = p[0];
+               a.x += dte * v.x; a.y += dte * v.y;
+               p[0] = a;
+            }
          }
       }
    }
 }
 
+/* 128 reals per workgroup: 32 accumulator registers per lane and 37 KB of
+ * LDS let three workgroups share a CU (measured 14% faster than 256 reals
+ * at two per CU on the 32-water cell: the C2 update of one workgroup
+ * overlaps the products of the others) */
 int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,
                        double *C2, size_t ldc, double dte, hipStream_t s)
 {
+   constexpr int AKT = 128;
    if (K % 8) { cpsd_set_error("kbapply_f32: K=%zu not a multiple of 8", K); return -1; }
    const int ntn = (Nc + AN - 1) / AN;
-   const long blocks = (long)((K + AK - 1) / AK) * ntn;
+   const long blocks = (long)((K + AKT - 1) / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL(k_kbapply_f32, dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X, xsm, ldx, Nc, K,
-                      C2, ldc, dte, ntn, per);
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X, xsm, ldx,
+                      Nc, K, C2, ldc, dte, ntn, per);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
