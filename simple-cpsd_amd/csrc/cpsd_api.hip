@@ -1015,11 +1015,19 @@ extern "C" int cpsd_dev_create(cpsd_dev **h, int nfft, int ne, int nion, const i
    return 0;
 }
 
-extern "C" int cpsd_dev_set_orbitals(cpsd_dev *h, const double *c) { return ctx_set_orbitals(h, c); }
+extern "C" int cpsd_dev_set_orbitals(cpsd_dev *h, const double *c)
+{
+   if (!h || !c) { cpsd_set_error("cpsd_dev_set_orbitals: null handle or orbitals"); return -1; }
+   return ctx_set_orbitals(h, c);
+}
 
 extern "C" int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E,
                             double *deltae, double *deltac, double *deltar)
 {
+   if (!h || !E || !deltae || !deltac || !deltar) {
+      cpsd_set_error("cpsd_dev_run: null handle or output");
+      return -1;
+   }
    return ctx_run(h, inner, dt, fmass, E, deltae, deltac, deltar);
 }
 
@@ -1038,6 +1046,7 @@ extern "C" int cpsd_dev_get_ions(cpsd_dev *h, double *r1, double *r2, double *fi
 extern "C" int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn,
                             double *hml, double *lmd)
 {
+   if (!h) { cpsd_set_error("null handle"); return -1; }
    return ctx_get(h, c1, c2, cpsi, dn, hml, lmd);
 }
 
@@ -1067,6 +1076,7 @@ extern "C" int cpsd_dev_set_graphs(cpsd_dev *h, int enable)
 
 extern "C" int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff)
 {
+   if (!h || !iterations || !adiff) { cpsd_set_error("cpsd_dev_lagrange_info: null handle or output"); return -1; }
    *iterations = h->last_lag_iters;
    *adiff = h->last_lag_adiff;
    return 0;
@@ -1074,12 +1084,14 @@ extern "C" int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adif
 
 extern "C" int cpsd_dev_sizes(cpsd_dev *h, int *sizes)
 {
+   if (!h || !sizes) { cpsd_set_error("cpsd_dev_sizes: null handle or output"); return -1; }
    sizes[0] = h->ng; sizes[1] = h->ngp; sizes[2] = h->nproj;
    return 0;
 }
 
 extern "C" int cpsd_dev_profile(cpsd_dev *h, int enable)
 {
+   if (!h) { cpsd_set_error("null handle"); return -1; }
    TRY(ph_flush(h));
    if (enable && !h->pool) {
       h->pool_cap = 2048;
@@ -1096,6 +1108,7 @@ extern "C" int cpsd_dev_profile(cpsd_dev *h, int enable)
 extern "C" int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, double *ms, int *calls,
                                      int max_phases)
 {
+   if (!h) { cpsd_set_error("null handle"); return -1; }
    std::string s;
    int k = 0;
    TRY(ph_flush(h));
@@ -1116,6 +1129,7 @@ extern "C" int cpsd_dev_profile_read(cpsd_dev *h, char *names, int names_len, do
  * of doubles written, or -1 */
 extern "C" long cpsd_dev_debug_fetch(cpsd_dev *h, int what, double *out, long cap)
 {
+   if (!h) { cpsd_set_error("null handle"); return -1; }
    const size_t orb = (size_t)h->ne * h->ngp;
    const void *src = nullptr;
    size_t nd = 0;

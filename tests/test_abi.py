@@ -35,3 +35,34 @@ def test_dropin_signature_is_the_reference_one():
     txt = open(os.path.join(cpsd_amd.ROOT, "include", "cpsd_abi.h")).read()
     m = re.search(r"void cpsd\((.*?)\);", txt, flags=re.S)
     assert m and len(m.group(1).split(",")) == 42
+
+
+def test_resident_api_rejects_a_null_handle():
+    """every cpsd_dev_* call on a null handle returns an error with a message
+    and touches no device (host logic only)"""
+    import ctypes as C
+    import numpy as np
+    L = cpsd_amd.hip_lib()
+    d = np.zeros(16)
+    i = np.zeros(4, dtype=np.int32)
+    z = C.c_double()
+    k = C.c_int()
+    calls = [
+        lambda: L.cpsd_dev_set_orbitals(None, d),
+        lambda: L.cpsd_dev_run(None, 1, 5.8, 4e5, d, C.byref(z), C.byref(z), C.byref(z)),
+        lambda: L.cpsd_dev_set_ions(None, 0, d, d),
+        lambda: L.cpsd_dev_get_ions(None, None, None, None),
+        lambda: L.cpsd_dev_get(None, None, None, None, None, None, None),
+        lambda: L.cpsd_dev_epilogue(None, None, None, None),
+        lambda: L.cpsd_dev_set_precision(None, 1),
+        lambda: L.cpsd_dev_set_graphs(None, 1),
+        lambda: L.cpsd_dev_lagrange_info(None, C.byref(k), C.byref(z)),
+        lambda: L.cpsd_dev_sizes(None, i),
+        lambda: L.cpsd_dev_profile(None, 1),
+        lambda: L.cpsd_dev_profile_read(None, C.create_string_buffer(64), 64, d, i, 4),
+        lambda: L.cpsd_dev_debug_fetch(None, 0, d, 16),
+    ]
+    for f in calls:
+        assert f() == -1
+        assert "null" in cpsd_amd._err()
+    L.cpsd_dev_destroy(None)
