@@ -1,7 +1,7 @@
 """GPU parity of the HIP SD step against the oracle, through the C-ABI.
 
 Bars: FP64 everywhere; energies and their components within 1e-9 Ha of the
-oracle after a fixed number of steps (the north-star bar is 1e-8), densities
+oracle after a fixed number of steps (the energy bar for parity is 1e-8), densities
 within 1e-10 max-abs, FFTs within 1e-12 relative to the transform's scale.
 """
 import os
