@@ -18,10 +18,11 @@ steps right after it (the events cost ~2% of a step, so they stay out of
 |E(k) - E(k-1)| < 1e-7 Ha between energy evaluations (10 steps apart).
 
 The "mixed" object repeats the steps/s, phase and time-to-converge legs in
-the FP32-transform mode (cpsd_dev_set_precision(h, 1): the orbital FFTs of
-9 of every 10 steps on FP32 values, everything else and every energy
-evaluation FP64) and reports its converged energy's error against the FP64
-run's, per atom.  `value` stays the FP64 rate.
+the FP32-transform mode (cpsd_dev_set_precision(h, 1): the orbital FFTs and
+KB products of 9 of every 10 steps on FP32 values, everything else and every
+energy evaluation FP64) and reports its converged energy's error against the
+FP64 run's, per atom; "mixed.bf16_projection" does the same for mode 2 (the
+KB projection with BF16 inputs).  `value` stays the FP64 rate.
 """
 import argparse
 import json
@@ -153,7 +154,7 @@ def run(args):
                 break
             if nsteps % 1000 == 0 and Eprev is not None:
                 print("converge(%s): %d steps, E %.10f, dE %.3e" %
-                      ("mixed" if mode else "fp64", nsteps, Ec[0], Ec[0] - Eprev),
+                      (("fp64", "mixed", "mixed-bf16")[mode], nsteps, Ec[0], Ec[0] - Eprev),
                       file=sys.stderr, flush=True)
             last = None if Eprev is None else Ec[0] - Eprev
             Eprev = Ec[0]
@@ -192,6 +193,24 @@ def run(args):
                 cm["abs_E_error_vs_fp64_per_atom"] = abs(cm["E"] - conv["E"]) / (3 * w["nmol"])
                 cm["speedup_vs_fp64"] = round(conv["seconds"] / cm["seconds"], 4)
             mixed["time_to_converge"] = cm
+        # mode 2: the same with a BF16-input KB projection
+        dev.set_orbitals(psi)
+        dev.set_precision(2)
+        steps(args.warmup)
+        t0 = time.perf_counter()
+        Eb, _, _ = steps(args.steps)
+        el_b = time.perf_counter() - t0
+        dev.set_precision(0)
+        mixed["bf16_projection"] = dict(
+            value=round(args.steps / el_b, 3), unit="SD steps/s", ms_per_step=round(el_b * 1000.0 / args.steps, 4),
+            speedup_vs_fp64=round(elapsed / el_b, 4),
+            abs_E_diff_vs_fp64_per_atom_at_E_last=abs(float(Eb[0]) - float(E[0])) / (3 * w["nmol"]))
+        if args.converge_steps > 0:
+            cb = converge(2)
+            if conv is not None and conv["converged"] and cb["converged"]:
+                cb["abs_E_error_vs_fp64_per_atom"] = abs(cb["E"] - conv["E"]) / (3 * w["nmol"])
+                cb["speedup_vs_fp64"] = round(conv["seconds"] / cb["seconds"], 4)
+            mixed["bf16_projection"]["time_to_converge"] = cb
     dev.close()
 
     ms = elapsed * 1000.0 / args.steps

@@ -112,14 +112,17 @@ int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn, 
 int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
 /* precision of the orbital transforms: mode 0 runs every step in FP64 (the
  * default, the reference's float_double.h build); mode 1 runs the 3D FFTs of
- * the orbitals (G->r, the density sum, V psi and r->G) on FP32 values in
+ * the orbitals (G->r, the density sum, V psi and r->G) and the KB projection
+ * and apply (FP32 MFMA products) on FP32 values in
  * every step of a cpsd_dev_run() call but its last, while the density
- * accumulation, the potentials, the projector, overlap and constraint
- * products, the Lagrange iteration and the energies stay FP64.  The last
+ * accumulation, the potentials, the overlap and constraint products, the
+ * Lagrange iteration and the energies stay FP64.  The last
  * step, whose density and -H psi the energies are formed from, is FP64, so
- * E, deltae and deltac are FP64 values of the current orbitals.  The
- * reference has no such mode; its all-FP32 build (float_single.h with
- * sfftpack3d) is the nearest. */
+ * E, deltae and deltac are FP64 values of the current orbitals.  Mode 2 is
+ * mode 1 with the KB projection's inputs rounded to BF16 (FP32 sums), which
+ * moves the converged energy by up to a few 1e-7 Ha per atom.  The reference
+ * has no such mode; its all-FP32 build (float_single.h with sfftpack3d) is
+ * the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
 /* enable != 0: each cpsd_dev_run() call (its inner steps and the energy
  * evaluation) is captured once into a HIP graph and replayed on later calls

@@ -393,8 +393,9 @@ class DeviceSD:
         return dict(ortho=ortho.reshape(ne, ne), eig=eig, V=V)
 
     def set_precision(self, mode):
-        """0: FP64 throughout; 1: FP32 orbital FFTs in every step of a run()
-        but its last (energies stay FP64 values of the current orbitals)"""
+        """0: FP64 throughout; 1: FP32 orbital FFTs and KB products in every
+        step of a run() but its last (energies stay FP64 values of the current
+        orbitals); 2: as 1 with a BF16-input KB projection"""
         _check(hip_lib().cpsd_dev_set_precision(self.h, int(mode)), "cpsd_dev_set_precision")
 
     def set_graphs(self, enable=True):

@@ -100,11 +100,13 @@ struct cpsd_dev {
    double *dnpart, *dn, *xcp, *xce, *V;
    int ndnblk[2], nob[2], fb;   /* density blocks of the FP64 [0] and FP32 [1] pass */
    /* precision of the orbital transforms and KB products of every step but
-    * the last of a call: 0 FP64, 1 FP32 (the last step, which feeds the
-    * energy, is FP64) */
+    * the last of a call: 0 FP64; 1 FP32; 2 FP32 with a BF16-input KB
+    * projection (the last step, which feeds the energy, is FP64) */
    int mixed;
    float *Bf;             /* FP32 copy of the projectors B, [nproj][2 ngp] */
    int bf_valid;          /* Bf matches B */
+   __bf16 *Bb;            /* BF16 copy of the projectors (mode 2) */
+   int bb_valid;
    /* <c1|c1> of the next step, carried from this step's overlaps and lmd
     * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
    double *g11c[2], *g11w;
@@ -113,7 +115,7 @@ struct cpsd_dev {
    /* HIP graphs of whole calls (inner steps + energy tail), keyed on what
     * the captured launches bake in; see ctx_run */
    int graphs;
-   int ran_mode[2];       /* an eager call has run in this precision mode */
+   int ran_mode[3];       /* an eager call has run in this precision mode */
    struct GraphEntry {
       int inner, mixed, cur0, g11i0, g11v0;
       double dt, dte;
@@ -194,7 +196,7 @@ static void ctx_free(cpsd_dev *c)
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
-                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w};
+                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->Bb};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
@@ -487,6 +489,19 @@ static int run_structure(cpsd_dev *c)
    a.s = c->s; a.vall = c->vall; a.B = c->B;
    TRY(sd_structure(a, st));
    c->bf_valid = 0;
+   c->bb_valid = 0;
+   return 0;
+}
+
+/* the BF16 projectors of mode 2's projection */
+static int ensure_bb(cpsd_dev *c)
+{
+   const size_t n = (size_t)c->nproj * 2 * c->ngp;
+   if (!c->Bb) TRY(dalloc(&c->Bb, n));
+   if (!c->bb_valid) {
+      TRY(launch_to_bf16(reinterpret_cast<const double *>(c->B), c->Bb, n, c->stream));
+      c->bb_valid = 1;
+   }
    return 0;
 }
 
@@ -606,9 +621,12 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       const int sp = splits_for(c->nproj, ne, K, 0);
       const int small = gemm_tn_small_ok(c->nproj, ne);
       if (f32) TRY(ensure_bf(c));
+      if (f32 && c->mixed == 2) TRY(ensure_bb(c));
       if (small) {
          TRY(launch_gemm_tn_small(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K, 0,
                                   c->P, st));
+      } else if (f32 && c->mixed == 2) {
+         TRY(launch_kbproj_bf16(c->Bb, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
       } else if (f32) {
          TRY(launch_kbproj_f32(c->Bf, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
       } else {
@@ -785,7 +803,7 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
     * mode's first call (one-time kernel attributes, FP32 projectors) run
     * eagerly. */
    const bool use_graph = c->graphs && !c->move && !c->prof && c->ran_mode[c->mixed] &&
-                          (!c->mixed || c->bf_valid);
+                          (!c->mixed || c->bf_valid) && (c->mixed != 2 || c->bb_valid);
    cpsd_dev::GraphEntry *ge = nullptr;
    if (use_graph) {
       for (int i = 0; i < c->ngc; ++i) {
@@ -1059,8 +1077,9 @@ extern "C" int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double
 extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
 {
    if (!h) { cpsd_set_error("null handle"); return -1; }
-   if (mode != 0 && mode != 1) {
-      cpsd_set_error("cpsd_dev_set_precision: mode %d is not 0 (FP64) or 1 (FP32 orbital FFTs)", mode);
+   if (mode < 0 || mode > 2) {
+      cpsd_set_error("cpsd_dev_set_precision: mode %d is not 0 (FP64), 1 (FP32) or 2 (FP32, BF16 projection)",
+                     mode);
       return -1;
    }
    h->mixed = mode;

@@ -164,6 +164,11 @@ int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const doub
 int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,
                        double *C2, size_t ldc, double dte, hipStream_t s);
 int launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+/* the projection with BF16 projectors Bb and C1 w rounded to BF16, FP32 sums
+ * (precision mode 2) */
+int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const double *w, size_t K, int splits,
+                       double *partial, size_t partial_cap, hipStream_t s);
+int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s);
 
 /* ---- epilogue (host) ---------------------------------------------------- */
 /* eigenpairs of the symmetric n x n column-major matrix a (upper triangle

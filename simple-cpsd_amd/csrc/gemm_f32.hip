@@ -316,6 +316,154 @@ int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ld
    return 0;
 }
 
+/* ------------------------------------------------ BF16-input projection */
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c)
+{
+   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+#define QK 32           /* K step: one 16x16x32 instruction per block */
+#define QP (QK + 8)     /* row pitch (bf16): a 16-byte read of 16 rows at one k hits distinct banks */
+
+/* partial[z][p][n] = sum_{k in split z} w_k Bb[p][k] C1[n][k] with BF16
+ * projectors Bb, C1 w rounded to BF16 as it is staged in LDS, FP32 sums;
+ * the layout, grid and split reduction of k_kbproj_f32.  Lane l supplies 8
+ * consecutive k of row l%16 to A and of column l%16 to B, the same k for
+ * both, so the instruction's k order does not matter. */
+template<int TM>
+__global__ void __launch_bounds__(512, 4)
+k_kbproj_bf16(const __bf16 *__restrict__ Bb, size_t ldb, int M, const double *__restrict__ C, size_t ldc,
+              int N, const double *__restrict__ w, size_t K, size_t kchunk, int ntm, int ntile, int per,
+              double *__restrict__ partial)
+{
+   constexpr int BI = TM / 32;
+   __shared__ __bf16 As[2][TM][QP];
+   __shared__ __bf16 Bs[2][PN][QP];
+   const int bid = blockIdx.x;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   const int t = (int)(g % ntile), kz = (int)(g / ntile);
+   const int tm = t % ntm, tn = t / ntm;
+   const size_t kb = (size_t)kz * kchunk;
+   const size_t ke = min(K, kb + kchunk);
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wm = wave & 1, wn = wave >> 1;
+   const int q = lane >> 4, rr = lane & 15;
+   /* loaders: A row ar (8 bf16 at aq, 4 threads per 32-k row); B rows
+    * lr + 32 h, k pair lq (16 threads per row) */
+   const int ar = tid >> 2, aq = (tid & 3) * 8;
+   const int lr = tid >> 4, lq = (tid & 15) * 2;
+
+   f4 acc[BI][2];
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   uint4 ra;
+   double2 rb[4], rw;
+   auto gload = [&](size_t k0) {
+      const int am = tm * TM + ar;
+      ra = make_uint4(0u, 0u, 0u, 0u);
+      if (ar < TM && am < M) ra = *reinterpret_cast<const uint4 *>(Bb + (size_t)am * ldb + k0 + aq);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+         const int bn = tn * PN + lr + 32 * h;
+         rb[h] = bn < N ? *reinterpret_cast<const double2 *>(C + (size_t)bn * ldc + k0 + lq)
+                        : make_double2(0.0, 0.0);
+      }
+      rw = *reinterpret_cast<const double2 *>(w + k0 + lq);
+   };
+   auto lstore = [&](int bf) {
+      if (ar < TM) *reinterpret_cast<uint4 *>(&As[bf][ar][aq]) = ra;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+         __bf16 *d = &Bs[bf][lr + 32 * h][lq];
+         d[0] = (__bf16)(float)(rb[h].x * rw.x);
+         d[1] = (__bf16)(float)(rb[h].y * rw.y);
+      }
+   };
+   auto mma = [&](int bf) {
+      bf8 a[BI], b[2];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) a[i] = *reinterpret_cast<const bf8 *>(&As[bf][(TM / 2) * wm + 16 * i + rr][8 * q]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf8 *>(&Bs[bf][32 * wn + 16 * j + rr][8 * q]);
+#pragma unroll
+      for (int i = 0; i < BI; ++i)
+#pragma unroll
+         for (int j = 0; j < 2; ++j) acc[i][j] = mfma_bf16(a[i], b[j], acc[i][j]);
+   };
+
+   if (kb < ke) {
+      gload(kb);
+      lstore(0);
+   }
+   __syncthreads();
+   if (kb + QK < ke) gload(kb + QK);
+   int buf = 0;
+   for (size_t k0 = kb; k0 < ke; k0 += QK) {
+      if (k0 + QK < ke) lstore(buf ^ 1);
+      if (k0 + 2 * QK < ke) gload(k0 + 2 * QK);
+      mma(buf);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+         const int n = tn * PN + 32 * wn + 16 * j + rr;
+#pragma unroll
+         for (int r = 0; r < 4; ++r) {
+            const int p = tm * TM + (TM / 2) * wm + 16 * i + 4 * q + r;
+            if (p < M && n < N) partial[((size_t)kz * M + p) * N + n] = (double)acc[i][j][r];
+         }
+      }
+}
+
+int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const double *w, size_t K, int splits,
+                       double *partial, size_t partial_cap, hipStream_t s)
+{
+   if (K % QK) { cpsd_set_error("kbproj_bf16: K=%zu not a multiple of %d", K, QK); return -1; }
+   if (splits < 1 || (size_t)splits * M * N > partial_cap) {
+      cpsd_set_error("kbproj_bf16: %d splits of %d x %d do not fit the partial buffer", splits, M, N);
+      return -1;
+   }
+   const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
+   const int tm = pad96 < pad128 ? 96 : 128;
+   const int ntm = (M + tm - 1) / tm, ntile = ntm * ((N + PN - 1) / PN);
+   size_t kchunk = (K + splits - 1) / splits;
+   kchunk = (kchunk + QK - 1) / QK * QK;
+   const long pairs = (long)ntile * splits;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   if (tm == 96)
+      hipLaunchKernelGGL((k_kbproj_bf16<96>), dim3((unsigned)pairs), dim3(512), 0, s, Bb, K, M, C, K, N, w, K,
+                         kchunk, ntm, ntile, per, partial);
+   else
+      hipLaunchKernelGGL((k_kbproj_bf16<128>), dim3((unsigned)pairs), dim3(512), 0, s, Bb, K, M, C, K, N, w, K,
+                         kchunk, ntm, ntile, per, partial);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16(const double *__restrict__ a, __bf16 *__restrict__ b, size_t n)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i < n) b[i] = (__bf16)(float)a[i];
+}
+
+int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
+{
+   if (n == 0) return 0;
+   hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, n);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
 /* ----------------------------------------------------- FP32 projectors */
 
 __global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n)

@@ -43,7 +43,7 @@ def test_mixed_mode_argument_checked(gpu):
     st = water_oracle_state()
     dev = cpsd_amd.DeviceSD(cell_from_state(st), 4)
     with pytest.raises(cpsd_amd.CpsdError):
-        dev.set_precision(2)
+        dev.set_precision(3)
     dev.close()
 
 
@@ -172,3 +172,33 @@ def test_mixed_shapes_vs_fp64(gpu, nmol, L, nfft, ne):
     assert 2 * o32["dn"].sum() * dv == pytest.approx(2 * ne, rel=1e-10)
     for d in devs:
         d.close()
+
+
+def test_bf16_projection_converges_to_fp64(gpu):
+    """mode 2 (BF16-input KB projection, 24 projectors so the MFMA kernel
+    runs): converged energy within 1e-6 Ha per atom of FP64's, and the BF16
+    kernel did run (mode 2's result differs from mode 1's)"""
+    say("mode 2 (BF16 projection) converged vs FP64, 4 waters 32^3 ne=16")
+    cell = cpsd_amd.water_cell(4, 14.0, 32, seed=0)
+    ne, natom = 16, 12
+    psi = cell.random_orbitals(ne, seed=1, decay=0.05)
+    out = {}
+    for mode in (0, 1, 2):
+        d = cpsd_amd.DeviceSD(cell, ne)
+        d.set_precision(mode)
+        d.set_orbitals(psi)
+        Ep = None
+        for _ in range(1000):
+            E, _, _ = d.run(10)
+            if Ep is not None and abs(E[0] - Ep) < 1e-7:
+                break
+            Ep = E[0]
+        out[mode] = E[0]
+        o = d.get(c2=True)["c2"]
+        S = gram(o, ne, 32)
+        assert np.max(np.abs(S - np.eye(ne))) < 1e-11
+        d.close()
+    say("E fp64 %.12f mode1 %.12f mode2 %.12f" % (out[0], out[1], out[2]))
+    assert abs(out[1] - out[0]) < 1e-10 * natom
+    assert abs(out[2] - out[0]) < 1e-6 * natom
+    assert out[2] != out[1]
