@@ -305,12 +305,81 @@ def run(args):
     return out
 
 
+def batch_cells(seed):
+    """64 seeded water cells of 1-8 molecules: cell edge 12-22 bohr growing with
+    the molecule count, grid 24^3-64^3 by edge, 4 orbitals per molecule"""
+    rng = np.random.default_rng(seed)
+    cells = []
+    for _ in range(64):
+        nmol = int(rng.integers(1, 9))
+        L = 12.0 + 10.0 * (nmol - 1) / 7.0 + float(rng.uniform(-0.4, 0.4))
+        n = 24 if L < 14 else 32 if L < 17 else 48 if L < 20 else 64
+        cells.append((nmol, L, n))
+    return cells
+
+
+def run_batch(args):
+    """--workload batch-64: the 64 cells advance 10-step calls one cell after
+    the other (each call waited for), then with every cell's call started
+    before any is finished (one stream per cell); cell-steps per second"""
+    import cpsd_amd
+    t0 = time.perf_counter()
+    specs = batch_cells(args.seed)
+    devs = []
+    for i, (nmol, L, n) in enumerate(specs):
+        cell = cpsd_amd.water_cell(nmol, L, n, seed=args.seed + i)
+        d = cpsd_amd.DeviceSD(cell, 4 * nmol)
+        d.set_graphs(args.graphs)
+        d.set_orbitals(cell.random_orbitals(4 * nmol, seed=args.seed + 100 + i, decay=0.05))
+        devs.append(d)
+    t_setup = time.perf_counter() - t0
+    inner = 10
+    calls = max(1, args.steps // inner)
+    for _ in range(max(1, args.warmup // inner)):
+        for d in devs:
+            d.run(inner)
+        cpsd_amd.run_batch(devs, inner)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        for d in devs:
+            d.run(inner)
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        cpsd_amd.run_batch(devs, inner)
+    bat = time.perf_counter() - t0
+    for d in devs:
+        d.close()
+    cell_steps = len(devs) * calls * inner
+    return {
+        "metric": "SD cell-steps/s, 64 independent water cells, 1 MI355X",
+        "value": round(cell_steps / bat, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": calls * inner,
+        "warmup": args.warmup,
+        "ms_per_step": round(bat * 1000.0 / (calls * inner), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded water cells and Gaussian orbitals)",
+        "config": {"workload": "batch-64", "cells": len(devs),
+                   "waters": [c[0] for c in specs], "fft_grids": sorted(set(c[2] for c in specs)),
+                   "orbitals": "4 per water", "inner": inner, "graphs": bool(args.graphs),
+                   "setup_s": round(t_setup, 2)},
+        "sequential": {"value": round(cell_steps / seq, 1), "unit": "cell-steps/s",
+                       "ms_per_step_all_cells": round(seq * 1000.0 / (calls * inner), 4)},
+        "speedup_vs_sequential": round(seq / bat, 4),
+    }
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=500)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--workload", default="h2o-32", choices=sorted(WORKLOADS))
+    p.add_argument("--workload", default="h2o-32", choices=sorted(WORKLOADS) + ["batch-64"])
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--converge-steps", type=int, default=10000)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
@@ -329,7 +398,7 @@ def main():
     saved = os.dup(1)
     os.dup2(2, 1)
     try:
-        out = run(args)
+        out = run_batch(args) if args.workload == "batch-64" else run(args)
     finally:
         sys.stdout.flush()
         import ctypes

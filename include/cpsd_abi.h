@@ -101,6 +101,14 @@ int cpsd_dev_get_ions(cpsd_dev *h, double *r1, double *r2, double *fion);
  * (as in the reference), E[0..9] on exit; deltas as in cpsd().         */
 int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, double *E, double *deltae,
                  double *deltac, double *deltar);
+/* cpsd_dev_run() in two halves: _start enqueues the call (inner steps and
+ * the energy evaluation, E[0] read as the previous energy) on the context's
+ * stream and returns at once; _finish waits for it and returns what
+ * cpsd_dev_run() returns.  Starting the calls of several contexts before
+ * finishing any advances independent cells concurrently.  One started call
+ * per context at a time.  No counterpart in the reference. */
+int cpsd_dev_run_start(cpsd_dev *h, int inner, double dt, double fmass, const double *E);
+int cpsd_dev_run_finish(cpsd_dev *h, double *E, double *deltae, double *deltac, double *deltar);
 /* any pointer may be NULL; layouts as in cpsd() */
 int cpsd_dev_get(cpsd_dev *h, double *c1, double *c2, double *cpsi, double *dn, double *hml,
                  double *lmd);

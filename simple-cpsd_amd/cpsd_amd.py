@@ -36,7 +36,8 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_lagrange_info", "cpsd_dev_sizes", "cpsd_dev_profile",
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
-               "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision", "cpsd_dev_set_graphs")
+               "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision", "cpsd_dev_set_graphs",
+               "cpsd_dev_run_start", "cpsd_dev_run_finish")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
                 "cpsdh_write_ionout")
@@ -73,6 +74,8 @@ def hip_lib():
         L.cpsd_dev_lagrange_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DPTR]
         L.cpsd_dev_set_precision.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.cpsd_dev_run_start.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
+        L.cpsd_dev_run_finish.argtypes = [C.c_void_p, _dp, _DPTR, _DPTR, _DPTR]
         L.cpsd_dev_sizes.argtypes = [C.c_void_p, _ip]
         L.cpsd_dev_profile.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.c_int, _dp, _ip, C.c_int]
@@ -133,6 +136,15 @@ def _err():
 def _check(rc, what):
     if rc != 0:
         raise CpsdError("%s failed: %s" % (what, _err()))
+
+
+def run_batch(devs, inner, dt=5.8, fmass=400000.0):
+    """one call of each of several independent cells: every call is started
+    (each context on its own stream) before any is finished, so the cells
+    advance concurrently; returns run()'s result per cell"""
+    for d in devs:
+        d.run_start(inner, dt, fmass)
+    return [d.run_finish() for d in devs]
 
 
 # ------------------------------------------------------------- restart files
@@ -342,6 +354,17 @@ class DeviceSD:
     def set_orbitals(self, psi):
         _check(hip_lib().cpsd_dev_set_orbitals(self.h, np.ascontiguousarray(psi, dtype=np.float64)),
                "cpsd_dev_set_orbitals")
+
+    def run_start(self, inner, dt=5.8, fmass=400000.0):
+        """enqueue a call and return at once (see run_batch)"""
+        _check(hip_lib().cpsd_dev_run_start(self.h, inner, dt, fmass, self.E), "cpsd_dev_run_start")
+
+    def run_finish(self):
+        de, dc, dr = C.c_double(), C.c_double(), C.c_double()
+        _check(hip_lib().cpsd_dev_run_finish(self.h, self.E, C.byref(de), C.byref(dc), C.byref(dr)),
+               "cpsd_dev_run_finish")
+        self.deltar = dr.value
+        return self.E.copy(), de.value, dc.value
 
     def set_ions(self, move, r2=None, amass=None):
         """Positions (default: the cell's) and, with move=1, ion motion with

@@ -116,6 +116,11 @@ struct cpsd_dev {
     * the captured launches bake in; see ctx_run */
    int graphs;
    int ran_mode[3];       /* an eager call has run in this precision mode */
+   /* a started call's results (E[0..11], Lagrange info, error flag) land in
+    * pinned host memory; pending until cpsd_dev_run_finish */
+   double *hout;
+   int *herr;
+   int pending;
    struct GraphEntry {
       int inner, mixed, cur0, g11i0, g11v0;
       double dt, dte;
@@ -203,6 +208,8 @@ static void ctx_free(cpsd_dev *c)
    free(c->amass_h);
    free(c->Gp_h);
    for (int i = 0; i < c->ngc; ++i) (void)hipGraphExecDestroy(c->gcache[i].exec);
+   if (c->hout) (void)hipHostFree(c->hout);
+   if (c->herr) (void)hipHostFree(c->herr);
    if (c->pool) {
       for (int i = 0; i < 2 * c->pool_cap; ++i) (void)hipEventDestroy(c->pool[i]);
       free(c->pool);
@@ -321,6 +328,8 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    }
 
    CPSD_CHECK(hipStreamCreate(&c->stream));
+   CPSD_CHECK(hipHostMalloc((void **)&c->hout, 16 * sizeof(double), hipHostMallocDefault));
+   CPSD_CHECK(hipHostMalloc((void **)&c->herr, sizeof(int), hipHostMallocDefault));
    hipStream_t st = c->stream;
    const size_t nn2 = (size_t)n * n;
    TRY(dalloc(&c->rstart, nn2)); TRY(dalloc(&c->rlen, nn2)); TRY(dalloc(&c->roff, nn2));
@@ -776,10 +785,15 @@ static int energy_tail(cpsd_dev *c, int inner, double dt, double dte)
    return 0;
 }
 
-static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, double *deltae,
-                   double *deltac, double *deltar)
+/* a call in two halves: ctx_run_start enqueues the inner steps, the energy
+ * evaluation and the copy of the results to pinned host memory on the
+ * context's stream and returns; ctx_run_finish waits for them.  Contexts
+ * whose calls are all started before any is finished advance concurrently,
+ * each on its own stream. */
+static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const double *E)
 {
    if (inner < 1) { cpsd_set_error("inner=%d must be >= 1", inner); return -1; }
+   if (c->pending) { cpsd_set_error("a started call has not been finished"); return -1; }
    const double dte = dt / sqrt(fmass);
    hipStream_t st = c->stream;
    TRY(h2d(c->E, E, 1, st));
@@ -843,12 +857,20 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
       c->ran_mode[c->mixed] = 1;
    }
    if (c->move) c->struct_valid = 0;
-   double out[12], info[2];
-   int err = 0;
-   CPSD_CHECK(hipMemcpyAsync(out, c->E, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
-   CPSD_CHECK(hipMemcpyAsync(info, c->info, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-   CPSD_CHECK(hipMemcpyAsync(&err, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
-   CPSD_CHECK(hipStreamSynchronize(st));
+   CPSD_CHECK(hipMemcpyAsync(c->hout, c->E, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipMemcpyAsync(c->hout + 12, c->info, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+   CPSD_CHECK(hipMemcpyAsync(c->herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
+   c->pending = 1;
+   return 0;
+}
+
+static int ctx_run_finish(cpsd_dev *c, double *E, double *deltae, double *deltac, double *deltar)
+{
+   if (!c->pending) { cpsd_set_error("no started call to finish"); return -1; }
+   c->pending = 0;
+   CPSD_CHECK(hipStreamSynchronize(c->stream));
+   const double *out = c->hout, *info = c->hout + 12;
+   const int err = *c->herr;
    if (err) { cpsd_set_error("Lagrange grid barrier timed out"); return -1; }
    c->last_lag_iters = (int)info[0];
    c->last_lag_adiff = info[1];
@@ -868,6 +890,13 @@ static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, d
    }
    c->have_run = 1;
    return 0;
+}
+
+static int ctx_run(cpsd_dev *c, int inner, double dt, double fmass, double *E, double *deltae,
+                   double *deltac, double *deltar)
+{
+   TRY(ctx_run_start(c, inner, dt, fmass, E));
+   return ctx_run_finish(c, E, deltae, deltac, deltar);
 }
 
 static int ctx_set_ions(cpsd_dev *c, int move, const double *amass, const double *r2)
@@ -1047,6 +1076,21 @@ extern "C" int cpsd_dev_run(cpsd_dev *h, int inner, double dt, double fmass, dou
       return -1;
    }
    return ctx_run(h, inner, dt, fmass, E, deltae, deltac, deltar);
+}
+
+extern "C" int cpsd_dev_run_start(cpsd_dev *h, int inner, double dt, double fmass, const double *E)
+{
+   if (!h || !E) { cpsd_set_error("cpsd_dev_run_start: null handle or E"); return -1; }
+   return ctx_run_start(h, inner, dt, fmass, E);
+}
+
+extern "C" int cpsd_dev_run_finish(cpsd_dev *h, double *E, double *deltae, double *deltac, double *deltar)
+{
+   if (!h || !E || !deltae || !deltac || !deltar) {
+      cpsd_set_error("cpsd_dev_run_finish: null handle or output");
+      return -1;
+   }
+   return ctx_run_finish(h, E, deltae, deltac, deltar);
 }
 
 extern "C" int cpsd_dev_set_ions(cpsd_dev *h, int move, const double *amass, const double *r2)

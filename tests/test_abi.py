@@ -61,6 +61,8 @@ def test_resident_api_rejects_a_null_handle():
         lambda: L.cpsd_dev_profile(None, 1),
         lambda: L.cpsd_dev_profile_read(None, C.create_string_buffer(64), 64, d, i, 4),
         lambda: L.cpsd_dev_debug_fetch(None, 0, d, 16),
+        lambda: L.cpsd_dev_run_start(None, 1, 5.8, 4e5, d),
+        lambda: L.cpsd_dev_run_finish(None, d, C.byref(z), C.byref(z), C.byref(z)),
     ]
     for f in calls:
         assert f() == -1
