@@ -453,6 +453,26 @@ __device__ __forceinline__ void lag_barrier(unsigned *bar, unsigned nbar, int nw
    else grid_barrier(bar, nbar * nwg, err);
 }
 
+/* the largest per-workgroup maximum: thread t reads entries t, t + 256, ...
+ * (loads in flight together rather than one dependent load per workgroup),
+ * then a tree over the 256 threads; every workgroup reduces the same values
+ * and max does not depend on order, so all of them take the same decision */
+__device__ __forceinline__ double grid_max(const double *wgmax, int nwg, double *red)
+{
+   double v = 0.0;
+   for (int q = threadIdx.x; q < nwg; q += 256)
+      v = fmax(v, __hip_atomic_load(wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+   red[threadIdx.x] = v;
+   __syncthreads();
+   for (int s2 = 128; s2 > 0; s2 >>= 1) {
+      if ((int)threadIdx.x < s2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s2]);
+      __syncthreads();
+   }
+   const double g = red[0];
+   __syncthreads();
+   return g;
+}
+
 /* C_tile += A * B over k (column-major ne x ne), TS x TS tile at (ti, tj);
  * one or more outputs per thread accumulated in acc[]. */
 template<int TS, int PER>
@@ -543,8 +563,7 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
       }
       if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
       lag_barrier(a.bar, ++nbar, nwg, a.err);
-      gmax = 0.0;
-      for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      gmax = grid_max(a.wgmax, nwg, red);
       if (gmax < 1e-15) break;
       if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
    }
@@ -582,18 +601,37 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
    const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
    const int nwg = gridDim.x;
    const size_t nn = (size_t)ne * ne;
-   /* R[r][k] = M(ti*TS + r, k) */
+   /* panel loads: all LQ loads of a thread are issued before any LDS store */
+   constexpr int LQ = TS * NM / 256;
+   static_assert(TS * NM % 256 == 0, "panel loads cover TS x NM in whole rounds");
+   /* R[r][k] = M(ti*TS + r, k), k < ne */
    auto load_rows = [&](const double *M, double (*R)[P]) {
-      for (int e = threadIdx.x; e < TS * ne; e += 256) {
+      double v[LQ];
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
          const int r = e % TS, k = e / TS, row = ti * TS + r;
-         R[r][k] = row < ne ? M[row + (size_t)k * ne] : 0.0;
+         v[u] = (row < ne && k < ne) ? M[row + (size_t)k * ne] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
+         if (e / TS < ne) R[e % TS][e / TS] = v[u];
       }
    };
-   /* Cp[c][k] = M(k, tj*TS + c) */
+   /* Cp[c][k] = M(k, tj*TS + c), k < ne */
    auto load_cols = [&](const double *M, double (*Cp)[P]) {
-      for (int e = threadIdx.x; e < TS * ne; e += 256) {
-         const int k = e % ne, c = e / ne, col = tj * TS + c;
-         Cp[c][k] = col < ne ? M[k + (size_t)col * ne] : 0.0;
+      double v[LQ];
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
+         const int k = e % NM, c = e / NM, col = tj * TS + c;
+         v[u] = (col < ne && k < ne) ? M[k + (size_t)col * ne] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
+         if (e % NM < ne) Cp[e / NM][e % NM] = v[u];
       }
    };
    constexpr int PER = TS * TS / 256;
@@ -652,8 +690,7 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
       }
       if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
       lag_barrier(a.bar, ++nbar, nwg, a.err);
-      gmax = 0.0;
-      for (int q = 0; q < nwg; ++q) gmax = fmax(gmax, __hip_atomic_load(a.wgmax + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      gmax = grid_max(a.wgmax, nwg, red);
       if (gmax < 1e-15) break;
       if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
    }
