@@ -6,6 +6,7 @@
  */
 #include "cpsd_internal.h"
 #include "sd_kernels.h"
+#include "mfma64.h"
 
 /* ------------------------------------------------- packing (boundary) */
 
@@ -579,16 +580,20 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
    }
 }
 
-/* ne <= NM: each workgroup keeps its constant panels (rows ti of s11 and
- * s21, columns tj of s12) in LDS for all iterations; an iteration loads the
- * X panels and, after the first barrier, the st1 panel, i.e. two round trips
- * to L2 and two grid barriers per iteration (the general kernel above makes
- * one round trip per 32-wide K chunk of each of its four products).
- * Matrices are column-major: M(r, c) = M[r + c*ne]. */
-template<int TS, int NM>
+/* ne <= NM: each workgroup owns one 16 x 16 tile of X and keeps its
+ * constant panels (rows ti of s11 and s21, columns tj of s12) in LDS for all
+ * iterations; an iteration loads the X panels and, after the first barrier,
+ * the st1 panel, i.e. two round trips to L2 and two grid barriers per
+ * iteration (the general kernel above makes one round trip per 32-wide K
+ * chunk of each of its four products).  The tile products run on the FP64
+ * 4x4x4 MFMA (mfma64.h): the four waves take a quarter of K each and their
+ * 16 x 16 partial tiles are added in wave order, so results do not depend on
+ * timing.  Matrices are column-major: M(r, c) = M[r + c*ne]. */
+template<int NM>
 __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
 {
-   constexpr int P = NM + 1;
+   constexpr int TS = 16;
+   constexpr int P = NM + 2;   /* row pitch: a 64-bit read of 16 rows x 2 k hits distinct banks */
    extern __shared__ double lpan[];
    double (*s11r)[P] = reinterpret_cast<double (*)[P]>(lpan);
    double (*s21r)[P] = s11r + TS;
@@ -596,11 +601,22 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
    double (*xc)[P] = s12c + TS;
    double (*xr)[P] = xc + TS;
    double (*stc)[P] = xr + TS;
+   __shared__ double part[4][TS][TS + 1];
    __shared__ double red[256];
    const int ne = a.ne, nt = (ne + TS - 1) / TS;
    const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
    const int nwg = gridDim.x;
    const size_t nn = (size_t)ne * ne;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   /* K padded to whole 4-wide steps for each of the four waves */
+   const int kp = (ne + 15) & ~15, kq = kp / 4, kb = wave * kq, ke = kb + kq;
+   const int s1 = rot_step(lane);
+   const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
+   /* this thread's element of the tile */
+   const int rl = tid % TS, cl = tid / TS, r = ti * TS + rl, c = tj * TS + cl;
+   const bool mine = r < ne && c < ne;
+   const size_t e = r + (size_t)c * ne;
+
    /* panel loads: all LQ loads of a thread are issued before any LDS store */
    constexpr int LQ = TS * NM / 256;
    static_assert(TS * NM % 256 == 0, "panel loads cover TS x NM in whole rounds");
@@ -609,14 +625,14 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
       double v[LQ];
 #pragma unroll
       for (int u = 0; u < LQ; ++u) {
-         const int e = threadIdx.x + 256 * u;
-         const int r = e % TS, k = e / TS, row = ti * TS + r;
+         const int x = tid + 256 * u;
+         const int rr = x % TS, k = x / TS, row = ti * TS + rr;
          v[u] = (row < ne && k < ne) ? M[row + (size_t)k * ne] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < LQ; ++u) {
-         const int e = threadIdx.x + 256 * u;
-         if (e / TS < ne) R[e % TS][e / TS] = v[u];
+         const int x = tid + 256 * u;
+         if (x / TS < ne) R[x % TS][x / TS] = v[u];
       }
    };
    /* Cp[c][k] = M(k, tj*TS + c), k < ne */
@@ -624,35 +640,52 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
       double v[LQ];
 #pragma unroll
       for (int u = 0; u < LQ; ++u) {
-         const int e = threadIdx.x + 256 * u;
-         const int k = e % NM, c = e / NM, col = tj * TS + c;
+         const int x = tid + 256 * u;
+         const int k = x % NM, cc = x / NM, col = tj * TS + cc;
          v[u] = (col < ne && k < ne) ? M[k + (size_t)col * ne] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < LQ; ++u) {
-         const int e = threadIdx.x + 256 * u;
-         if (e % NM < ne) Cp[e / NM][e % NM] = v[u];
+         const int x = tid + 256 * u;
+         if (x % NM < ne) Cp[x / NM][x % NM] = v[u];
       }
    };
-   constexpr int PER = TS * TS / 256;
-   auto dot = [&](double (*R)[P], double (*Cp)[P], int r, int c) {
-      double s0 = 0.0, s1 = 0.0;
-      int k = 0;
-      for (; k + 1 < ne; k += 2) { s0 += R[r][k] * Cp[c][k]; s1 += R[r][k + 1] * Cp[c][k + 1]; }
-      if (k < ne) s0 += R[r][k] * Cp[c][k];
-      return s0 + s1;
+   /* acc += Pr(16 x K) Qc^T over this wave's K quarter; Pr[i][k] = P(i, k),
+    * Qc[j][k] = Q(k, j) */
+   auto tile = [&](double (*Pr)[P], double (*Qc)[P], double (&acc)[4]) {
+      for (int m0 = kb; m0 < ke; m0 += 4) {
+         const int m = m0 + q;
+         const double pa = Pr[lane & 15][m];
+         double bq[4];
+         rotations(Qc[lane & 15][m], bq);
+#pragma unroll
+         for (int t = 0; t < 4; ++t) acc[t] = mfma4(pa, bq[t], acc[t]);
+      }
    };
+   /* the sum of the four waves' tiles at this thread's element */
+   auto reduce = [&](const double (&acc)[4]) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) part[wave][4 * ab + q][4 * ((ab + s1 * t) & 3) + bb] = acc[t];
+      __syncthreads();
+      const double v = ((part[0][rl][cl] + part[1][rl][cl]) + part[2][rl][cl]) + part[3][rl][cl];
+      __syncthreads();
+      return v;
+   };
+
+   /* K padding read by the MFMA steps stays zero: the loads write k < ne only */
+   for (int x = tid; x < 6 * TS * (kp - ne); x += 256) {
+      const int pnl = x / (TS * (kp - ne)), y = x % (TS * (kp - ne));
+      (s11r + pnl * TS)[y / (kp - ne)][ne + y % (kp - ne)] = 0.0;
+   }
    unsigned nbar = 0;
    int it = 0;
    double gmax = 0.0;
    load_rows(a.s11, s11r);
    load_rows(a.s21, s21r);
    load_cols(a.s12, s12c);
+   const double s22v = mine ? a.s22[e] : 0.0;
    /* X_0 = s22 */
-   for (int q = 0; q < PER; ++q) {
-      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
-      if (r < ne && c < ne) a.X[r + (size_t)c * ne] = a.s22[r + (size_t)c * ne];
-   }
+   if (mine) a.X[e] = s22v;
    lag_barrier(a.bar, ++nbar, nwg, a.err);
    while (it < 30) {
       const double *X = a.X + (size_t)(it & 1) * nn;
@@ -662,44 +695,39 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
       load_rows(X, xr);
       __syncthreads();
       /* st1 = s11 X */
-      for (int q = 0; q < PER; ++q) {
-         const int o = threadIdx.x + q * 256, rl = o % TS, cl = o / TS;
-         const int r = ti * TS + rl, c = tj * TS + cl;
-         if (r < ne && c < ne) a.st1[r + (size_t)c * ne] = dot(s11r, xc, rl, cl);
-      }
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      tile(s11r, xc, acc);
+      const double st = reduce(acc);
+      if (mine) a.st1[e] = st;
       lag_barrier(a.bar, ++nbar, nwg, a.err);
       load_cols(a.st1, stc);
       __syncthreads();
       /* X_new = s22 + s21 X + X s12 + X st1 */
+      acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+      tile(s21r, xc, acc);
+      tile(xr, s12c, acc);
+      tile(xr, stc, acc);
+      const double v = s22v + reduce(acc);
       double lmax = 0.0;
-      for (int q = 0; q < PER; ++q) {
-         const int o = threadIdx.x + q * 256, rl = o % TS, cl = o / TS;
-         const int r = ti * TS + rl, c = tj * TS + cl;
-         if (r < ne && c < ne) {
-            const size_t e = r + (size_t)c * ne;
-            const double v = a.s22[e] + (dot(s21r, xc, rl, cl) + dot(xr, s12c, rl, cl) + dot(xr, stc, rl, cl));
-            Xn[e] = v;
-            lmax = fmax(lmax, fabs(v - X[e]));
-         }
+      if (mine) {
+         Xn[e] = v;
+         lmax = fabs(v - xc[cl][r]);   /* X(r, c) */
       }
-      red[threadIdx.x] = lmax;
+      red[tid] = lmax;
       __syncthreads();
       for (int s2 = 128; s2 > 0; s2 >>= 1) {
-         if ((int)threadIdx.x < s2) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s2]);
+         if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
          __syncthreads();
       }
-      if (threadIdx.x == 0) a.wgmax[blockIdx.x] = red[0];
+      if (tid == 0) a.wgmax[blockIdx.x] = red[0];
       lag_barrier(a.bar, ++nbar, nwg, a.err);
       gmax = grid_max(a.wgmax, nwg, red);
       if (gmax < 1e-15) break;
       if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
    }
    const double *Xf = a.X + (size_t)(it & 1) * nn;
-   for (int q = 0; q < PER; ++q) {
-      const int o = threadIdx.x + q * 256, r = ti * TS + o % TS, c = tj * TS + o / TS;
-      if (r < ne && c < ne) a.lmd[r + (size_t)c * ne] = Xf[r + (size_t)c * ne];
-   }
-   if (blockIdx.x == 0 && threadIdx.x == 0) {
+   if (mine) a.lmd[e] = Xf[e];
+   if (blockIdx.x == 0 && tid == 0) {
       a.info[0] = (double)it;
       a.info[1] = gmax;
    }
@@ -931,14 +959,14 @@ int sd_lagrange(const LagArgs &a, hipStream_t s)
    CPSD_CHECK(hipGetLastError());
    if (a.ne <= 128) {
       const int nt = (a.ne + 15) / 16;
-      const int lds = 6 * 16 * (128 + 1) * (int)sizeof(double);
+      const int lds = 6 * 16 * (128 + 2) * (int)sizeof(double);
       static bool attr = false;
       if (!attr) {
-         CPSD_CHECK(hipFuncSetAttribute((const void *)k_lagrange_res<16, 128>,
+         CPSD_CHECK(hipFuncSetAttribute((const void *)k_lagrange_res<128>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds));
          attr = true;
       }
-      hipLaunchKernelGGL((k_lagrange_res<16, 128>), dim3(nt * nt), dim3(256), lds, s, a);
+      hipLaunchKernelGGL((k_lagrange_res<128>), dim3(nt * nt), dim3(256), lds, s, a);
    } else if (a.ne <= 256) {
       const int nt = (a.ne + 15) / 16;
       hipLaunchKernelGGL(k_lagrange<16>, dim3(nt * nt), dim3(256), 0, s, a);
