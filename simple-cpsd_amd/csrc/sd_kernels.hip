@@ -481,16 +481,25 @@ __device__ __forceinline__ void tile_mm(const double *A, const double *B, int ne
                                         double (&acc)[PER], double (*sa)[33], double (*sb)[33])
 {
    /* chunks of 32 along k: sa[r][kk] = A[ti*TS + r, k0+kk], sb[c][kk] = B[k0+kk, tj*TS + c] */
+   /* 256 threads; each chunk's loads are issued together before the LDS stores */
+   constexpr int CQ = TS * 32 / 256;
    for (int k0 = 0; k0 < ne; k0 += 32) {
-      for (int e = threadIdx.x; e < TS * 32; e += blockDim.x) {
+      double va[CQ], vb[CQ];
+#pragma unroll
+      for (int u = 0; u < CQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
          const int r = e % TS, kk = e / TS;
          const int row = ti * TS + r, k = k0 + kk;
-         sa[r][kk] = (row < ne && k < ne) ? A[row + (size_t)k * ne] : 0.0;
+         va[u] = (row < ne && k < ne) ? A[row + (size_t)k * ne] : 0.0;
+         const int kb = e % 32, c = e / 32;
+         const int col = tj * TS + c, k2 = k0 + kb;
+         vb[u] = (col < ne && k2 < ne) ? B[k2 + (size_t)col * ne] : 0.0;
       }
-      for (int e = threadIdx.x; e < TS * 32; e += blockDim.x) {
-         const int kk = e % 32, c = e / 32;
-         const int col = tj * TS + c, k = k0 + kk;
-         sb[c][kk] = (col < ne && k < ne) ? B[k + (size_t)col * ne] : 0.0;
+#pragma unroll
+      for (int u = 0; u < CQ; ++u) {
+         const int e = threadIdx.x + 256 * u;
+         sa[e % TS][e / TS] = va[u];
+         sb[e / 32][e % 32] = vb[u];
       }
       __syncthreads();
 #pragma unroll
