@@ -17,6 +17,7 @@
  */
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -31,6 +32,21 @@
    } while (0)
 
 void cpsd_set_error(const char *fmt, ...);
+
+/* hipFuncAttributeMaxDynamicSharedMemorySize = bytes for kernel k on the
+ * current device, set on the first launch per device; done is the launch
+ * site's own record (one bit per device ordinal, devices from 64 up are set
+ * on every launch) */
+static inline int dyn_lds_once(const void *k, int bytes, std::atomic<unsigned long long> &done)
+{
+   int dev = 0;
+   CPSD_CHECK(hipGetDevice(&dev));
+   const unsigned long long bit = dev < 64 ? 1ull << dev : 0ull;
+   if (bit && (done.load(std::memory_order_acquire) & bit)) return 0;
+   CPSD_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+   done.fetch_or(bit, std::memory_order_release);
+   return 0;
+}
 
 /* ---- FFT ------------------------------------------------------------ */
 

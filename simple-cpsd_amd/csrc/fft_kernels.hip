@@ -534,12 +534,8 @@ template<int N, int MODE, typename V>
 static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
 {
    const size_t lds = plane_lds_bytes<N, V>();
-   static bool attr_set = false;
-   if (!attr_set) {
-      CPSD_CHECK(hipFuncSetAttribute((const void *)k_plane<N, MODE, V>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      attr_set = true;
-   }
+   static std::atomic<unsigned long long> attr_set{0};
+   if (dyn_lds_once((const void *)k_plane<N, MODE, V>, (int)lds, attr_set) != 0) return -1;
    hipLaunchKernelGGL((k_plane<N, MODE, V>), dim3(N, nblocks), dim3(GridCfg<N>::PLANE_THREADS),
                       lds, s, a);
    CPSD_CHECK(hipGetLastError());

@@ -969,12 +969,8 @@ int sd_lagrange(const LagArgs &a, hipStream_t s)
    if (a.ne <= 128) {
       const int nt = (a.ne + 15) / 16;
       const int lds = 6 * 16 * (128 + 2) * (int)sizeof(double);
-      static bool attr = false;
-      if (!attr) {
-         CPSD_CHECK(hipFuncSetAttribute((const void *)k_lagrange_res<128>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-         attr = true;
-      }
+      static std::atomic<unsigned long long> attr{0};
+      if (dyn_lds_once((const void *)k_lagrange_res<128>, lds, attr) != 0) return -1;
       hipLaunchKernelGGL((k_lagrange_res<128>), dim3(nt * nt), dim3(256), lds, s, a);
    } else if (a.ne <= 256) {
       const int nt = (a.ne + 15) / 16;
