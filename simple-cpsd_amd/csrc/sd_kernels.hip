@@ -429,19 +429,23 @@ __global__ void k_kbcoef(const double *__restrict__ P, int nproj, int ne, const 
 
 /* ---------------------------------------- Lagrange multiplier fixed point */
 
+/* thread 0 publishes the workgroup's writes (agent-scope release), arrives,
+ * and polls with relaxed agent-scope loads, which read past the XCD's L2
+ * without invalidating it; one acquire fence after the poll makes the other
+ * workgroups' writes visible to the whole workgroup */
 __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *err)
 {
    __syncthreads();
    if (threadIdx.x == 0) {
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       atomicAdd(bar, 1u);
       unsigned spins = 0;
-      while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
          __builtin_amdgcn_s_sleep(4);
          if (++spins > (1u << 21)) { atomicExch(err, 1); break; }
       }
-      __threadfence();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
    }
    __syncthreads();
 }
