@@ -500,18 +500,22 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
    const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm)) &&
                       !(m >= skip0 && m < skip1 && n >= skip0 && n < skip1);
    const int nz = (symm && m / TT == n / TT) ? sp_dd : splits;
-   double s0 = 0.0, s1 = 0.0;
+   /* wave g sums splits g, g + 4, ... into eight accumulators, so that eight
+    * loads are in flight per lane, then adds them in a fixed tree */
+   double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
    if (valid) {
       const size_t stride = (size_t)M * N;
       const double *p = partial + (size_t)m * N + n;
       int z = g;
-      for (; z + 4 < nz; z += 8) {
-         s0 += p[(size_t)z * stride];
-         s1 += p[(size_t)(z + 4) * stride];
+      for (; z + 28 < nz; z += 32) {
+#pragma unroll
+         for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(z + 4 * u) * stride];
       }
-      if (z < nz) s0 += p[(size_t)z * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+         if (z + 4 * u < nz) acc[u] += p[(size_t)(z + 4 * u) * stride];
    }
-   red[g][l] = s0 + s1;
+   red[g][l] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
    __syncthreads();
    if (g == 0 && valid) {
       const double v = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);

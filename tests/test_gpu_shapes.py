@@ -164,9 +164,10 @@ def test_full_h2o32_properties(gpu):
 
 
 @pytest.mark.parametrize("nfft,L,nmol,ne", [(16, 10.0, 1, 1), (16, 10.0, 1, 5), (24, 12.0, 2, 17),
-                                            (24, 12.0, 2, 33)])
+                                            (24, 12.0, 2, 33), (24, 12.0, 2, 100)])
 def test_odd_orbital_counts_vs_oracle(gpu, nfft, L, nmol, ne):
     """one orbital, odd counts and counts just past a 16-row block or a
-    32-wide split step, against the oracle"""
+    32-wide split step, and 100 orbitals (a 7 x 7 grid of Lagrange tiles whose
+    MFMA K steps run over zero padding from 100 to 112), against the oracle"""
     say("ne=%d at %d^3" % (ne, nfft))
     run_pair(nmol, L, nfft, ne, steps=2, inner=3)
