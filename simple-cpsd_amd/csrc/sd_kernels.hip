@@ -346,6 +346,100 @@ __global__ void k_smat(const double *__restrict__ Gm, const double *__restrict__
    }
 }
 
+/* ------------------------------------------ 16 x 16 tiles on the MFMA */
+
+/* acc += P Q over k in [kb, kb + 4n) for a 16 x 16 tile on the FP64 4x4x4
+ * MFMA (mfma64.h); Pr[i][k] = P(i, k) and Qc[j][k] = Q(k, j) in LDS.  The
+ * result lands in the rotated layout that tile16_reduce undoes. */
+template<int LP>
+__device__ __forceinline__ void tile16_mfma(double (*Pr)[LP], double (*Qc)[LP], int kb, int ke, int lane,
+                                            double (&acc)[4])
+{
+   for (int m0 = kb; m0 < ke; m0 += 4) {
+      const int m = m0 + (lane >> 4);
+      const double pa = Pr[lane & 15][m];
+      double bq[4];
+      rotations(Qc[lane & 15][m], bq);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma4(pa, bq[t], acc[t]);
+   }
+}
+
+/* the four waves' partial tiles (each over a quarter of K) added in wave
+ * order at element (rl, cl) of the tile; all 256 threads call it */
+__device__ __forceinline__ double tile16_reduce(double (*part)[16][17], const double (&acc)[4], int wave, int lane,
+                                                int s1, int rl, int cl)
+{
+   const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
+#pragma unroll
+   for (int t = 0; t < 4; ++t) part[wave][4 * ab + q][4 * ((ab + s1 * t) & 3) + bb] = acc[t];
+   __syncthreads();
+   const double v = ((part[0][rl][cl] + part[1][rl][cl]) + part[2][rl][cl]) + part[3][rl][cl];
+   __syncthreads();
+   return v;
+}
+
+/* W = G11 lmd and T1 = G21 lmd for ne <= NM, one 16 x 16 tile of each per
+ * workgroup (tile (ti, tj): rows i, columns b), the sums over j on the MFMA
+ * with the four waves taking a quarter each.  Operands as in k_gram_next_w
+ * below: G11(i, j) = g11c[j ne + i] (symmetric) or the Gram's C1 x C1 block,
+ * G21(i, j) = <c1_j|c2_i> from the Gram's C1 x C2 block, lmd(j, b) =
+ * lmd[j + b ne]. */
+template<int NM>
+__global__ void __launch_bounds__(256) k_gram_next_w_mf(const double *__restrict__ Gm,
+                                                        const double *__restrict__ g11c, int ne, int c1_second,
+                                                        const double *__restrict__ lmd, double *__restrict__ W,
+                                                        double *__restrict__ T1)
+{
+   constexpr int TS = 16, P = NM + 2;
+   __shared__ double g11r[TS][P], g21r[TS][P], lmc[TS][P];
+   __shared__ double part[4][TS][TS + 1];
+   const int nt = (ne + TS - 1) / TS;
+   const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int kp = (ne + 15) & ~15, kq = kp / 4, kb = wave * kq, ke = kb + kq;
+   const int s1 = rot_step(lane);
+   const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
+   const double *gb = g11c ? g11c : Gm + (size_t)o1 * n2 + o1;
+   const size_t gld = g11c ? (size_t)ne : (size_t)n2;
+   const double *tb = Gm + (size_t)o1 * n2 + o2;
+   constexpr int LQ = TS * NM / 256;
+   static_assert(TS * NM % 256 == 0, "panel loads cover TS x NM in whole rounds");
+   /* row panels: R[r][k] = base[k ld + ti*TS + r]; column panel: lmc[c][k] =
+    * lmd(k, tj*TS + c); zero outside ne, up to the padded K */
+   double vg[LQ], vt[LQ], vl[LQ];
+#pragma unroll
+   for (int u = 0; u < LQ; ++u) {
+      const int x = tid + 256 * u;
+      const int r = x % TS, k = x / TS, row = ti * TS + r;
+      const bool in = row < ne && k < ne;
+      vg[u] = in ? gb[(size_t)k * gld + row] : 0.0;
+      vt[u] = in ? tb[(size_t)k * n2 + row] : 0.0;
+      const int kk = x % NM, c = x / NM, col = tj * TS + c;
+      vl[u] = (col < ne && kk < ne) ? lmd[kk + (size_t)col * ne] : 0.0;
+   }
+#pragma unroll
+   for (int u = 0; u < LQ; ++u) {
+      const int x = tid + 256 * u;
+      if (x / TS < kp) {
+         g11r[x % TS][x / TS] = vg[u];
+         g21r[x % TS][x / TS] = vt[u];
+      }
+      if (x % NM < kp) lmc[x / NM][x % NM] = vl[u];
+   }
+   __syncthreads();
+   double aw[4] = {0.0, 0.0, 0.0, 0.0}, at[4] = {0.0, 0.0, 0.0, 0.0};
+   tile16_mfma<P>(g11r, lmc, kb, ke, lane, aw);
+   tile16_mfma<P>(g21r, lmc, kb, ke, lane, at);
+   const int rl = tid % TS, cl = tid / TS, i = ti * TS + rl, b = tj * TS + cl;
+   const double w = tile16_reduce(part, aw, wave, lane, s1, rl, cl);
+   const double t = tile16_reduce(part, at, wave, lane, s1, rl, cl);
+   if (i < ne && b < ne) {
+      W[(size_t)i * ne + b] = w;
+      T1[(size_t)i * ne + b] = t;
+   }
+}
+
 /* W = G11 lmd and T1 = G21 lmd (G21[i][j] = <c2_i|c1_j>), both stored as
  * [i][b].  A workgroup forms 64 outputs (i, b), i fastest; its four thread
  * groups sum over the four quarters of j and the quarters are added in a
@@ -624,7 +718,6 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
    /* K padded to whole 4-wide steps for each of the four waves */
    const int kp = (ne + 15) & ~15, kq = kp / 4, kb = wave * kq, ke = kb + kq;
    const int s1 = rot_step(lane);
-   const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
    /* this thread's element of the tile */
    const int rl = tid % TS, cl = tid / TS, r = ti * TS + rl, c = tj * TS + cl;
    const bool mine = r < ne && c < ne;
@@ -663,27 +756,11 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
          if (x % NM < ne) Cp[x / NM][x % NM] = v[u];
       }
    };
-   /* acc += Pr(16 x K) Qc^T over this wave's K quarter; Pr[i][k] = P(i, k),
-    * Qc[j][k] = Q(k, j) */
+   /* acc += P Q over this wave's K quarter; the four waves' tiles summed */
    auto tile = [&](double (*Pr)[P], double (*Qc)[P], double (&acc)[4]) {
-      for (int m0 = kb; m0 < ke; m0 += 4) {
-         const int m = m0 + q;
-         const double pa = Pr[lane & 15][m];
-         double bq[4];
-         rotations(Qc[lane & 15][m], bq);
-#pragma unroll
-         for (int t = 0; t < 4; ++t) acc[t] = mfma4(pa, bq[t], acc[t]);
-      }
+      tile16_mfma<P>(Pr, Qc, kb, ke, lane, acc);
    };
-   /* the sum of the four waves' tiles at this thread's element */
-   auto reduce = [&](const double (&acc)[4]) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) part[wave][4 * ab + q][4 * ((ab + s1 * t) & 3) + bb] = acc[t];
-      __syncthreads();
-      const double v = ((part[0][rl][cl] + part[1][rl][cl]) + part[2][rl][cl]) + part[3][rl][cl];
-      __syncthreads();
-      return v;
-   };
+   auto reduce = [&](const double (&acc)[4]) { return tile16_reduce(part, acc, wave, lane, s1, rl, cl); };
 
    /* K padding read by the MFMA steps stays zero: the loads write k < ne only */
    for (int x = tid; x < 6 * TS * (kp - ne); x += 256) {
@@ -952,7 +1029,13 @@ int sd_gram_next(const double *Gm, const double *g11, int ne, int c1_second, con
    double *W = work, *T1 = work + (size_t)ne * ne;
    if (ne % 4) { cpsd_set_error("gram_next: ne=%d is not a multiple of 4", ne); return -1; }
    const dim3 grid((unsigned)(((size_t)ne * ne + 63) / 64));
-   hipLaunchKernelGGL(k_gram_next_w, grid, dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W, T1);
+   if (ne <= 128) {
+      const int nt = (ne + 15) / 16;
+      hipLaunchKernelGGL(k_gram_next_w_mf<128>, dim3(nt * nt), dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W,
+                         T1);
+   } else {
+      hipLaunchKernelGGL(k_gram_next_w, grid, dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W, T1);
+   }
    CPSD_CHECK(hipGetLastError());
    hipLaunchKernelGGL(k_gram_next, grid, dim3(256), 0, s, Gm, ne, c1_second, lmd, dte, W, T1, out);
    CPSD_CHECK(hipGetLastError());
