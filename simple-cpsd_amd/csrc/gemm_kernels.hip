@@ -487,11 +487,13 @@ int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *
  * workgroup covers 64 consecutive outputs with four thread groups: group g
  * sums the splits z = g mod 4 in two interleaved chains, and the four group
  * sums are added in g order, so the result does not depend on scheduling. */
-__global__ void __launch_bounds__(256) k_reduce_partials(const double *__restrict__ partial, int splits,
+#define RW 16   /* waves per reduction workgroup, each over every RW-th split */
+
+__global__ void __launch_bounds__(64 * RW) k_reduce_partials(const double *__restrict__ partial, int splits,
                                                          int sp_dd, int M, int N, int symm, int skip0,
                                                          int skip1, double *__restrict__ out)
 {
-   __shared__ double red[4][64];
+   __shared__ double red[RW][64];
    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
    const size_t idx = (size_t)blockIdx.x * 64 + l;
    const int m = (int)(idx / N), n = (int)(idx % N);
@@ -500,25 +502,27 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double *__restric
    const bool valid = idx < (size_t)M * N && !(symm && (m / symm) > (n / symm)) &&
                       !(m >= skip0 && m < skip1 && n >= skip0 && n < skip1);
    const int nz = (symm && m / TT == n / TT) ? sp_dd : splits;
-   /* wave g sums splits g, g + 4, ... into eight accumulators, so that eight
+   /* wave g sums splits g, g + RW, ... into eight accumulators, so that eight
     * loads are in flight per lane, then adds them in a fixed tree */
    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
    if (valid) {
       const size_t stride = (size_t)M * N;
       const double *p = partial + (size_t)m * N + n;
       int z = g;
-      for (; z + 28 < nz; z += 32) {
+      for (; z + 7 * RW < nz; z += 8 * RW) {
 #pragma unroll
-         for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(z + 4 * u) * stride];
+         for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(z + RW * u) * stride];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-         if (z + 4 * u < nz) acc[u] += p[(size_t)(z + 4 * u) * stride];
+         if (z + RW * u < nz) acc[u] += p[(size_t)(z + RW * u) * stride];
    }
    red[g][l] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
    __syncthreads();
    if (g == 0 && valid) {
-      const double v = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+      double v = red[0][l];
+#pragma unroll
+      for (int w = 1; w < RW; ++w) v += red[w][l];
       out[idx] = v;
       if (symm && (m / symm) < (n / symm)) out[(size_t)n * N + m] = v;
    }
@@ -528,7 +532,7 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
                            int symm, hipStream_t s)
 {
    const size_t tot = (size_t)M * N;
-   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
                       splits, splits, M, N, symm, 0, 0, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
@@ -538,7 +542,7 @@ int launch_reduce_partials_split(const double *partial, int sp_off, int sp_dd, i
                                  double *out, hipStream_t s)
 {
    const size_t tot = (size_t)M * M;
-   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
                       sp_off, sp_dd, M, M, 16, skip0, skip1, out);
    CPSD_CHECK(hipGetLastError());
    return 0;
@@ -550,7 +554,7 @@ int launch_reduce_partials_symm(const double *partial, int M, size_t K, int spli
    int so, sd;
    gemm_tn_symm_splits(M, K, splits, &so, &sd);
    const size_t tot = (size_t)M * M;
-   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(256), 0, s, partial,
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
                       so, sd, M, M, 16, 0, 0, out);
    CPSD_CHECK(hipGetLastError());
    return 0;

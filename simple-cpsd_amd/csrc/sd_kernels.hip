@@ -171,8 +171,18 @@ __global__ void k_dn_reduce_xc(const double *__restrict__ dnpart, int nparts, si
 {
    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (g >= n3) return;
+   /* the parts added in order, loaded four at a time */
    double s = 0.0;
-   for (int b = 0; b < nparts; ++b) s += dnpart[b * n3 + g];
+   int b = 0;
+   for (; b + 4 <= nparts; b += 4) {
+      const double v0 = dnpart[(size_t)b * n3 + g], v1 = dnpart[(size_t)(b + 1) * n3 + g];
+      const double v2 = dnpart[(size_t)(b + 2) * n3 + g], v3 = dnpart[(size_t)(b + 3) * n3 + g];
+      s += v0;
+      s += v1;
+      s += v2;
+      s += v3;
+   }
+   for (; b < nparts; ++b) s += dnpart[(size_t)b * n3 + g];
    const double d = s * scal2;
    dn[g] = d;
    double p, e;
