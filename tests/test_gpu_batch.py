@@ -1,7 +1,8 @@
 """Several independent cells advanced together (cpsd_dev_run_start / _finish,
 cpsd_amd.run_batch) against the same cells run one after the other, bit for
 bit: each cell has its own stream and buffers, so running them concurrently
-must not change any result (with and without HIP-graph replay)."""
+must not change any result (with and without HIP-graph replay, in FP64 and in
+the mixed precision modes)."""
 import sys
 
 import numpy as np
@@ -19,22 +20,23 @@ def say(msg):
     sys.stdout.flush()
 
 
-def make(graphs):
+def make(graphs, mode=0):
     devs = []
     for i, (nmol, L, n) in enumerate(CELLS):
         cell = cpsd_amd.water_cell(nmol, L, n, seed=i)
         d = cpsd_amd.DeviceSD(cell, 4 * nmol)
+        d.set_precision(mode)
         d.set_graphs(graphs)
         d.set_orbitals(cell.random_orbitals(4 * nmol, seed=10 + i, decay=0.05))
         devs.append(d)
     return devs
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_batch_equals_sequential(gpu, graphs):
-    say("4 cells batched vs one after the other, graphs=%s" % graphs)
-    seq = make(False)
-    bat = make(graphs)
+@pytest.mark.parametrize("graphs,mode", [(False, 0), (True, 0), (False, 1), (True, 2)])
+def test_batch_equals_sequential(gpu, graphs, mode):
+    say("4 cells batched vs one after the other, graphs=%s, precision mode %d" % (graphs, mode))
+    seq = make(False, mode)
+    bat = make(graphs, mode)
     for call in range(4):
         ref = [d.run(5) for d in seq]
         got = cpsd_amd.run_batch(bat, 5)

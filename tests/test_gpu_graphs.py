@@ -11,7 +11,8 @@ The cases cover:
 - a change of dt (a second graph), and a return to the first dt (a replay of
   the first);
 - new orbitals in between;
-- the mixed mode;
+- the mixed modes (FP32 transforms; with the BF16 KB projection, whose MFMA
+  kernel runs in the 128-orbital case);
 - 128 orbitals, where the overlap is carried from step to step.
 """
 import sys
@@ -56,7 +57,7 @@ def same_state(devs):
         assert np.array_equal(a[k], b[k]), k
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_graph_replays_water(gpu, mode):
     say("graph replay, water 32^3 ne=4, mode %d" % mode)
     st = water_oracle_state()
@@ -82,11 +83,12 @@ def test_graph_replays_water(gpu, mode):
 
 
 def test_graph_replays_carried_overlap(gpu):
-    """128 orbitals at 24^3: the Gram with a carried <c1|c1> in the graphs"""
-    say("graph replay, ne=128 at 24^3, both modes")
+    """128 orbitals at 24^3: the Gram with a carried <c1|c1> in the graphs,
+    in all three precision modes"""
+    say("graph replay, ne=128 at 24^3, modes 0-2")
     cell = cpsd_amd.water_cell(2, 12.0, 24, seed=7)
     psi = cell.random_orbitals(128, seed=8, decay=0.05)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         devs = pair(cell, 128, psi, mode)
         for _ in range(5):
             same_call(devs, 3)
