@@ -11,7 +11,9 @@ Bars:
   by dte times the rounding of -H psi, about 1e-8 here;
 - orthonormality and the electron count at the FP64 level, since the constraint
   and the density sum stay FP64;
-- a one-step call (its only step is the last, FP64 one) bit-identical to FP64.
+- a one-step call (its only step is the last, FP64 one) bit-identical to FP64;
+- with moving ions, positions and forces within bounds set from the measured
+  drift of the FP32 steps' forces (mode 1 and mode 2).
 """
 import sys
 
@@ -202,3 +204,38 @@ def test_bf16_projection_converges_to_fp64(gpu):
     assert abs(out[1] - out[0]) < 1e-10 * natom
     assert abs(out[2] - out[0]) < 1e-6 * natom
     assert out[2] != out[1]
+
+
+@pytest.mark.parametrize("mode,tol_e,tol_r,tol_f", [(1, 1e-6, 1e-6, 1e-6), (2, 2e-5, 3e-4, 2e-3)])
+def test_mixed_moving_ions_vs_fp64(gpu, mode, tol_e, tol_r, tol_f):
+    """move = 1 in the mixed modes: 4 waters at 32^3, 16 orbitals and 24
+    projectors (so the FP32 or BF16 projection kernel runs), 4 calls of 5
+    steps against FP64 from the same start.  Every step moves the ions, the
+    FP32 ones with forces from their FP32 products, so positions and forces
+    drift from FP64's.  Measured after 20 steps, with the ions moved about
+    2 bohr: mode 1 4e-8 bohr and 6e-8 Ha/bohr, mode 2 (BF16 projection
+    coefficients in the nonlocal force) 3e-5 bohr and 3e-4 Ha/bohr; the
+    bounds are about ten times those.  A missing or wrong force term moves
+    forces by ~0.1 Ha/bohr."""
+    say("mode %d with moving ions vs FP64, 4 waters 32^3 ne=16" % mode)
+    cell = cpsd_amd.water_cell(4, 14.0, 32, seed=0)
+    ne, natom = 16, 12
+    psi = cell.random_orbitals(ne, seed=1, decay=0.05)
+    devs = []
+    for m in (0, mode):
+        d = cpsd_amd.DeviceSD(cell, ne)
+        d.set_precision(m)
+        d.set_ions(1)
+        d.set_orbitals(psi)
+        devs.append(d)
+    for _ in range(4):
+        E64, _, _ = devs[0].run(5)
+        Em, _, _ = devs[1].run(5)
+        a, b = devs[0].get_ions(), devs[1].get_ions()
+        assert abs(Em[0] - E64[0]) < tol_e * natom, (Em[0], E64[0])
+        assert np.max(np.abs(a["r2"] - b["r2"])) < tol_r
+        assert np.max(np.abs(a["fion"] - b["fion"])) < tol_f
+    assert np.max(np.abs(a["r2"] - cell.r)) > 0.5        # the ions did move
+    assert not np.array_equal(a["r2"], b["r2"])         # the FP32 steps did run
+    for d in devs:
+        d.close()
