@@ -1043,6 +1043,10 @@ int sd_gram_next(const double *Gm, const double *g11, int ne, int c1_second, con
       const int nt = (ne + 15) / 16;
       hipLaunchKernelGGL(k_gram_next_w_mf<128>, dim3(nt * nt), dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W,
                          T1);
+   } else if (ne <= 256) {
+      const int nt = (ne + 15) / 16;
+      hipLaunchKernelGGL(k_gram_next_w_mf<256>, dim3(nt * nt), dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W,
+                         T1);
    } else {
       hipLaunchKernelGGL(k_gram_next_w, grid, dim3(256), 0, s, Gm, g11, ne, c1_second, lmd, W, T1);
    }
