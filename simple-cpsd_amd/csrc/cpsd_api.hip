@@ -709,7 +709,7 @@ static int sd_step(cpsd_dev *c, double dte, int last)
                                                   c->Gm, st));
       else if (!small) TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
       TRY(sd_smat(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->s11, c->s12, c->s21,
-                  c->s22, st));
+                  c->s22, c->bar, st));
       TRY(ph_end(c, PH_GRAMRED));
    }
    /* 12 */

@@ -22,7 +22,7 @@ struct LagArgs {
    double *lmd;
    double *wgmax;    /* per workgroup */
    double *info;     /* [iterations, last max|dX|] */
-   unsigned *bar;
+   unsigned *bar;    /* grid-barrier counter, zero at launch (sd_smat clears it) */
    int *err;
 };
 
@@ -61,9 +61,10 @@ int sd_ewald_real(const double *r1, const int *katm, const double *zv, int nion,
 int sd_erfc_as(const double *w, int n, double *out, hipStream_t s);
 /* s11, s12, s21, s22 of ccpsdv3/cpsd.c:415-437 from the Gram Gm of [C1 C2]
  * (or of [C2 C1], c1_second); with g11 non-null, <c1_i|c1_j> is read from
- * the dense ne x ne g11 instead of Gm */
+ * the dense ne x ne g11 instead of Gm.  A non-null bar (the Lagrange
+ * kernel's barrier counter) is set to zero in the same launch. */
 int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *s11, double *s12,
-            double *s21, double *s22, hipStream_t s);
+            double *s21, double *s22, unsigned *bar, hipStream_t s);
 /* <c2f_a|c2f_b> of the orbitals after the constraint c2f = c2 + dte c1 lmd,
  * from the step's Gram (G11 from g11 when non-null) and lmd[i + j ne]:
  * G22 + dte (G21 lmd + lmd^T G12) + dte^2 lmd^T G11 lmd, into out (ne x ne);

@@ -331,9 +331,10 @@ __global__ void k_erfc_as(const double *__restrict__ w, int n, double *__restric
  * filled from one triangle and mirrored, as the reference does. */
 __global__ void k_smat(const double *__restrict__ Gm, const double *__restrict__ g11c, int ne, int c1_second,
                        double dte, double *__restrict__ s11, double *__restrict__ s12, double *__restrict__ s21,
-                       double *__restrict__ s22)
+                       double *__restrict__ s22, unsigned *__restrict__ bar)
 {
    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (bar && idx == 0) *bar = 0u;
    if (idx >= ne * ne) return;
    const int r = idx % ne, c = idx / ne;   /* element (r, c) at r + c*ne */
    const int i = min(r, c), j = max(r, c);
@@ -833,11 +834,6 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
    }
 }
 
-__global__ void k_zero_u32(unsigned *p, int n)
-{
-   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n) p[i] = 0u;
-}
 
 /* ------------------------------------------------------ energy assembly */
 
@@ -1025,10 +1021,10 @@ int sd_erfc_as(const double *w, int n, double *out, hipStream_t s)
 }
 
 int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *s11, double *s12,
-            double *s21, double *s22, hipStream_t s)
+            double *s21, double *s22, unsigned *bar, hipStream_t s)
 {
    hipLaunchKernelGGL(k_smat, g1((size_t)ne * ne), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, s11, s12, s21,
-                      s22);
+                      s22, bar);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -1065,8 +1061,6 @@ int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double sca
 
 int sd_lagrange(const LagArgs &a, hipStream_t s)
 {
-   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.bar, 1);
-   CPSD_CHECK(hipGetLastError());
    if (a.ne <= 128) {
       const int nt = (a.ne + 15) / 16;
       const int lds = 6 * 16 * (128 + 2) * (int)sizeof(double);
