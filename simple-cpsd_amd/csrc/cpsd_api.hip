@@ -644,8 +644,8 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       }
       TRY(ph_end(c, PH_KBPROJ));
       TRY(ph_begin(c, PH_KBRED));
-      if (!small) TRY(launch_reduce_partials(c->partial, sp, c->nproj, ne, c->P, 0, st));
-      TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
+      if (!small) TRY(launch_reduce_partials_kb(c->partial, sp, c->nproj, ne, c->P, c->pnrm, scal2, c->Xkb, st));
+      else TRY(sd_kbcoef(c->P, c->nproj, ne, c->pnrm, scal2, c->Xkb, st));
       TRY(ph_end(c, PH_KBRED));
    }
    if (c->move) {

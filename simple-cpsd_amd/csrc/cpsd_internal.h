@@ -135,6 +135,10 @@ int launch_gemm_tn_splitk(const double *A, size_t lda, int M, const double *B, s
  * split count) */
 int launch_reduce_partials(const double *partial, int splits, int M, int N, double *out,
                            int symm, hipStream_t s);
+/* the KB projections' split-K reduction into P (nproj x ne) that also writes
+ * the apply coefficients X[m + n nproj] = -P[m][n] scal2 / pnrm[m] */
+int launch_reduce_partials_kb(const double *partial, int splits, int nproj, int ne, double *P, const double *pnrm,
+                              double scal2, double *X, hipStream_t s);
 /* the same for a symmetric launch_gemm_tn_splitk(..., splits, symm = 1, ...),
  * whose diagonal and off-diagonal tiles use different split counts */
 int launch_reduce_partials_symm(const double *partial, int M, size_t K, int splits, double *out,

@@ -489,9 +489,13 @@ int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *
  * sums are added in g order, so the result does not depend on scheduling. */
 #define RW 16   /* waves per reduction workgroup, each over every RW-th split */
 
+/* with X non-null (KB projections: out = P, nproj x ne), also the apply
+ * coefficients X[m + n M] = -P[m][n] scal2 / pnrm[m] (ccpsdv3/cpsd.c:289) */
 __global__ void __launch_bounds__(64 * RW) k_reduce_partials(const double *__restrict__ partial, int splits,
                                                          int sp_dd, int M, int N, int symm, int skip0,
-                                                         int skip1, double *__restrict__ out)
+                                                         int skip1, double *__restrict__ out,
+                                                         const double *__restrict__ pnrm, double scal2,
+                                                         double *__restrict__ X)
 {
    __shared__ double red[RW][64];
    const int g = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -525,6 +529,7 @@ __global__ void __launch_bounds__(64 * RW) k_reduce_partials(const double *__res
       for (int w = 1; w < RW; ++w) v += red[w][l];
       out[idx] = v;
       if (symm && (m / symm) < (n / symm)) out[(size_t)n * N + m] = v;
+      if (X) X[m + (size_t)n * M] = -v * scal2 / pnrm[m];
    }
 }
 
@@ -533,7 +538,17 @@ int launch_reduce_partials(const double *partial, int splits, int M, int N, doub
 {
    const size_t tot = (size_t)M * N;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
-                      splits, splits, M, N, symm, 0, 0, out);
+                      splits, splits, M, N, symm, 0, 0, out, nullptr, 0.0, nullptr);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int launch_reduce_partials_kb(const double *partial, int splits, int nproj, int ne, double *P, const double *pnrm,
+                              double scal2, double *X, hipStream_t s)
+{
+   const size_t tot = (size_t)nproj * ne;
+   hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
+                      splits, splits, nproj, ne, 0, 0, 0, P, pnrm, scal2, X);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -543,7 +558,7 @@ int launch_reduce_partials_split(const double *partial, int sp_off, int sp_dd, i
 {
    const size_t tot = (size_t)M * M;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
-                      sp_off, sp_dd, M, M, 16, skip0, skip1, out);
+                      sp_off, sp_dd, M, M, 16, skip0, skip1, out, nullptr, 0.0, nullptr);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -555,7 +570,7 @@ int launch_reduce_partials_symm(const double *partial, int M, size_t K, int spli
    gemm_tn_symm_splits(M, K, splits, &so, &sd);
    const size_t tot = (size_t)M * M;
    hipLaunchKernelGGL(k_reduce_partials, dim3((tot + 63) / 64), dim3(64 * RW), 0, s, partial,
-                      so, sd, M, M, 16, 0, 0, out);
+                      so, sd, M, M, 16, 0, 0, out, nullptr, 0.0, nullptr);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
