@@ -15,6 +15,38 @@ def example0_expected():
         return json.load(f)
 
 
+def water_expected():
+    with open(os.path.join(GOLDEN, "h2o", "expected.json")) as f:
+        return json.load(f)
+
+
+def water_positions(dx=0.0, dy=0.0):
+    """the recorded water geometry with both H moved by dx outward (along
+    +-x) and by dy along y: the two symmetric directions in which the
+    6-decimal rounding of the recorded H coordinates can move them"""
+    exp = water_expected()
+    r = np.array(exp["positions_bohr"], dtype=float)
+    r[3] += dx; r[6] -= dx
+    r[4] += dy; r[7] += dy
+    return r
+
+
+def rounding_window(energy_of, h=1e-5):
+    """How far E(200) can move when each recorded H coordinate moves by up
+    to half a unit of its last recorded decimal: sum over the 4 rounded
+    coordinates of |dE/dq| * 5e-7, from forward differences of energy_of
+    (a function of the positions).  O sits at the exact origin and the H
+    z = 0 exactly, so they carry no rounding.  By the mirror symmetry
+    x -> -x, dE/dx_H1 = -dE/dx_H2 and dE/dy_H1 = dE/dy_H2, so two symmetric
+    displacements give all four derivatives."""
+    exp = water_expected()
+    half = 0.5 * 10.0 ** (-exp["positions_recorded_decimals"])
+    e0 = energy_of(water_positions())
+    gx = (energy_of(water_positions(dx=h)) - e0) / h / 2.0   # per H coordinate
+    gy = (energy_of(water_positions(dy=h)) - e0) / h / 2.0
+    return e0, 2.0 * half * (abs(gx) + abs(gy))
+
+
 def read_control(path):
     t = open(path).read().split()
     return dict(ispin=int(t[0]), imove=int(t[1]), icube=int(t[2]), unit=float(t[3]),

@@ -12,7 +12,8 @@ import pytest
 
 import cpsd_amd
 import oracle_lib as O
-from example_inputs import example0_expected, example0_oracle_state, water_oracle_state
+from example_inputs import (example0_expected, example0_oracle_state, rounding_window,
+                            water_expected, water_oracle_state)
 
 pytestmark = pytest.mark.gpu
 
@@ -193,6 +194,44 @@ def test_example0_golden_1000_steps_on_gpu(gpu):
     assert abs(E[0] - exp["E0"]) < 5e-9
     assert de == pytest.approx(exp["deltae"], rel=5e-3)
     dev.close()
+
+
+def test_water_golden_200_steps_on_gpu(gpu):
+    """The reference's own water run (two kinds: O s+p, H s): 20 drop-in
+    cpsd() calls of 10 steps against the oracle (E components 1e-9 Ha, dn
+    1e-10), and the resident path's E(200) against the reference's printed
+    energy within the window that the 6-decimal recorded geometry leaves
+    (the same window as the oracle's CPU test, here from GPU runs)."""
+    say("water golden 200 steps")
+    exp = water_expected()
+    st = water_oracle_state()
+    cell = cell_from_state(st)
+    c_start = st.c2.copy()
+    c2 = st.c2.copy()
+    E = np.zeros(10)
+    for call in range(exp["outer"]):
+        out = cpsd_amd.cpsd_dropin(cell, exp["ne"], exp["inner"], c2, E, exp["dt"], exp["fmass"])
+        c2, E = out["c2"], out["E"]
+    for _ in range(exp["outer"]):
+        Er, der, dcr = st.step(exp["inner"], exp["dt"], exp["fmass"])
+    compare(E, Er)
+    assert abs(out["deltae"] - der) < 1e-9
+    assert np.max(np.abs(out["dn"] - st.dn)) < 1e-10
+
+    def energy_of(r):
+        dev = cpsd_amd.DeviceSD(cell_from_state(water_oracle_state(positions=r, katm=[0, 1, 1])),
+                                exp["ne"])
+        dev.set_orbitals(c_start)
+        Eg = None
+        for _ in range(exp["outer"]):
+            Eg, _, _ = dev.run(exp["inner"], exp["dt"], exp["fmass"])
+        dev.close()
+        return Eg[0]
+
+    E0, window = rounding_window(energy_of)
+    assert abs(E0 - E[0]) < 1e-9          # resident and drop-in agree
+    assert 1e-9 < window < 5e-8
+    assert abs(E0 - exp["E0"]) < window + 5e-10
 
 
 def test_invariants_large_cell(gpu):

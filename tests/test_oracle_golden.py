@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from example_inputs import example0_expected, example0_oracle_state, water_oracle_state
+from example_inputs import (example0_expected, example0_oracle_state, rounding_window,
+                            water_expected, water_oracle_state, water_positions)
 
 
 def test_example0_energy_after_1000_steps():
@@ -53,3 +54,32 @@ def test_lagrange_converges_and_lmd_symmetric_ish():
     st = water_oracle_state()
     st.step(3)
     assert O.lib().oracle_last_lagrange_iterations() < 30
+
+
+def _water_energy(positions, mutate=None):
+    exp = water_expected()
+    st = water_oracle_state(positions=positions, katm=[0, 1, 1])
+    if mutate is not None:
+        mutate(st)
+    E = None
+    for _ in range(exp["outer"]):
+        E, _, _ = st.step(exp["inner"], exp["dt"], exp["fmass"])
+    return E[0]
+
+
+def test_water_energy_after_200_steps():
+    """The reference's own water run (one O00/H00 molecule, two kinds, O s+p
+    and H s projectors): E after 200 steps.  The run's geometry is known to
+    6 decimals only, and E(200) from the plane-wave start moves by about
+    2e-2 Ha per bohr of H displacement, so the recorded value pins the
+    two-kind path to the window that rounding leaves (about 3e-8 Ha), plus
+    the 5e-10 of the 11 printed digits."""
+    exp = water_expected()
+    E0, window = rounding_window(_water_energy)
+    assert 1e-9 < window < 5e-8
+    assert abs(E0 - exp["E0"]) < window + 5e-10
+    # the window still separates a two-kind mistake of 1e-6 relative in one
+    # of the H kind's tables (planted in the oracle's input, not its code)
+    def bad(st):
+        st.vnlnrm[9] *= 1.0 + 1e-6
+    assert abs(_water_energy(water_positions(), bad) - exp["E0"]) > window + 5e-10
