@@ -169,6 +169,26 @@ int cpsd_kbpp_device(double zv, int lmax, int nrho, double drho, const double *r
  * Batched versions of the FFT entry points and the LDA kernel, used by
  * the parity tests.                                                     */
 int cpsd_fft3d_batch(int dir, int nfft, int nbatch, double *A);  /* dir +1: cr, -1: rc */
+
+/* ---- allocation-free entry point -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cpsd_fft3d_batch() on the caller's DEVICE array
+ * A[nbatch][(nfft+2) nfft nfft] doubles, transformed in place with the same
+ * kernels (dir +1: half-complex in, padded real out as cr_fft3b_; dir -1:
+ * the reverse as rc_fft3f_), enqueued on `stream` (a hipStream_t; NULL is
+ * the null stream) and not waited for.  It calls no hipMalloc, hipFree or
+ * other allocating routine: the z-transformed intermediate and the kernels'
+ * tables live in a file-scope device array of CPSD_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds until the process ends, so nbatch is limited to
+ * cpsd_fft3d_batch_noalloc_max(nfft) (3 at 128^3, 9 at 96^3, 240 at 32^3);
+ * larger requests return -1 with the reason in cpsd_last_error() and enqueue
+ * nothing.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams or host threads) race and must be
+ * serialised by the caller.  Returns 0 on success.                      */
+#define CPSD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int cpsd_fft3d_batch_noalloc(int dir, int nfft, int nbatch, double *A, void *stream);
+long cpsd_fft3d_batch_noalloc_max(int nfft);
 int cpsd_lda_device(int n, const double *dn, double *xcp, double *xce);
 int cpsd_erfc_device(int n, const double *w, double *out);
 /* the host eigensolver of cpsd_rotate() and cpsd_dev_epilogue() (cyclic

@@ -83,6 +83,9 @@ def hip_lib():
         L.cpsd_dev_debug_fetch.argtypes = [C.c_void_p, C.c_int, _dp, C.c_long]
         L.cpsd_dev_debug_fetch.restype = C.c_long
         L.cpsd_fft3d_batch.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.cpsd_fft3d_batch_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.cpsd_fft3d_batch_noalloc_max.argtypes = [C.c_int]
+        L.cpsd_fft3d_batch_noalloc_max.restype = C.c_long
         L.cpsd_lda_device.argtypes = [C.c_int, _dp, _dp, _dp]
         L.cpsd_erfc_device.argtypes = [C.c_int, _dp, _dp]
         L.cpsd_dev_epilogue.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
