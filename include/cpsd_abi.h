@@ -132,6 +132,19 @@ int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
  * has no such mode; its all-FP32 build (float_single.h with sfftpack3d) is
  * the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
+/* the preconditioned SD step (no counterpart in the reference): with
+ * K[nfft3d] set (one factor per half-complex G in the reference's layout,
+ * e.g. the output of cpsd_precond_predict()), every step of a call becomes
+ * c2 = c1 + K o dte (cpsi + c1 h) before the overlaps, the Lagrange
+ * iteration and the constraint, with h the Lagrange matrix hml of the last
+ * energy evaluation of these orbitals; the reference's step
+ * c2 = c1 + dte cpsi (ccpsdv3/cpsd.c:399-400) is K = 1, h = 0.  At the
+ * fixed point h = <c1|H|c1>, so the converged orbitals are the reference's.
+ * Calls start preconditioning once an energy evaluation of the current
+ * orbitals exists (the first call after cpsd_dev_set_orbitals() or the
+ * epilogue runs plain steps).  cpsi, hml and the energies keep the
+ * reference's definitions.  K = NULL turns it off. */
+int cpsd_dev_set_preconditioner(cpsd_dev *h, const double *K);
 /* enable != 0: each cpsd_dev_run() call (its inner steps and the energy
  * evaluation) is captured once into a HIP graph and replayed on later calls
  * with the same inner, dt, fmass, precision mode and state parity, which

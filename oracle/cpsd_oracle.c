@@ -193,6 +193,45 @@ double oracle_erfc_as(double w)
 static int g_lag_iters = 0;
 int oracle_last_lagrange_iterations(void) { return g_lag_iters; }
 
+/* ---- preconditioned SD (test infrastructure for the library's
+ * cpsd_dev_set_preconditioner; no counterpart in the reference) ---------- */
+static const double *g_precond_K;   /* [nfft3d], one factor per complex G */
+static const double *g_precond_h;   /* [ne*ne] column-major */
+
+void oracle_set_preconditioner(const double *K, const double *h)
+{
+   g_precond_K = K;
+   g_precond_h = h;
+}
+
+/* c2 = c1 + K o (c2 - c1 + dte c1 h): with c2 = c1 + dte cpsi on entry this
+ * is c1 + K o dte (cpsi + c1 h), the residual -H c1 + c1 h scaled per G */
+static void apply_preconditioner(int nfft3d, size_t n2, int ne, double dte, const double *c1, double *c2)
+{
+   double *t = (double *) malloc(n2 * ne * sizeof(double));
+   size_t k;
+   int i, m;
+   for (i = 0; i < ne; ++i) {
+      double *ti = t + i * n2;
+      for (k = 0; k < n2; ++k) ti[k] = 0.0;
+      for (m = 0; m < ne; ++m) {
+         const double hm = g_precond_h[m + (size_t)i * ne];
+         const double *cm = c1 + m * n2;
+         for (k = 0; k < n2; ++k) ti[k] += cm[k] * hm;
+      }
+   }
+   for (i = 0; i < ne; ++i) {
+      const double *a = c1 + i * n2, *ti = t + i * n2;
+      double *b = c2 + i * n2;
+      for (k = 0; k < (size_t)nfft3d; ++k) {
+         const double f = g_precond_K[k];
+         b[2*k]     = a[2*k]     + f * (b[2*k]     - a[2*k]     + dte * ti[2*k]);
+         b[2*k + 1] = a[2*k + 1] + f * (b[2*k + 1] - a[2*k + 1] + dte * ti[2*k + 1]);
+      }
+   }
+   free(t);
+}
+
 int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
                 int nfft, int nfft3d, int n2ft3d, const double *unitg,
                 double omega, double dt, double fmass, const int *katm,
@@ -403,6 +442,11 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
       /* steepest descent, :399-411 */
       memcpy(c2, c1, nn * sizeof(double));
       axpy(nn, dte, cpsi, c2);
+      /* preconditioned form (no counterpart in the reference; the checker
+       * of the library's preconditioned step): the step of every orbital
+       * becomes K(G) times the residual with a given Lagrange matrix h,
+       * c2 = c1 + K o (dte cpsi + dte c1 h) */
+      if (g_precond_K) apply_preconditioner(nfft3d, n2, n0, dte, c1, c2);
       if (move)
          for (ii = 0; ii < nion; ++ii)
             for (j = 0; j < 3; ++j) r2[3*ii+j] = r1[3*ii+j] + dti[ii] * fion[3*ii+j];

@@ -95,6 +95,12 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
                 double *cpsdwork, double *E, double *deltae, double *deltac,
                 double *deltar);
 int oracle_last_lagrange_iterations(void);
+/* preconditioned SD step (no counterpart in the reference): with K set,
+ * every step becomes c2 = c1 + K o dte (cpsi + c1 h) before the overlaps and
+ * the constraint; K[nfft3d] one factor per half-complex G, h[ne*ne]
+ * column-major; K = NULL restores the reference's step.  Pointers are kept,
+ * not copied. */
+void oracle_set_preconditioner(const double *K, const double *h);
 
 /* Symmetric eigen-decomposition used by the epilogue (ccpsdv3/main.c:470-484:
  * dsyev + eigsrt).  Cyclic Jacobi; eigenvalues sorted descending as eigsrt

@@ -37,7 +37,8 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_profile_read", "cpsd_dev_destroy", "cpsd_dev_debug_fetch", "cpsd_fft3d_batch",
                "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
                "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision", "cpsd_dev_set_graphs",
-               "cpsd_dev_run_start", "cpsd_dev_run_finish")
+               "cpsd_dev_run_start", "cpsd_dev_run_finish", "cpsd_fft3d_batch_noalloc",
+               "cpsd_fft3d_batch_noalloc_max", "cpsd_dev_set_preconditioner")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
                 "cpsdh_write_ionout")
@@ -74,6 +75,7 @@ def hip_lib():
         L.cpsd_dev_lagrange_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), _DPTR]
         L.cpsd_dev_set_precision.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_set_graphs.argtypes = [C.c_void_p, C.c_int]
+        L.cpsd_dev_set_preconditioner.argtypes = [C.c_void_p, C.c_void_p]
         L.cpsd_dev_run_start.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.cpsd_dev_run_finish.argtypes = [C.c_void_p, _dp, _DPTR, _DPTR, _DPTR]
         L.cpsd_dev_sizes.argtypes = [C.c_void_p, _ip]
@@ -423,6 +425,18 @@ class DeviceSD:
         step of a run() but its last (energies stay FP64 values of the current
         orbitals); 2: as 1 with a BF16-input KB projection"""
         _check(hip_lib().cpsd_dev_set_precision(self.h, int(mode)), "cpsd_dev_set_precision")
+
+    def set_preconditioner(self, K):
+        """K: one factor per half-complex G ([nfft3d], the reference's layout)
+        for the preconditioned step c2 = c1 + K o dte (cpsi + c1 hml), or None
+        for the reference's step"""
+        if K is None:
+            _check(hip_lib().cpsd_dev_set_preconditioner(self.h, None), "cpsd_dev_set_preconditioner")
+            return
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        if K.size != self.cell.nfft3d:
+            raise CpsdError("preconditioner needs %d factors, got %d" % (self.cell.nfft3d, K.size))
+        _check(hip_lib().cpsd_dev_set_preconditioner(self.h, K.ctypes.data), "cpsd_dev_set_preconditioner")
 
     def set_graphs(self, enable=True):
         """capture each run() into a HIP graph and replay it on later calls

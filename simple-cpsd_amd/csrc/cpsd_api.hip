@@ -61,11 +61,12 @@ static void fail_loudly(const char *where)
 /* one phase per hot kernel where the bench reports a roofline for it;
  * the split-K reductions are phases of their own */
 enum { PH_ZG2R, PH_DENSITY, PH_DNRED, PH_DNG, PH_VTOT, PH_VPSI, PH_ZR2G, PH_KBPROJ, PH_KBRED,
-       PH_KBAPPLY, PH_GRAM, PH_GRAMRED, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_FORCE, PH_COUNT };
+       PH_KBAPPLY, PH_GRAM, PH_GRAMRED, PH_LAGRANGE, PH_CONSTRAINT, PH_ENERGY, PH_FORCE, PH_PRECOND,
+       PH_COUNT };
 static const char *ph_names[PH_COUNT] = {
    "zpass_g2r", "plane_density", "dn_reduce_xc", "dng_r2c", "vtot_c2r", "plane_vpsi",
    "zpass_r2g_hpsi", "kb_projection", "kb_reduce", "kb_apply_sd", "gram", "gram_reduce",
-   "lagrange", "constraint", "energy_tail", "ion_forces"};
+   "lagrange", "constraint", "energy_tail", "ion_forces", "precondition"};
 
 template<typename T>
 static int dalloc(T **p, size_t n)
@@ -111,6 +112,11 @@ struct cpsd_dev {
     * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
    double *g11c[2], *g11w;
    int g11_idx, g11_valid;
+   /* preconditioned step (cpsd_dev_set_preconditioner): one factor per
+    * packed coefficient; the steps of a call use it when hml holds the
+    * Lagrange matrix of the current orbitals' last energy evaluation */
+   double *Kp;
+   int precond, hml_valid;
    long gram_wgs;         /* workgroups of the full Gram launch */
    /* HIP graphs of whole calls (inner steps + energy tail), keyed on what
     * the captured launches bake in; see ctx_run */
@@ -122,7 +128,7 @@ struct cpsd_dev {
    int *herr;
    int pending;
    struct GraphEntry {
-      int inner, mixed, cur0, g11i0, g11v0;
+      int inner, mixed, cur0, g11i0, g11v0, pre;
       double dt, dte;
       int cur1, g11i1, g11v1;
       hipGraphExec_t exec;
@@ -146,6 +152,7 @@ struct cpsd_dev {
    double *r2d, *fion, *dti, *Gp, *fgpart, *fnlpart, *fnlsum;
    int *poff, fchunk;
    int *katm_h; double *amass_h;
+   int *gidx_h;           /* grid index of each packed coefficient */
    double *Gp_h;
    /* profiling: event pairs recorded on the stream without synchronising;
     * they are resolved when the pool fills or the profile is read */
@@ -201,11 +208,12 @@ static void ctx_free(cpsd_dev *c)
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
-                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->Bb};
+                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->Bb, c->Kp};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
    free(c->amass_h);
+   free(c->gidx_h);
    free(c->Gp_h);
    for (int i = 0; i < c->ngc; ++i) (void)hipGraphExecDestroy(c->gcache[i].exec);
    if (c->hout) (void)hipHostFree(c->hout);
@@ -360,6 +368,9 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    TRY(h2d(c->rcell, rcell, 3 * (size_t)nsh, st));
    TRY(h2d(c->katm, katm, nion, st)); TRY(h2d(c->lmmax, lmmax, nkind, st));
    TRY(dalloc(&c->poff, nion)); TRY(h2d(c->poff, poff.data(), nion, st));
+   c->gidx_h = (int *)malloc((size_t)ngp * sizeof(int));
+   if (!c->gidx_h) { cpsd_set_error("host allocation failed"); return -1; }
+   memcpy(c->gidx_h, gidx.data(), (size_t)ngp * sizeof(int));
    c->katm_h = (int *)malloc((size_t)nion * sizeof(int));
    c->Gp_h = (double *)calloc(3 * (size_t)ngp, sizeof(double));
    if (!c->katm_h || !c->Gp_h) { cpsd_set_error("host allocation failed"); return -1; }
@@ -684,6 +695,20 @@ static int sd_step(cpsd_dev *c, double dte, int last)
       TRY(launch_gemm_tall(last ? EPI_KBAPPLY : EPI_CONSTRAINT, t, st));
    }
    TRY(ph_end(c, PH_KBAPPLY));
+   /* 10b: the preconditioned step.  C2 holds c1 + dte cpsi; it becomes
+    * c1 + K o dte (cpsi + c1 h), the per-G scaled residual, with h the
+    * Lagrange matrix hml of the last energy evaluation (exact at the fixed
+    * point, where h = <c1|H|c1>).  The overlaps and the constraint below
+    * then act on this c2 as they act on the plain one. */
+   if (c->precond && c->hml_valid) {
+      TRY(ph_begin(c, PH_PRECOND));
+      TallArgs t;
+      t.A = A2; t.lda = K; t.X = c->hml; t.xsm = 1; t.ldx = ne; t.Mr = ne; t.Nc = ne; t.K = K;
+      t.C2 = reinterpret_cast<double *>(C2); t.C1 = reinterpret_cast<const double *>(C1); t.CP = nullptr;
+      t.ldc = K; t.dte = dte; t.write_cp = 0; t.Kp = c->Kp;
+      TRY(launch_gemm_tall(EPI_PRECOND, t, st));
+      TRY(ph_end(c, PH_PRECOND));
+   }
    /* 11 */
    /* with <c1|c1> carried from the previous step, the Gram skips its C1 x C1
     * block (whose reduced entries are then not read) */
@@ -823,11 +848,13 @@ static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const 
       for (int i = 0; i < c->ngc; ++i) {
          cpsd_dev::GraphEntry &e = c->gcache[i];
          if (e.inner == inner && e.mixed == c->mixed && e.cur0 == c->cur && e.g11i0 == c->g11_idx &&
-             e.g11v0 == c->g11_valid && e.dt == dt && e.dte == dte) { ge = &e; break; }
+             e.g11v0 == c->g11_valid && e.pre == (c->precond && c->hml_valid) && e.dt == dt &&
+             e.dte == dte) { ge = &e; break; }
       }
       if (!ge && c->ngc < 8) {
          cpsd_dev::GraphEntry e;
          e.inner = inner; e.mixed = c->mixed; e.cur0 = c->cur; e.g11i0 = c->g11_idx; e.g11v0 = c->g11_valid;
+         e.pre = c->precond && c->hml_valid;
          e.dt = dt; e.dte = dte;
          hipGraph_t g = nullptr;
          CPSD_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -857,6 +884,7 @@ static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const 
       c->ran_mode[c->mixed] = 1;
    }
    if (c->move) c->struct_valid = 0;
+   c->hml_valid = 1;   /* the energy tail formed hml of these orbitals */
    CPSD_CHECK(hipMemcpyAsync(c->hout, c->E, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
    CPSD_CHECK(hipMemcpyAsync(c->hout + 12, c->info, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
    CPSD_CHECK(hipMemcpyAsync(c->herr, c->err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -951,6 +979,7 @@ static int ctx_set_orbitals(cpsd_dev *c, const double *cin)
    TRY(sd_check_unmasked(c->fullbuf, fstride, c->pidx, c->nfft3d, c->ne, c->flag, st));
    c->cur = 0;
    c->g11_valid = 0;
+   c->hml_valid = 0;
    TRY(sd_pack(c->fullbuf, fstride, c->gidx, c->ng, c->ngp, c->Cbase, c->ngp, c->ne, st));
    int flag = 0;
    CPSD_CHECK(hipMemcpyAsync(&flag, c->flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1001,6 +1030,7 @@ static int ctx_epilogue(cpsd_dev *c, double *ortho, double *eig, double *V)
 {
    if (!c->have_run) { cpsd_set_error("epilogue: no SD step has run on this context"); return -1; }
    c->g11_valid = 0;
+   c->hml_valid = 0;   /* c1 is rotated below */
    const int ne = c->ne;
    const size_t K = 2 * (size_t)c->ngp, orb = (size_t)ne * c->ngp, ee = (size_t)ne * ne;
    hipStream_t st = c->stream;
@@ -1128,6 +1158,37 @@ extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
    }
    h->mixed = mode;
    return 0;
+}
+
+/* K[nfft3d]: one factor per half-complex G of the reference's layout (the
+ * factors outside the cutoff mask are not used); NULL turns it off */
+static int ctx_set_preconditioner(cpsd_dev *c, const double *K)
+{
+   if (!K) { c->precond = 0; return 0; }
+   std::vector<double> kp(c->ngp, 0.0);
+   for (int p = 0; p < c->ng; ++p) {
+      const double f = K[c->gidx_h[p]];
+      if (!(f >= 0.0) || !(f < 1e6)) {
+         cpsd_set_error("cpsd_dev_set_preconditioner: K at G index %d is %g, not a finite factor >= 0",
+                        c->gidx_h[p], f);
+         return -1;
+      }
+      kp[p] = f;
+   }
+   /* the buffer is allocated once, so captured calls keep reading it */
+   if (!c->Kp) TRY(dalloc(&c->Kp, c->ngp));
+   CPSD_CHECK(hipMemcpyAsync(c->Kp, kp.data(), (size_t)c->ngp * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+   CPSD_CHECK(hipStreamSynchronize(c->stream));
+   c->precond = 1;
+   return 0;
+}
+
+extern "C" int cpsd_dev_set_preconditioner(cpsd_dev *h, const double *K)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   if (h->pending) { cpsd_set_error("cpsd_dev_set_preconditioner: a started call has not been finished"); return -1; }
+   return ctx_set_preconditioner(h, K);
 }
 
 extern "C" int cpsd_dev_set_graphs(cpsd_dev *h, int enable)

@@ -157,8 +157,10 @@ int launch_gram_c2(const double *A, size_t lda, int M, int c2row, const double *
 int launch_reduce_partials_split(const double *partial, int sp_off, int sp_dd, int M, int skip0, int skip1,
                                  double *out, hipStream_t s);
 
-/* EPI_STORE: C2[n][k] = result (the rotation c1 = c2 V of the epilogue) */
-enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2, EPI_STORE = 3 };
+/* EPI_STORE: C2[n][k] = result (the rotation c1 = c2 V of the epilogue);
+ * EPI_PRECOND: C2 = C1 + Kp o (C2 - C1 + dte result), the preconditioned
+ * step with result = C1 h (Kp one factor per complex coefficient) */
+enum { EPI_KBAPPLY = 0, EPI_CONSTRAINT = 1, EPI_NLFORCE = 2, EPI_STORE = 3, EPI_PRECOND = 4 };
 struct TallArgs {
    const double *A;  size_t lda;   /* A rows (m < Mr), contiguous along k */
    const double *X;  int ldx;      /* X[m*xsm + n*ldx] */
@@ -171,6 +173,7 @@ struct TallArgs {
     * w Gp (Z.im beta.re - Z.re beta.im) with beta = Bp[n][G]; one [Nc][3]
     * record per K block in fpart */
    const double2 *Bp; const double *w, *Gp; int ngp; double *fpart;
+   const double *Kp = nullptr;     /* EPI_PRECOND: [K / 2] */
    int ntn, per;                   /* set by the launcher */
 };
 int launch_gemm_tall(int epi, const TallArgs &a, hipStream_t s);

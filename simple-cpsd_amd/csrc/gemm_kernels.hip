@@ -789,6 +789,13 @@ __global__ void __launch_bounds__(512, 4) k_gemm_tall(TallArgs a)
             *reinterpret_cast<double2 *>(a.C2 + idx) = c2;
          } else if (EPI == EPI_STORE) {
             *reinterpret_cast<double2 *>(a.C2 + idx) = v;
+         } else if (EPI == EPI_PRECOND) {
+            const double2 c1 = *reinterpret_cast<const double2 *>(a.C1 + idx);
+            double2 c2 = *reinterpret_cast<const double2 *>(a.C2 + idx);
+            const double f = a.Kp[(k0 >> 1) + lane];
+            c2.x = c1.x + f * (c2.x - c1.x + a.dte * v.x);
+            c2.y = c1.y + f * (c2.y - c1.y + a.dte * v.y);
+            *reinterpret_cast<double2 *>(a.C2 + idx) = c2;
          } else {
             /* lane holds Z_n(G) for G = k0/2 + lane (ccpsdv3/cpsd.c:296-302 with
              * the orbital sum moved inside) */
@@ -817,6 +824,7 @@ int launch_gemm_tall(int epi, const TallArgs &args, hipStream_t s)
    if (epi == EPI_KBAPPLY) hipLaunchKernelGGL((k_gemm_tall<EPI_KBAPPLY, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    else if (epi == EPI_CONSTRAINT) hipLaunchKernelGGL((k_gemm_tall<EPI_CONSTRAINT, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    else if (epi == EPI_STORE) hipLaunchKernelGGL((k_gemm_tall<EPI_STORE, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
+   else if (epi == EPI_PRECOND) hipLaunchKernelGGL((k_gemm_tall<EPI_PRECOND, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    else hipLaunchKernelGGL((k_gemm_tall<EPI_NLFORCE, 64>), dim3((unsigned)blocks), dim3(512), 0, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;

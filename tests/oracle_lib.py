@@ -64,6 +64,7 @@ def lib():
                                   C.POINTER(C.c_double)]
         L.oracle_cpsd.restype = C.c_int
         L.oracle_last_lagrange_iterations.restype = C.c_int
+        L.oracle_set_preconditioner.argtypes = [C.c_void_p, C.c_void_p]
         L.oracle_syev_sorted.argtypes = [C.c_int, dp, dp]
         _lib = L
     return _lib
@@ -199,18 +200,34 @@ class State:
         self.deltac = C.c_double(0.0)
         self.deltar = C.c_double(0.0)
 
-    def step(self, inner, dt=5.8, fmass=400000.0, move=0):
+    def step(self, inner, dt=5.8, fmass=400000.0, move=0, K=None, h=None):
+        """inner SD steps + energy; with K (one factor per half-complex G,
+        [nfft3d]) and h (ne x ne, column-major) the preconditioned step
+        c2 = c1 + K o dte (cpsi + c1 h) of oracle_set_preconditioner"""
         s = self.s
-        rc = lib().oracle_cpsd(inner, move, s.nion, 1, self.ne, s.nfft, s.nfft3d, s.n2ft3d,
+        if K is not None:
+            K = np.ascontiguousarray(K, dtype=np.float64)
+            h = np.ascontiguousarray(h, dtype=np.float64)
+            assert K.size == s.nfft3d and h.size == self.ne[0] ** 2
+            lib().oracle_set_preconditioner(K.ctypes.data, h.ctypes.data)
+        try:
+            rc = self._step(inner, dt, fmass, move)
+        finally:
+            if K is not None:
+                lib().oracle_set_preconditioner(None, None)
+        if rc != 0:
+            raise RuntimeError("oracle_cpsd failed: %d" % rc)
+        return self.E.copy(), self.deltae.value, self.deltac.value
+
+    def _step(self, inner, dt, fmass, move):
+        s = self.s
+        return lib().oracle_cpsd(inner, move, s.nion, 1, self.ne, s.nfft, s.nfft3d, s.n2ft3d,
                                s.unitg, s.omega, dt, fmass, s.katm, self.amass, s.zv, 27,
                                s.rcell, s.cewald, s.rcut, s.masker, s.G, s.tg, s.vc, s.vg,
                                self.lmmax, self.vnlnrm, self.vl, self.vnl, self.r1, self.r2,
                                self.fion, self.c1, self.c2, self.cpsi, self.dn, self.hml,
                                self.lmd, self.work, self.E, C.byref(self.deltae),
                                C.byref(self.deltac), C.byref(self.deltar))
-        if rc != 0:
-            raise RuntimeError("oracle_cpsd failed: %d" % rc)
-        return self.E.copy(), self.deltae.value, self.deltac.value
 
 
 def eigsrt(D, V):

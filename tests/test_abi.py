@@ -9,6 +9,7 @@ import cpsd_amd
 def declared(header):
     txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"^\s*#.*$", "", txt, flags=re.M)      # preprocessor lines (macros)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt)
     skip = {"if", "defined"}
     return sorted(set(n for n in names if n not in skip))
