@@ -11,6 +11,9 @@
  *   cpsdh_makepsi      cmakepsi/dmakepsi.c:86-133 (plane-wave start)
  *   cpsdh_write_elc, cpsdh_read_elc, cpsdh_write_ionout
  *                      the restart files of ccpsdv3/main.c:115-140, 539-570
+ *   cpsdh_write_vpp, cpsdh_read_vpp
+ *                      the .vpp files of ckbppv3/dkbppv3.c:138-150 and
+ *                      ccpsdv3/main.c:266-283
  * and, with no counterpart in the reference, a seeded Gaussian starting
  * state orthonormalised under the reference's G-space inner product
  * (ccpsdv3/gcdotc.c), used for the large workloads.
@@ -39,6 +42,21 @@ int cpsdh_read_psp(const char *path, char *atom, double *zv, double *amass, int 
 int cpsdh_kbpp(double zv, int lmax, int nrho, double drho, const double *rho, const double *vp,
                const double *wp, int nfft3d, const double *G, double *vl, double *vnl,
                double *vnlnrm);
+
+/* .vpp files (written by ckbppv3/dkbppv3.c:138-150, read by
+ * ccpsdv3/main.c:266-283): int icube, int nfft, double unit, char atom[2],
+ * double amass, double zv, int lmax, double rc[lmax+1], double
+ * vnlnrm[lmax^2], double vl[nfft3d], double vnl[lmax^2 nfft3d], native
+ * byte order, no padding; amass unscaled.  The reader takes atom[3] (NUL
+ * terminated), rc[4], vnlnrm[9], vl[cap_nfft3d], vnl[9 cap_nfft3d] and
+ * refuses a file with more or less data than its header implies.
+ * Return 0 on success. */
+int cpsdh_write_vpp(const char *path, int icube, int nfft, double unit, const char *atom, double amass,
+                    double zv, int lmax, const double *rc, const double *vnlnrm, const double *vl,
+                    const double *vnl);
+int cpsdh_read_vpp(const char *path, int *icube, int *nfft, double *unit, char *atom, double *amass,
+                   double *zv, int *lmax, double *rc, double *vnlnrm, double *vl, double *vnl,
+                   long cap_nfft3d);
 
 /* plane-wave orbitals from PSI0.DAT-style rows (i, j, k, l) */
 int cpsdh_makepsi(int nfft, int ne, const int *spec, double *psi);

@@ -142,93 +142,195 @@ int cpsdh_read_psp(const char *path, char *atom, double *zv, double *amass, int 
    return 0;
 }
 
-/* composite Simpson weights as ckbppv3/dkbppv3.c:310-328 applies them:
- * h/3 * (2 sum(even) + 4 sum(odd) - y_0 - y_{n-1}) */
-static double simpson(int n, const double *y, double h)
+/* Weight of point j of an n-point composite Simpson sum over spacing h, as
+ * ckbppv3/dkbppv3.c:310-328 forms it, h/3 (2 sum(even) + 4 sum(odd) - y_0 -
+ * y_{n-1}): interior even points 2, odd points 4, the ends one less. */
+static double simpson_weight(int j, int n, double h)
 {
-   double even = 0.0, odd = 0.0;
-   int i;
-   for (i = 0; i < n; i += 2) even += y[i];
-   for (i = 1; i < n; i += 2) odd += y[i];
-   return (2.0 * even + 4.0 * odd - y[0] - y[n - 1]) * (h / 3.0);
+   double w = (j % 2 == 0) ? 2.0 : 4.0;
+   if (j == 0) w -= 1.0;
+   if (j == n - 1) w -= 1.0;
+   return w * h / 3.0;
 }
 
+struct qkey {
+   double q2;
+   int k;
+};
+
+static int qkey_cmp(const void *a, const void *b)
+{
+   const double x = ((const struct qkey *)a)->q2, y = ((const struct qkey *)b)->q2;
+   return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* The Kleinman-Bylander transform of ckbppv3/dkbppv3.c:335-444.
+ *
+ * With dv_l = V_l - V_lmax for l < lmax (channel lmax is the local part) and
+ * the Simpson sums S[.] over the radial table, the transform of G != 0
+ * (q = |G|, g = G/q, x = q r) is
+ *   s:     vnl_0 = sqrt(4 pi)  S[sin x  wp_0 dv_0] / q
+ *   p:     vnl_p = sqrt(12 pi) S[(sin x/x - cos x) wp_1 dv_1] / q  (g_x, g_y, g_z)
+ *   d:     vnl_d = sqrt(60 pi) S[(3 (sin x/x - cos x)/x - sin x) wp_2 dv_2] / q
+ *          times (3 g_z^2 - 1)/(2 sqrt 3), g_x g_y, g_y g_z, g_z g_x, (g_x^2 - g_y^2)/2
+ *   local: vl = 4 pi S[r sin x dv_lmax] / q - 4 pi zv cos(q r_last) / q^2
+ * (the p and d sums leave out r_0, whose integrand the reference sets to 0).
+ * The four radial sums depend on q only: everything that does not depend on
+ * q (Simpson weight times wp_l dv_l, or times r dv_lmax) is folded into one
+ * table, the G vectors are grouped by |G|^2 (the lattice repeats each |G|
+ * many times), and each distinct q costs one pass over the table with one
+ * sin and cos per point.  G = 0 and the normalisations
+ * vnlnrm_l = S[dv_l wp_l^2] are direct Simpson sums. */
 int cpsdh_kbpp(double zv, int lmax, int nrho, double drho, const double *rho, const double *vp_in,
                const double *wp, int nfft3d, const double *G, double *vl, double *vnl,
                double *vnlnrm)
 {
    const double pi = 4.0 * atan(1.0), fourpi = 4.0 * pi;
-   const double p0 = sqrt(fourpi), p1 = sqrt(3.0 * fourpi), p2 = sqrt(15.0 * fourpi);
+   const double c0 = sqrt(fourpi), c1 = sqrt(3.0 * fourpi), c2 = sqrt(15.0 * fourpi);
    const int lmmax = lmax * lmax;
-   double *dv = (double *)malloc((size_t)(lmax + 1) * nrho * sizeof(double));
-   double *f = (double *)malloc(nrho * sizeof(double));
-   int l, i, k;
-   if (!dv || !f) { free(dv); free(f); return 1; }
-   /* channel l < lmax carries V_l - V_lmax; channel lmax is the local part */
-   for (l = 0; l <= lmax; ++l)
-      for (i = 0; i < nrho; ++i)
-         dv[i + l * nrho] = (l < lmax) ? vp_in[i + l * nrho] - vp_in[i + lmax * nrho] : vp_in[i + l * nrho];
+   double *tab = (double *)malloc(4 * (size_t)nrho * sizeof(double));   /* s, p, d, local */
+   struct qkey *keys = (struct qkey *)malloc((size_t)(nfft3d > 1 ? nfft3d - 1 : 1) * sizeof(struct qkey));
+   int *gstart = (int *)malloc((size_t)nfft3d * sizeof(int));
+   int j, l, ngroups = 0;
+   if (!tab || !keys || !gstart || lmax < 0 || lmax > 3 || nrho < 2) {
+      free(tab); free(keys); free(gstart);
+      return 1;
+   }
+   for (j = 0; j < nrho; ++j) {
+      const double w = simpson_weight(j, nrho, drho);
+      const double vloc = vp_in[j + (size_t)lmax * nrho];
+      for (l = 0; l < 3; ++l)
+         tab[(size_t)l * nrho + j] = (l < lmax && (l == 0 || j > 0))
+                                        ? w * wp[j + (size_t)l * nrho] * (vp_in[j + (size_t)l * nrho] - vloc)
+                                        : 0.0;
+      tab[3 * (size_t)nrho + j] = w * rho[j] * vloc;
+   }
+   for (l = 0; l < 9; ++l) vnlnrm[l] = 0.0;
    for (l = 0; l < lmax; ++l) {
-      double a;
-      for (i = 0; i < nrho; ++i) f[i] = dv[i + l * nrho] * wp[i + l * nrho] * wp[i + l * nrho];
-      a = simpson(nrho, f, drho);
-      for (i = l * l; i < (l + 1) * (l + 1); ++i) vnlnrm[i] = a;
-   }
-   for (i = lmmax; i < 9; ++i) vnlnrm[i] = 0.0;
-
-#pragma omp parallel
-   {
-      double *fl = (double *)malloc(nrho * sizeof(double));
-      double *cs = (double *)malloc(nrho * sizeof(double));
-      double *sn = (double *)malloc(nrho * sizeof(double));
-#pragma omp for schedule(dynamic, 256)
-      for (k = 1; k < nfft3d; ++k) {
-         double gx = G[k], gy = G[k + nfft3d], gz = G[k + 2 * nfft3d];
-         const double q = sqrt(gx * gx + gy * gy + gz * gz);
-         int j;
-         gx /= q; gy /= q; gz /= q;
-         for (j = 0; j < nrho; ++j) { cs[j] = cos(q * rho[j]); sn[j] = sin(q * rho[j]); }
-         if (lmax > 2) {
-            double d;
-            fl[0] = 0.0;
-            for (j = 1; j < nrho; ++j) {
-               const double t = 3.0 * (sn[j] / (q * rho[j]) - cs[j]) / (q * rho[j]) - sn[j];
-               fl[j] = t * wp[j + 2 * nrho] * dv[j + 2 * nrho];
-            }
-            d = p2 * simpson(nrho, fl, drho) / q;
-            vnl[k + 4 * (size_t)nfft3d] = d * (3.0 * gz * gz - 1.0) / (2.0 * sqrt(3.0));
-            vnl[k + 5 * (size_t)nfft3d] = d * gx * gy;
-            vnl[k + 6 * (size_t)nfft3d] = d * gy * gz;
-            vnl[k + 7 * (size_t)nfft3d] = d * gz * gx;
-            vnl[k + 8 * (size_t)nfft3d] = d * (gx * gx - gy * gy) / 2.0;
-         }
-         if (lmax > 1) {
-            double p;
-            fl[0] = 0.0;
-            for (j = 1; j < nrho; ++j)
-               fl[j] = (sn[j] / (q * rho[j]) - cs[j]) * wp[j + nrho] * dv[j + nrho];
-            p = p1 * simpson(nrho, fl, drho) / q;
-            vnl[k + 1 * (size_t)nfft3d] = p * gx;
-            vnl[k + 2 * (size_t)nfft3d] = p * gy;
-            vnl[k + 3 * (size_t)nfft3d] = p * gz;
-         }
-         if (lmax > 0) {
-            for (j = 0; j < nrho; ++j) fl[j] = sn[j] * wp[j] * dv[j];
-            vnl[k] = p0 * simpson(nrho, fl, drho) / q;
-         }
-         for (j = 0; j < nrho; ++j) fl[j] = rho[j] * dv[j + lmax * nrho] * sn[j];
-         vl[k] = simpson(nrho, fl, drho) * fourpi / q - zv * fourpi * cs[nrho - 1] / (q * q);
+      double a = 0.0;
+      for (j = 0; j < nrho; ++j) {
+         const double dvl = vp_in[j + (size_t)l * nrho] - vp_in[j + (size_t)lmax * nrho];
+         a += simpson_weight(j, nrho, drho) * dvl * wp[j + (size_t)l * nrho] * wp[j + (size_t)l * nrho];
       }
-      free(fl); free(cs); free(sn);
+      for (j = l * l; j < (l + 1) * (l + 1); ++j) vnlnrm[j] = a;
    }
-   for (i = 0; i < nrho; ++i) f[i] = dv[i + lmax * nrho] * (rho[i] * rho[i]);
-   vl[0] = fourpi * simpson(nrho, f, drho) + 2.0 * pi * zv * rho[nrho - 1] * rho[nrho - 1];
-   for (i = 0; i < nrho; ++i) f[i] = rho[i] * wp[i] * dv[i];
-   vnl[0] = p0 * simpson(nrho, f, drho);
-   for (l = 1; l < lmmax; ++l) vnl[(size_t)l * nfft3d] = 0.0;
-   free(dv);
-   free(f);
+
+   /* G != 0 grouped by |G|^2 (equal to rounding) */
+   for (j = 1; j < nfft3d; ++j) {
+      const double gx = G[j], gy = G[j + nfft3d], gz = G[j + 2 * (size_t)nfft3d];
+      keys[j - 1].q2 = gx * gx + gy * gy + gz * gz;
+      keys[j - 1].k = j;
+   }
+   qsort(keys, (size_t)(nfft3d - 1), sizeof(struct qkey), qkey_cmp);
+   for (j = 0; j < nfft3d - 1; ++j)
+      if (j == 0 || keys[j].q2 > keys[j - 1].q2 * (1.0 + 1e-14)) gstart[ngroups++] = j;
+   gstart[ngroups] = nfft3d - 1;
+
+#pragma omp parallel for schedule(dynamic, 16)
+   for (int gi = 0; gi < ngroups; ++gi) {
+      const double q = sqrt(keys[gstart[gi]].q2);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, sl = 0.0;
+      for (int jj = 0; jj < nrho; ++jj) {
+         const double x = q * rho[jj], sn = sin(x);
+         s0 += sn * tab[jj];
+         sl += sn * tab[3 * (size_t)nrho + jj];
+         if (jj > 0 && lmax > 1) {
+            const double b1 = sn / x - cos(x);
+            s1 += b1 * tab[(size_t)nrho + jj];
+            s2 += (3.0 * b1 / x - sn) * tab[2 * (size_t)nrho + jj];
+         }
+      }
+      const double vs = c0 * s0 / q, vp = c1 * s1 / q, vd = c2 * s2 / q;
+      const double vloc = fourpi * sl / q - zv * fourpi * cos(q * rho[nrho - 1]) / (q * q);
+      for (int e = gstart[gi]; e < gstart[gi + 1]; ++e) {
+         const int k = keys[e].k;
+         const double qk = sqrt(keys[e].q2);
+         const double gx = G[k] / qk, gy = G[k + nfft3d] / qk, gz = G[k + 2 * (size_t)nfft3d] / qk;
+         vl[k] = vloc;
+         if (lmax > 0) vnl[k] = vs;
+         if (lmax > 1) {
+            vnl[k + 1 * (size_t)nfft3d] = vp * gx;
+            vnl[k + 2 * (size_t)nfft3d] = vp * gy;
+            vnl[k + 3 * (size_t)nfft3d] = vp * gz;
+         }
+         if (lmax > 2) {
+            vnl[k + 4 * (size_t)nfft3d] = vd * (3.0 * gz * gz - 1.0) / (2.0 * sqrt(3.0));
+            vnl[k + 5 * (size_t)nfft3d] = vd * gx * gy;
+            vnl[k + 6 * (size_t)nfft3d] = vd * gy * gz;
+            vnl[k + 7 * (size_t)nfft3d] = vd * gz * gx;
+            vnl[k + 8 * (size_t)nfft3d] = vd * (gx * gx - gy * gy) / 2.0;
+         }
+      }
+   }
+
+   /* G = 0: vl = 4 pi S[r^2 dv_lmax] + 2 pi zv r_last^2, vnl_s = sqrt(4 pi) S[r wp_0 dv_0] */
+   {
+      double a = 0.0, b = 0.0;
+      for (j = 0; j < nrho; ++j) {
+         a += tab[3 * (size_t)nrho + j] * rho[j];
+         b += tab[j] * rho[j];
+      }
+      vl[0] = fourpi * a + 2.0 * pi * zv * rho[nrho - 1] * rho[nrho - 1];
+      if (lmax > 0) vnl[0] = c0 * b;
+      for (l = 1; l < lmmax; ++l) vnl[(size_t)l * nfft3d] = 0.0;
+   }
+   free(tab);
+   free(keys);
+   free(gstart);
    return 0;
+}
+
+/* .vpp files, the output of ckbppv3/dkbppv3.c:138-150 and the input of
+ * ccpsdv3/main.c:266-283: int icube, int nfft, double unit, char atom[2],
+ * double amass, double zv, int lmax, double rc[lmax+1],
+ * double vnlnrm[lmax^2], double vl[nfft3d], double vnl[lmax^2 nfft3d],
+ * native byte order, no padding (amass is the table's value; the
+ * reference's reader scales it by 1822.89 afterwards). */
+int cpsdh_write_vpp(const char *path, int icube, int nfft, double unit, const char *atom, double amass, double zv,
+                    int lmax, const double *rc, const double *vnlnrm, const double *vl, const double *vnl)
+{
+   const size_t nfft3d = (size_t)(nfft / 2 + 1) * nfft * nfft, lmmax = (size_t)lmax * lmax;
+   char a2[2] = {0, 0};
+   FILE *f;
+   int ok;
+   if (lmax < 0 || lmax > 3 || nfft < 2) return 1;
+   a2[0] = atom[0];
+   if (atom[0]) a2[1] = atom[1];
+   f = fopen(path, "wb");
+   if (!f) return 2;
+   ok = fwrite(&icube, sizeof(int), 1, f) == 1 && fwrite(&nfft, sizeof(int), 1, f) == 1 &&
+        fwrite(&unit, sizeof(double), 1, f) == 1 && fwrite(a2, 1, 2, f) == 2 &&
+        fwrite(&amass, sizeof(double), 1, f) == 1 && fwrite(&zv, sizeof(double), 1, f) == 1 &&
+        fwrite(&lmax, sizeof(int), 1, f) == 1 && fwrite(rc, sizeof(double), (size_t)lmax + 1, f) == (size_t)lmax + 1 &&
+        fwrite(vnlnrm, sizeof(double), lmmax, f) == lmmax && fwrite(vl, sizeof(double), nfft3d, f) == nfft3d &&
+        fwrite(vnl, sizeof(double), lmmax * nfft3d, f) == lmmax * nfft3d;
+   if (fclose(f) != 0) ok = 0;
+   return ok ? 0 : 3;
+}
+
+int cpsdh_read_vpp(const char *path, int *icube, int *nfft, double *unit, char *atom, double *amass, double *zv,
+                   int *lmax, double *rc, double *vnlnrm, double *vl, double *vnl, long cap_nfft3d)
+{
+   FILE *f = fopen(path, "rb");
+   size_t nfft3d, lmmax;
+   int ok;
+   if (!f) return 2;
+   atom[2] = 0;
+   ok = fread(icube, sizeof(int), 1, f) == 1 && fread(nfft, sizeof(int), 1, f) == 1 &&
+        fread(unit, sizeof(double), 1, f) == 1 && fread(atom, 1, 2, f) == 2 &&
+        fread(amass, sizeof(double), 1, f) == 1 && fread(zv, sizeof(double), 1, f) == 1 &&
+        fread(lmax, sizeof(int), 1, f) == 1;
+   if (!ok || *lmax < 0 || *lmax > 3 || *nfft < 2 || *nfft > 4096) { fclose(f); return 3; }
+   nfft3d = (size_t)(*nfft / 2 + 1) * *nfft * *nfft;
+   lmmax = (size_t)*lmax * *lmax;
+   if ((long)nfft3d > cap_nfft3d) { fclose(f); return 4; }
+   ok = fread(rc, sizeof(double), (size_t)*lmax + 1, f) == (size_t)*lmax + 1 &&
+        fread(vnlnrm, sizeof(double), lmmax, f) == lmmax && fread(vl, sizeof(double), nfft3d, f) == nfft3d &&
+        fread(vnl, sizeof(double), lmmax * nfft3d, f) == lmmax * nfft3d;
+   /* nothing may follow */
+   if (ok && fgetc(f) != EOF) ok = 0;
+   fclose(f);
+   return ok ? 0 : 3;
 }
 
 int cpsdh_makepsi(int nfft, int ne, const int *spec, double *psi)

@@ -44,12 +44,35 @@ def test_device_kb_matches_host(gpu, name, lmax, nfft, L):
     m = max(1, lmax * lmax) * n
     sc_n = max(1e-300, np.abs(hn[:m]).max())
     np.testing.assert_allclose(dn[:m], hn[:m], rtol=0, atol=1e-12 * sc_n)
-    np.testing.assert_array_equal(dr, hr)
+    np.testing.assert_allclose(dr, hr, rtol=1e-13, atol=0)
     # the oracle's restatement of the reference transform
     s = O.Setup(1, L, nfft, np.array([0], dtype=np.int32), np.array([p["zv"]]))
     ol, on, orr = O.kbpp(O.read_psp(cpsd_amd.os.path.join(cpsd_amd.DATA, "psp", name)), lmax, s)
     np.testing.assert_allclose(dl, ol, rtol=0, atol=1e-12 * sc_l)
     np.testing.assert_allclose(dn[:m], on[:m], rtol=0, atol=1e-12 * sc_n)
+
+
+def test_device_kb_closed_forms(gpu):
+    """every channel (lmax 3) of the device transform against the closed-form
+    Hankel transforms of Gaussian radial functions (the host test's table)"""
+    from test_host_inputs import gaussian_table, kb_closed_forms
+    say("kbpp closed forms")
+    r, vp, wp, drho, b, a, beta = gaussian_table()
+    s = O.Setup(1, 20.0, 16, np.array([0], dtype=np.int32), np.array([1.0]))
+    vl, vnl, nrm = np.zeros(s.nfft3d), np.zeros(9 * s.nfft3d), np.zeros(9)
+    assert cpsd_amd.hip_lib().cpsd_kbpp_device(0.0, 3, r.size, drho, r, vp, wp, s.nfft3d, s.G, vl, vnl, nrm) == 0
+    G = s.G.reshape(3, -1)[:, 1:]
+    q = np.sqrt((G * G).sum(0))
+    gx, gy, gz = G / q
+    cs, cp, cd, cl = kb_closed_forms(q, a, b, 0.0, r[-1])
+    V = vnl.reshape(9, s.nfft3d)[:, 1:]
+    assert np.max(np.abs(vl[1:] - cl)) < 1e-9 * np.max(np.abs(cl))
+    assert np.max(np.abs(V[0] - cs)) < 1e-9 * np.max(np.abs(cs))
+    for row, ang in ((1, gx), (2, gy), (3, gz)):
+        assert np.max(np.abs(V[row] - cp * ang)) < 1e-9 * np.max(np.abs(cp))
+    for row, ang in ((4, (3 * gz * gz - 1) / (2 * np.sqrt(3))), (5, gx * gy), (6, gy * gz), (7, gz * gx),
+                     (8, (gx * gx - gy * gy) / 2)):
+        assert np.max(np.abs(V[row] - cd * ang)) < 1e-9 * np.max(np.abs(cd))
 
 
 def test_device_kb_water_cell_runs_the_same(gpu):

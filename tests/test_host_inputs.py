@@ -46,8 +46,11 @@ def test_psp_reader_accepts_fourth_header_token():
     assert (c["zv"], c["lmax"], c["nrho"]) == (4.0, 2, 501)
 
 
-@pytest.mark.parametrize("name,lmax", [("C01", 2), ("O00", 2), ("H00", 1), ("O00", 1)])
+@pytest.mark.parametrize("name,lmax", [("C01", 2), ("O00", 2), ("H00", 1), ("O00", 1), ("Fe00", 3)])
 def test_kbpp_matches_oracle(name, lmax):
+    """the library's transform (grouped by |G|, Simpson weights folded into
+    one table) against the oracle's statement-by-statement restatement: the
+    two sum in different orders, so they agree to rounding"""
     s = O.Setup(1, 20.0, 24, [0], [1.0])
     path = os.path.join(O.DATA, "psp", name + ".dat")
     q = O.read_psp(path)
@@ -57,9 +60,10 @@ def test_kbpp_matches_oracle(name, lmax):
     assert cpsd_amd.host_lib().cpsdh_kbpp(p["zv"], lmax, p["nrho"], p["drho"], p["rho"], p["vp"],
                                           p["wp"], s.nfft3d, s.G, vl, vnl, nrm) == 0
     lm = lmax * lmax
-    assert np.allclose(vl, vl0, rtol=1e-14, atol=1e-14)
-    assert np.allclose(vnl[: lm * s.nfft3d], vnl0[: lm * s.nfft3d], rtol=1e-14, atol=1e-14)
-    assert np.allclose(nrm[:lm], n0[:lm], rtol=1e-14)
+    assert np.max(np.abs(vl - vl0)) < 1e-13 * np.max(np.abs(vl0))
+    a, b = vnl[: lm * s.nfft3d], vnl0[: lm * s.nfft3d]
+    assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(b))
+    assert np.allclose(nrm[:lm], n0[:lm], rtol=1e-13)
 
 
 def test_makepsi_matches_oracle():
@@ -90,3 +94,123 @@ def test_random_orbitals_orthonormal_hermitian_masked():
             assert a[0] == b[0] and a[1] == -b[1]
     same = np.array_equal(psi, cell.random_orbitals(ne, seed=11))
     assert same
+
+
+def gaussian_table(nrho=2001, drho=0.01, b=0.7, c=0.9):
+    """a radial table with closed-form Kleinman-Bylander transforms:
+    local channel V_3 = -exp(-b r^2), channels l < 3 with V_l - V_3 =
+    exp(-c r^2) and wp_l = r^(l+1) exp(-r^2/2), so wp_l dv_l = r^(l+1)
+    exp(-a r^2), a = c + 1/2"""
+    r = np.arange(nrho) * drho
+    vp = np.zeros(4 * nrho)
+    wp = np.zeros(4 * nrho)
+    vloc = -np.exp(-b * r * r)
+    for l in range(4):
+        vp[l * nrho:(l + 1) * nrho] = vloc + (np.exp(-c * r * r) if l < 3 else 0.0)
+        wp[l * nrho:(l + 1) * nrho] = r ** (l + 1) * np.exp(-r * r / 2)
+    return r, vp, wp, drho, b, c + 0.5, c + 1.0
+
+
+def kb_closed_forms(q, a, b, zv, rlast):
+    """Hankel transforms of the Gaussian table: int r^(l+2) j_l(q r)
+    exp(-a r^2) dr = sqrt(pi) q^l / (2^(l+2) a^(l+3/2)) exp(-q^2/(4a))"""
+    e = np.exp(-q * q / (4 * a))
+    s = np.sqrt(4 * np.pi) * np.sqrt(np.pi) / (4 * a ** 1.5) * e
+    p = np.sqrt(12 * np.pi) * np.sqrt(np.pi) * q / (8 * a ** 2.5) * e
+    d = np.sqrt(60 * np.pi) * np.sqrt(np.pi) * q * q / (16 * a ** 3.5) * e
+    loc = -np.pi ** 1.5 / b ** 1.5 * np.exp(-q * q / (4 * b)) - zv * 4 * np.pi * np.cos(q * rlast) / (q * q)
+    return s, p, d, loc
+
+
+def test_kbpp_closed_forms_all_channels():
+    """s, p, d and local channels (lmax 3) of the host transform against
+    the closed-form Hankel transforms of Gaussian radial functions, every
+    G != 0 of a 16^3 grid, with the angular factors; the normalisations
+    against int r^(2l+2) exp(-(c+1) r^2) dr and G = 0 against its limit"""
+    r, vp, wp, drho, b, a, beta = gaussian_table()
+    nrho = r.size
+    zv = 0.0
+    s = O.Setup(1, 20.0, 16, [0], [1.0])
+    vl, vnl, nrm = np.zeros(s.nfft3d), np.zeros(9 * s.nfft3d), np.zeros(9)
+    assert cpsd_amd.host_lib().cpsdh_kbpp(zv, 3, nrho, drho, r, vp, wp, s.nfft3d, s.G, vl, vnl, nrm) == 0
+    G = s.G.reshape(3, -1)[:, 1:]
+    q = np.sqrt((G * G).sum(0))
+    gx, gy, gz = G / q
+    cs, cp, cd, cl = kb_closed_forms(q, a, b, zv, r[-1])
+    n = s.nfft3d
+    V = vnl.reshape(9, n)[:, 1:]
+    tol = 1e-9
+    assert np.max(np.abs(vl[1:] - cl)) < tol * np.max(np.abs(cl))
+    assert np.max(np.abs(V[0] - cs)) < tol * np.max(np.abs(cs))
+    for row, ang in ((1, gx), (2, gy), (3, gz)):
+        assert np.max(np.abs(V[row] - cp * ang)) < tol * np.max(np.abs(cp))
+    for row, ang in ((4, (3 * gz * gz - 1) / (2 * np.sqrt(3))), (5, gx * gy), (6, gy * gz), (7, gz * gx),
+                     (8, (gx * gx - gy * gy) / 2)):
+        assert np.max(np.abs(V[row] - cd * ang)) < tol * np.max(np.abs(cd))
+    # <w_l|dv_l|w_l> = int r^(2l+2) exp(-(c+1) r^2) dr = (2l+1)!! sqrt(pi) / (2^(l+2) beta^(l+3/2))
+    for l in range(3):
+        dfact = np.prod(np.arange(2 * l + 1, 0, -2))
+        exact = dfact * np.sqrt(np.pi) / (2 ** (l + 2) * beta ** (l + 1.5))
+        assert np.allclose(nrm[l * l:(l + 1) * (l + 1)], exact, rtol=1e-9)
+    # G = 0: vl = 4 pi int r^2 V_3 dr, vnl_s = sqrt(4 pi) int r^2 exp(-a r^2) dr
+    assert abs(vl[0] - (-np.pi ** 1.5 / b ** 1.5)) < 1e-9 * np.pi ** 1.5 / b ** 1.5
+    assert abs(vnl[0] - np.sqrt(4 * np.pi) * np.sqrt(np.pi) / (4 * a ** 1.5)) < 1e-9
+    assert np.all(vnl.reshape(9, n)[1:, 0] == 0.0)
+
+
+def test_kbpp_closed_forms_separate_the_channels():
+    """the p and d closed forms differ by far more than the pin's
+    tolerance, so a transform that swapped the two radial factors (or
+    their normalisations) would fail test_kbpp_closed_forms_all_channels"""
+    r, vp, wp, drho, b, a, beta = gaussian_table()
+    s = O.Setup(1, 20.0, 16, [0], [1.0])
+    G = s.G.reshape(3, -1)[:, 1:]
+    q = np.sqrt((G * G).sum(0))
+    cs, cp, cd, cl = kb_closed_forms(q, a, b, 0.0, r[-1])
+    assert np.max(np.abs(cd - cp)) > 1e-3 * np.max(np.abs(cp))
+
+
+def test_vpp_write_read_byte_layout(tmp_path):
+    """cpsdh_write_vpp writes the fields of ckbppv3/dkbppv3.c:138-150 in
+    order with no padding; cpsdh_read_vpp reads them back and refuses a
+    truncated file"""
+    import ctypes as C
+    import struct
+    H = cpsd_amd.host_lib()
+    nfft, lmax = 8, 2
+    n3 = (nfft // 2 + 1) * nfft * nfft
+    rng = np.random.default_rng(3)
+    rc = np.array([0.5, 0.6, 0.7])
+    nrm = rng.standard_normal(lmax * lmax)
+    vl = rng.standard_normal(n3)
+    vnl = rng.standard_normal(lmax * lmax * n3)
+    path = str(tmp_path / "O00.vpp")
+    dp = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    assert H.cpsdh_write_vpp(path.encode(), 1, nfft, C.c_double(20.0), b"O", C.c_double(16.0), C.c_double(6.0),
+                             lmax, dp(rc), dp(nrm), dp(vl), dp(vnl)) == 0
+    raw = open(path, "rb").read()
+    head = struct.calcsize("=iid2sddi")
+    assert len(raw) == head + 8 * (lmax + 1 + lmax * lmax + n3 + lmax * lmax * n3)
+    icube, nf, unit, atom, amass, zv, lm = struct.unpack_from("=iid2sddi", raw, 0)
+    assert (icube, nf, unit, atom, amass, zv, lm) == (1, nfft, 20.0, b"O\x00", 16.0, 6.0, lmax)
+    body = np.frombuffer(raw[head:], dtype=np.float64)
+    o = 0
+    assert np.array_equal(body[o:o + lmax + 1], rc); o += lmax + 1
+    assert np.array_equal(body[o:o + 4], nrm); o += 4
+    assert np.array_equal(body[o:o + n3], vl); o += n3
+    assert np.array_equal(body[o:], vnl)
+    # read back
+    ic, nf2, lm2 = C.c_int(), C.c_int(), C.c_int()
+    un, am, z = C.c_double(), C.c_double(), C.c_double()
+    at = C.create_string_buffer(3)
+    rcb = np.zeros(4); nb = np.zeros(9); vlb = np.zeros(n3); vnb = np.zeros(9 * n3)
+    rc_p, nb_p, vl_p, vn_p = (a.ctypes.data_as(C.POINTER(C.c_double)) for a in (rcb, nb, vlb, vnb))
+    assert H.cpsdh_read_vpp(path.encode(), C.byref(ic), C.byref(nf2), C.byref(un), at, C.byref(am), C.byref(z),
+                            C.byref(lm2), rc_p, nb_p, vl_p, vn_p, C.c_long(n3)) == 0
+    assert (ic.value, nf2.value, un.value, at.value, am.value, z.value, lm2.value) == (1, nfft, 20.0, b"O", 16.0,
+                                                                                       6.0, lmax)
+    assert np.array_equal(rcb[:3], rc) and np.array_equal(nb[:4], nrm)
+    assert np.array_equal(vlb, vl) and np.array_equal(vnb[:4 * n3], vnl)
+    open(path, "wb").write(raw[:-8])
+    assert H.cpsdh_read_vpp(path.encode(), C.byref(ic), C.byref(nf2), C.byref(un), at, C.byref(am), C.byref(z),
+                            C.byref(lm2), rc_p, nb_p, vl_p, vn_p, C.c_long(n3)) != 0
