@@ -1,28 +1,34 @@
 #!/usr/bin/env python3
 """SD-step benchmark on one MI355X.
 
-Workload (default "h2o-32"): 32 water molecules (96 atoms) in an 18.6 bohr
-simple-cubic cell, 96^3 FFT grid, 128 doubly occupied orbitals, O s+p / H s
-Kleinman-Bylander projectors (192), LDA, dt = 5.8, fmass = 400000, energy
-evaluated every inner = 10 steps, FP64 throughout.  Positions: seeded 4x4x2
-lattice with +-0.3 bohr jitter and random orientations; orbitals: seeded
-complex Gaussian coefficients on the masked G vectors, orthonormalised.
+Workload (default "h2o-64", the cell the metric's target is quoted on): 64
+water molecules (192 atoms) in a 23.5 bohr simple-cubic cell, 128^3 FFT
+grid, 256 doubly occupied orbitals, O s+p / H s Kleinman-Bylander
+projectors (384), LDA, dt = 5.8, fmass = 400000, energy evaluated every
+inner = 10 steps.  Positions: seeded 4x4x4 lattice with +-0.3 bohr jitter
+and random orientations; orbitals: seeded complex Gaussian coefficients on
+the masked G vectors, orthonormalised.  Synthetic inputs of the configured
+shape (no downloaded data).
 
-A "step" is one electronic SD step (ccpsdv3/cpsd.c:169-491) of all orbitals;
-the timed region runs exactly --steps of them, resident in HBM, in calls of
-cpsd_dev_run(inner <= 10), each ending with the energy tail, with no
-instrumentation.  Per-phase GPU times come from HIP events recorded on the
-library's stream around each phase during a second pass of min(--steps, 100)
-steps right after it (the events cost ~2% of a step, so they stay out of
-`value`).  A run from the same start measures the time until
-|E(k) - E(k-1)| < 1e-7 Ha between energy evaluations (10 steps apart).
+A "step" is one electronic SD step (ccpsdv3/cpsd.c:169-491) of all orbitals.
+`value` is the FP64 rate: exactly --steps steps in calls of cpsd_dev_run
+(inner <= 10, each ending with the energy tail), orbitals resident in HBM, no
+instrumentation.  Per-phase GPU times come from HIP events on the library's
+stream during a second pass of min(--steps, 100) steps.
 
-The "mixed" object repeats the steps/s, phase and time-to-converge legs in
-the FP32-transform mode (cpsd_dev_set_precision(h, 1): the orbital FFTs and
-KB products of 9 of every 10 steps on FP32 values, everything else and every
-energy evaluation FP64) and reports its converged energy's error against the
-FP64 run's, per atom; "mixed.bf16_projection" does the same for mode 2 (the
-KB projection with BF16 inputs).  `value` stays the FP64 rate.
+The metric's comparison, time to |E(k) - E(k-1)| < 1e-7 Ha between energy
+evaluations from the same start:
+  time_to_converge.fp64          the reference's FP64 step (plain SD);
+  time_to_converge.mixed_learned precision mode 1 (FP32 orbital FFTs and KB
+                                 products in 9 of 10 steps, everything else
+                                 and every energy FP64) with the learned
+                                 per-G preconditioner (data/precond_mlp.json),
+                                 its converged energy's error per atom
+                                 against the FP64 run's, and the speedup.
+--all-legs adds FP64 + learned, mode 1 alone and mode 2 (BF16 KB projection).
+The drop-in leg times two cpsd() calls on host arrays (the reference's
+host-pointer contract, PCIe copies included); the second reuses the device
+context.
 """
 import argparse
 import json
@@ -40,6 +46,9 @@ WORKLOADS = {
     "h2o-32": dict(nmol=32, L=18.6, nfft=96, ne=128),
     "h2o-64": dict(nmol=64, L=23.5, nfft=128, ne=256),
 }
+# the reference binary timed during the survey, same host type
+REFERENCE_CPU_S_PER_STEP = {"h2o-32": (113.0, "measured, 1 core Xeon 2.7 GHz"),
+                            "h2o-64": (1080.0, "extrapolated from h2o-32 by the dense-flop ratio 9.5")}
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
@@ -92,9 +101,26 @@ def algorithmic(phase, w, ng, nproj, ndnblk):
     return None, None
 
 
+def step_floor_ms(w, ng, nproj):
+    """the per-step roofline floor of the plain FP64 step (SURVEY.md
+    section 8(d) formulas): FFT traffic (2ne+2) 2 n2 8 B and elementwise
+    traffic 6 ne n2 8 B at 8 TB/s, contractions 8 nproj ne NG + 8 ne^2 NG +
+    4 ne^2 NG flop at 78.6 TFLOP/s"""
+    n, ne = w["nfft"], w["ne"]
+    n2 = (n + 2) * n * n
+    fft_b = (2 * ne + 2) * 2 * n2 * 8
+    elem_b = 6 * ne * n2 * 8
+    flop = 8.0 * nproj * ne * ng + 8.0 * ne * ne * ng + 4.0 * ne * ne * ng
+    return dict(fft_ms=fft_b / HBM_PEAK_GBS / 1e6, elementwise_ms=elem_b / HBM_PEAK_GBS / 1e6,
+                contractions_ms=flop / FP64_MFMA_PEAK_TFLOPS / 1e9,
+                total_ms=fft_b / HBM_PEAK_GBS / 1e6 + elem_b / HBM_PEAK_GBS / 1e6 + flop / FP64_MFMA_PEAK_TFLOPS / 1e9)
+
+
 def run(args):
     import cpsd_amd
+    import learned
     w = WORKLOADS[args.workload]
+    natoms = 3 * w["nmol"]
     t0 = time.perf_counter()
     cell = cpsd_amd.water_cell(w["nmol"], w["L"], w["nfft"], seed=args.seed)
     psi = cell.random_orbitals(w["ne"], seed=args.seed + 1, decay=0.0)
@@ -115,9 +141,6 @@ def run(args):
         return E, de, dc
 
     steps(args.warmup)
-    # the timed steps run without instrumentation; the per-phase times come
-    # from a second, profiled pass of the same steps (HIP events on the
-    # library's stream between the phases)
     t0 = time.perf_counter()
     E, de, dc = steps(args.steps)
     elapsed = time.perf_counter() - t0
@@ -130,22 +153,27 @@ def run(args):
     prof = dev.profile_read()
     dev.profile(False)
 
-    # the run epilogue once (orthonormality check, eigenpairs of hml, c1 = c2 V;
-    # outside the timed steps)
+    # the run epilogue once (orthonormality check, eigenpairs of hml, c1 = c2 V)
     t0 = time.perf_counter()
     ep = dev.epilogue()
     epilogue = dict(seconds=round(time.perf_counter() - t0, 4),
                     ortho_max_dev=float(np.max(np.abs(ep["ortho"] - np.eye(w["ne"])))),
                     eig_max=float(ep["eig"][0]), eig_min=float(ep["eig"][-1]))
 
-    # time to |dE| < 1e-7 Ha between energy evaluations, from the same start
-    def converge(mode):
+    model = learned.load_model() if args.converge_steps > 0 else None
+
+    def configure(mode, learned_on):
         dev.set_precision(mode)
+        dev.set_preconditioner(None)
+        if learned_on:
+            dev.set_learned_preconditioner(model["params"], model["alpha"], model["kmin"], model["kmax"])
+
+    def converge(mode, learned_on, tag):
+        configure(mode, learned_on)
         dev.set_orbitals(psi)
         dev.E[:] = 0.0
         t0 = time.perf_counter()
-        done, nsteps, last = False, 0, None
-        Eprev = None
+        done, nsteps, last, Eprev = False, 0, None, None
         while nsteps < args.converge_steps:
             Ec, dec, dcc = dev.run(inner)
             nsteps += inner
@@ -153,69 +181,70 @@ def run(args):
                 done = True
                 break
             if nsteps % 1000 == 0 and Eprev is not None:
-                print("converge(%s): %d steps, E %.10f, dE %.3e" %
-                      (("fp64", "mixed", "mixed-bf16")[mode], nsteps, Ec[0], Ec[0] - Eprev),
+                print("converge(%s): %d steps, E %.10f, dE %.3e" % (tag, nsteps, Ec[0], Ec[0] - Eprev),
                       file=sys.stderr, flush=True)
             last = None if Eprev is None else Ec[0] - Eprev
             Eprev = Ec[0]
         tconv = time.perf_counter() - t0
-        dev.set_precision(0)
-        return dict(converged=done, steps=nsteps, seconds=round(tconv, 4),
-                    E=float(Ec[0]), last_dE=None if last is None else float(last))
+        fb = dev.fallback_info()
+        configure(0, False)
+        return dict(converged=done, steps=nsteps, seconds=round(tconv, 4), E=float(Ec[0]),
+                    last_dE=None if last is None else float(last), fallbacks=fb["fallbacks"])
 
-    conv = converge(0) if args.converge_steps > 0 else None
-
-    # the FP32-transform mode: steps/s, phases, time to converge and the
-    # converged energy's error against FP64
-    mixed = None
-    if args.mixed:
+    def rate(mode, learned_on):
+        """steps/s of the configured step over --steps steps (after warmup)"""
+        configure(mode, learned_on)
         dev.set_orbitals(psi)
-        dev.set_precision(1)
-        steps(args.warmup)
+        steps(max(args.warmup, 2 * inner))   # the preconditioner starts with the second call
         t0 = time.perf_counter()
-        Em, _, _ = steps(args.steps)
-        el_m = time.perf_counter() - t0
+        steps(args.steps)
+        el = time.perf_counter() - t0
         dev.profile(True)
         steps(prof_steps)
-        prof_m = dev.profile_read()
+        ph = dev.profile_read()
         dev.profile(False)
-        dev.set_precision(0)
-        mixed = dict(value=round(args.steps / el_m, 3), unit="SD steps/s",
-                     ms_per_step=round(el_m * 1000.0 / args.steps, 4),
-                     speedup_vs_fp64=round(elapsed / el_m, 4),
-                     fp32_steps_per_call=inner - 1, E_last=float(Em[0]),
-                     # both legs ran warmup + steps steps from the same start
-                     abs_E_diff_vs_fp64_per_atom_at_E_last=abs(float(Em[0]) - float(E[0])) / (3 * w["nmol"]),
-                     phases={k: round(t / prof_steps, 4) for k, (t, c) in prof_m.items() if c > 0})
-        if args.converge_steps > 0:
-            cm = converge(1)
-            if conv is not None and conv["converged"] and cm["converged"]:
-                cm["abs_E_error_vs_fp64_per_atom"] = abs(cm["E"] - conv["E"]) / (3 * w["nmol"])
-                cm["speedup_vs_fp64"] = round(conv["seconds"] / cm["seconds"], 4)
-            mixed["time_to_converge"] = cm
-        # mode 2: the same with a BF16-input KB projection
-        dev.set_orbitals(psi)
-        dev.set_precision(2)
-        steps(args.warmup)
-        t0 = time.perf_counter()
-        Eb, _, _ = steps(args.steps)
-        el_b = time.perf_counter() - t0
-        dev.set_precision(0)
-        mixed["bf16_projection"] = dict(
-            value=round(args.steps / el_b, 3), unit="SD steps/s", ms_per_step=round(el_b * 1000.0 / args.steps, 4),
-            speedup_vs_fp64=round(elapsed / el_b, 4),
-            abs_E_diff_vs_fp64_per_atom_at_E_last=abs(float(Eb[0]) - float(E[0])) / (3 * w["nmol"]))
-        if args.converge_steps > 0:
-            cb = converge(2)
-            if conv is not None and conv["converged"] and cb["converged"]:
-                cb["abs_E_error_vs_fp64_per_atom"] = abs(cb["E"] - conv["E"]) / (3 * w["nmol"])
-                cb["speedup_vs_fp64"] = round(conv["seconds"] / cb["seconds"], 4)
-            mixed["bf16_projection"]["time_to_converge"] = cb
+        configure(0, False)
+        return dict(value=round(args.steps / el, 3), unit="SD steps/s", ms_per_step=round(el * 1000.0 / args.steps, 4),
+                    phases={k: round(t / prof_steps, 4) for k, (t, c) in ph.items() if c > 0})
+
+    conv = {}
+    if args.converge_steps > 0:
+        conv["fp64"] = converge(0, False, "fp64")
+        ml = converge(1, True, "mixed+learned")
+        ml.update(rate(1, True))
+        if conv["fp64"]["converged"] and ml["converged"]:
+            ml["abs_E_error_vs_fp64_per_atom"] = abs(ml["E"] - conv["fp64"]["E"]) / natoms
+            ml["speedup_vs_fp64"] = round(conv["fp64"]["seconds"] / ml["seconds"], 4)
+        conv["mixed_learned"] = ml
+        if args.all_legs:
+            for key, mode, lon in (("fp64_learned", 0, True), ("mixed", 1, False), ("mixed_bf16", 2, False)):
+                r = converge(mode, lon, key)
+                if conv["fp64"]["converged"] and r["converged"]:
+                    r["abs_E_error_vs_fp64_per_atom"] = abs(r["E"] - conv["fp64"]["E"]) / natoms
+                    r["speedup_vs_fp64"] = round(conv["fp64"]["seconds"] / r["seconds"], 4)
+                conv[key] = r
     dev.close()
+
+    # the drop-in cpsd() on host arrays (reference layouts, PCIe both ways)
+    dropin = None
+    if args.dropin:
+        E0 = np.zeros(10)
+        c2 = psi
+        times = []
+        for call in range(2):
+            t0 = time.perf_counter()
+            out = cpsd_amd.cpsd_dropin(cell, w["ne"], inner, c2, E0)
+            times.append(time.perf_counter() - t0)
+            c2, E0 = out["c2"], out["E"]
+            del out
+        dropin = dict(first_call_s=round(times[0], 3), second_call_s=round(times[1], 3),
+                      ms_per_step_incl_pcie=round(times[1] * 1000.0 / inner, 3),
+                      note="second call reuses the device context; each call uploads c2 and copies "
+                           "c1, c2, cpsi, dn, hml, lmd back (%.2f GB)" % (
+                               (4 * w["ne"] * cell.n2ft3d + cell.n2ft3d) * 8 / 1e9))
 
     ms = elapsed * 1000.0 / args.steps
     value = args.steps / elapsed
-    # per-phase GPU time per SD step and the roofline of the dominant phase
     ndnblk = min(8, w["ne"])
     phases = {}
     for name, (tms, calls) in prof.items():
@@ -251,6 +280,10 @@ def run(args):
             sec = phases[name]["ms_per_step"] / 1000.0
             phases[name]["achieved"] = round(amount / sec / (1e12 if kind == "flop" else 1e9), 3)
             phases[name]["unit"] = "TFLOP/s" if kind == "flop" else "GB/s"
+    floor = step_floor_ms(w, sizes["ng"], sizes["nproj"])
+    step_roofline = dict(floor_ms=round(floor["total_ms"], 3), measured_ms=round(ms, 3),
+                         frac=round(floor["total_ms"] / ms, 4),
+                         parts_ms={k: round(v, 3) for k, v in floor.items() if k != "total_ms"})
 
     cpu = None
     if args.cpu_baseline:
@@ -270,12 +303,28 @@ def run(args):
                    seconds_per_step=round(est["total"], 4),
                    parts={k: round(v, 5) for k, v in est.items()},
                    nproc=os.cpu_count())
-        if conv is not None and conv["converged"]:
-            # the CPU would need the same number of steps
-            cpu["time_to_converge_s_extrapolated"] = round(conv["steps"] * est["total"], 1)
+        ref = REFERENCE_CPU_S_PER_STEP.get(args.workload)
+        if ref is not None:
+            cpu["reference_binary_s_per_step"] = ref[0]
+            cpu["reference_binary_note"] = ref[1]
+        fp = conv.get("fp64")
+        if fp is not None and fp["converged"]:
+            cpu["time_to_converge_s_extrapolated"] = round(fp["steps"] * est["total"], 1)
+
+    headline = None
+    if "mixed_learned" in conv:
+        ml, fp = conv["mixed_learned"], conv["fp64"]
+        headline = dict(
+            fp64_time_to_converge_s=fp["seconds"], fp64_steps=fp["steps"],
+            mixed_learned_time_to_converge_s=ml["seconds"], mixed_learned_steps=ml["steps"],
+            speedup=ml.get("speedup_vs_fp64"),
+            abs_E_error_per_atom=ml.get("abs_E_error_vs_fp64_per_atom"),
+            target="speedup >= 3 with |E - E_FP64| <= 1e-6 Ha/atom",
+            met=bool(ml.get("speedup_vs_fp64") is not None and ml["speedup_vs_fp64"] >= 3.0 and
+                     ml.get("abs_E_error_vs_fp64_per_atom", 1.0) <= 1e-6))
 
     out = {
-        "metric": "SD steps/s and time-to-|dE|<1e-7 Ha: mixed+learned vs FP64 GPU, 1 MI355X",
+        "metric": "SD steps/s and time-to-|ΔE|<1e-7 Ha: mixed+learned vs FP64 GPU, 1 MI355X",
         "value": round(value, 3),
         "unit": "SD steps/s",
         "n_gpus": 1,
@@ -287,19 +336,21 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded water positions and Gaussian orbitals)",
-        "config": {"workload": args.workload, "waters": w["nmol"], "atoms": 3 * w["nmol"],
+        "config": {"workload": args.workload, "waters": w["nmol"], "atoms": natoms,
                    "cell_bohr": w["L"], "fft_grid": w["nfft"], "orbitals": w["ne"],
                    "projectors": sizes["nproj"], "masked_G": sizes["ng"], "inner": inner,
                    "precision": "fp64", "start": "gaussian", "graphs": bool(args.graphs),
                    "setup_s": round(t_setup, 2)},
-        "time_to_converge": conv,
-        "mixed": mixed,
+        "headline": headline,
+        "time_to_converge": conv or None,
+        "dropin": dropin,
         "profiled_pass": {"steps": prof_steps, "ms_per_step": round(elapsed_prof * 1000.0 / prof_steps, 4)},
         "lagrange_iterations_last_step": lag_it,
         "epilogue": epilogue,
         "E_last": float(E[0]),
         "phases": phases,
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
     }
     return out
@@ -377,13 +428,15 @@ def run_batch(args):
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=500)
+    p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--workload", default="h2o-32", choices=sorted(WORKLOADS) + ["batch-64"])
+    p.add_argument("--workload", default="h2o-64", choices=sorted(WORKLOADS) + ["batch-64"])
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--converge-steps", type=int, default=10000)
     p.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    p.add_argument("--no-mixed", dest="mixed", action="store_false")
+    p.add_argument("--all-legs", action="store_true",
+                   help="also time FP64 + learned, mode 1 and mode 2 to convergence")
+    p.add_argument("--no-dropin", dest="dropin", action="store_false")
     p.add_argument("--graphs", action="store_true",
                    help="replay each 10-step call from a captured HIP graph")
     p.add_argument("--cpu-sample", type=int, default=16)

@@ -132,6 +132,23 @@ int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
  * has no such mode; its all-FP32 build (float_single.h with sfftpack3d) is
  * the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
+/* FP64 refresh period of modes 1 and 2: step k of a run (counted from the
+ * last cpsd_dev_set_orbitals, k = 1, 2, ...) runs in FP64 when k is a
+ * multiple of `period`, besides the last step of every call, which the
+ * energies need in FP64 (so a call with inner = 1 runs only FP64 steps);
+ * 0 (the default) refreshes on the last step of each call only.  Mode 2
+ * is refused with moving ions (set_ions move = 1), and moving ions with
+ * mode 2 set. */
+int cpsd_dev_set_refresh(cpsd_dev *h, int period);
+/* Safeguards of cpsd_dev_run(): a non-finite energy or delta is an error
+ * (the orbitals are lost; set them again).  With the fallback on (the
+ * default, rise = 1e-8), a call whose energy exceeds the previous call's by
+ * more than rise * max(1, |E|) while a mixed mode or a preconditioner is set
+ * turns both off, so the run continues with the reference's FP64 step;
+ * _info reports how often that happened and the mode and preconditioner now
+ * in force. */
+int cpsd_dev_set_fallback(cpsd_dev *h, int enable, double rise);
+int cpsd_dev_fallback_info(cpsd_dev *h, int *fallbacks, int *mode, int *precond);
 /* the preconditioned SD step (no counterpart in the reference): with
  * K[nfft3d] set (one factor per half-complex G in the reference's layout,
  * e.g. the output of cpsd_precond_predict()), every step of a call becomes
@@ -145,6 +162,39 @@ int cpsd_dev_set_precision(cpsd_dev *h, int mode);
  * epilogue runs plain steps).  cpsi, hml and the energies keep the
  * reference's definitions.  K = NULL turns it off. */
 int cpsd_dev_set_preconditioner(cpsd_dev *h, const double *K);
+/* ---- learned preconditioner (no counterpart in the reference) ----------
+ * A per-G MLP (4 features -> 32 -> 32 -> 1, tanh, FP32 weights;
+ * CPSD_MLP_NPARAM floats) predicts the factors of the preconditioned step
+ * from features of each masked G: the kinetic diagonal |G|^2/2, the
+ * diagonal of the KB operator and the local pseudopotential at G.
+ * cpsd_dev_set_learned_preconditioner() evaluates it on the context's G and
+ * sets K = clamp(alpha exp(y), kmin, kmax) as cpsd_dev_set_preconditioner()
+ * would.  Training (cpsd_mlp_train) fits the least-squares step factor
+ * K R ~ c_inf - c1 per G over states of the project's own SD runs, from the
+ * per-G sums that cpsd_dev_stats_* collect; X[n][4] features, A[n], B[n]
+ * sums, loss L = sum A K^2 - 2 B K.                                      */
+#define CPSD_MLP_NPARAM 1249
+int cpsd_mlp_nparam(void);
+int cpsd_dev_set_learned_preconditioner(cpsd_dev *h, const float *params, double alpha, double kmin,
+                                        double kmax);
+/* features of the masked G in packed order (kx fastest, then ky, kz as in
+ * the reference's grid), X[ng][4]; ng from cpsd_dev_sizes() */
+int cpsd_dev_precond_features(cpsd_dev *h, float *X);
+/* cinf: converged orbitals (ELCIN layout).  Between _begin and _end every
+ * energy evaluation of cpsd_dev_run() adds its state c1 (the last step's):
+ * A += |dte R|^2, B += Re(conj(dte R)(cinf - c1)) per G summed over the
+ * orbitals, R = cpsi + c1 hml; _read: A[ng], B[ng] in packed order */
+int cpsd_dev_stats_begin(cpsd_dev *h, const double *cinf);
+int cpsd_dev_stats_end(cpsd_dev *h);
+int cpsd_dev_stats_read(cpsd_dev *h, double *A, double *B);
+/* the loss and its gradient for params (one forward and backward pass on
+ * the GPU), `iters` Adam steps from params (updated in place; loss_hist[iters]
+ * may be NULL), and the network output y[n] */
+int cpsd_mlp_loss_grad(const float *X, const double *A, const double *B, long n, const float *params,
+                       float *grad, double *loss);
+int cpsd_mlp_train(const float *X, const double *A, const double *B, long n, float *params, int iters,
+                   float lr, double *loss_hist);
+int cpsd_mlp_predict(const float *X, long n, const float *params, float *y);
 /* enable != 0: each cpsd_dev_run() call (its inner steps and the energy
  * evaluation) is captured once into a HIP graph and replayed on later calls
  * with the same inner, dt, fmass, precision mode and state parity, which
@@ -153,6 +203,33 @@ int cpsd_dev_set_preconditioner(cpsd_dev *h, const double *K);
  * the first call in each precision mode, run uncaptured.  No counterpart in
  * the reference. */
 int cpsd_dev_set_graphs(cpsd_dev *h, int enable);
+/* Many independent cells advanced together (no counterpart in the
+ * reference).  Each cpsd_batch_run() call runs `inner` steps and the energy
+ * evaluation of every cell (E[10 i .. 10 i + 9], E[10 i] the previous energy
+ * on entry, deltae/deltac/deltar[i]).  A call is captured once into one HIP
+ * graph holding every cell's work on its own stream, and replayed with one
+ * hipGraphLaunch while every cell is in the state the capture started from;
+ * other calls (a cell's first call in its precision mode, moving ions,
+ * profiling, training statistics) run eagerly, one stream per cell.  The
+ * cells must not be run on their own while the batch is in use.  _info:
+ * graph launches, captures and eager calls so far. */
+typedef struct cpsd_batch cpsd_batch;
+int cpsd_batch_create(cpsd_batch **b, cpsd_dev *const *cells, int n);
+int cpsd_batch_run(cpsd_batch *b, int inner, double dt, double fmass, double *E, double *deltae,
+                   double *deltac, double *deltar);
+int cpsd_batch_info(cpsd_batch *b, long *graph_launches, long *captures, long *eager_calls);
+void cpsd_batch_destroy(cpsd_batch *b);
+/* The launch-plan autotuner (no counterpart in the reference): measures
+ * candidate plans on the context's own orbitals (split-K workgroup target
+ * of the small-output products, orbitals per FFT launch pair, orbitals per
+ * density partial plane), `steps` SD steps each, timed with HIP events, and
+ * keeps the fastest; plans are remembered per size class (grid, orbitals,
+ * projectors, precision mode, preconditioner) for the process, so later
+ * contexts of the class take theirs without measuring.  The orbitals are
+ * kept and the run state is reset as cpsd_dev_set_orbitals() resets it.
+ * plan[5] (may be NULL): fb, orbitals per density plane (FP64, FP32 pass),
+ * split-K target, 1 if taken from the process's table. */
+int cpsd_dev_autotune(cpsd_dev *h, int steps, int *plan);
 /* last step's Lagrange iteration count and final max|dX| */
 int cpsd_dev_lagrange_info(cpsd_dev *h, int *iterations, double *adiff);
 /* sizes: [0] ng (masked G vectors), [1] ngp, [2] nproj */

@@ -38,10 +38,15 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_lda_device", "cpsd_erfc_device", "cpsd_rotate", "cpsd_dev_epilogue",
                "cpsd_syev_sorted", "cpsd_kbpp_device", "cpsd_dev_set_precision", "cpsd_dev_set_graphs",
                "cpsd_dev_run_start", "cpsd_dev_run_finish", "cpsd_fft3d_batch_noalloc",
-               "cpsd_fft3d_batch_noalloc_max", "cpsd_dev_set_preconditioner")
+               "cpsd_fft3d_batch_noalloc_max", "cpsd_dev_set_preconditioner", "cpsd_mlp_nparam",
+               "cpsd_dev_set_learned_preconditioner", "cpsd_dev_precond_features", "cpsd_dev_stats_begin",
+               "cpsd_dev_stats_end", "cpsd_dev_stats_read", "cpsd_mlp_loss_grad", "cpsd_mlp_train",
+               "cpsd_mlp_predict", "cpsd_dev_set_refresh", "cpsd_dev_set_fallback", "cpsd_dev_fallback_info",
+               "cpsd_batch_create", "cpsd_batch_run", "cpsd_batch_info", "cpsd_batch_destroy",
+               "cpsd_dev_autotune")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
-                "cpsdh_write_ionout")
+                "cpsdh_write_ionout", "cpsdh_write_vpp", "cpsdh_read_vpp")
 
 
 class CpsdError(RuntimeError):
@@ -76,6 +81,25 @@ def hip_lib():
         L.cpsd_dev_set_precision.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_set_preconditioner.argtypes = [C.c_void_p, C.c_void_p]
+        L.cpsd_dev_set_refresh.argtypes = [C.c_void_p, C.c_int]
+        L.cpsd_dev_autotune.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.cpsd_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
+        L.cpsd_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.cpsd_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.cpsd_batch_destroy.argtypes = [C.c_void_p]
+        L.cpsd_batch_destroy.restype = None
+        L.cpsd_dev_set_fallback.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.cpsd_dev_fallback_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.cpsd_dev_set_learned_preconditioner.argtypes = [C.c_void_p, _fp, C.c_double, C.c_double, C.c_double]
+        L.cpsd_dev_precond_features.argtypes = [C.c_void_p, _fp]
+        L.cpsd_dev_stats_begin.argtypes = [C.c_void_p, _dp]
+        L.cpsd_dev_stats_end.argtypes = [C.c_void_p]
+        L.cpsd_dev_stats_read.argtypes = [C.c_void_p, _dp, _dp]
+        L.cpsd_mlp_loss_grad.argtypes = [_fp, _dp, _dp, C.c_long, _fp, _fp, _DPTR]
+        L.cpsd_mlp_train.argtypes = [_fp, _dp, _dp, C.c_long, _fp, C.c_int, C.c_float, _dp]
+        L.cpsd_mlp_predict.argtypes = [_fp, C.c_long, _fp, _fp]
         L.cpsd_dev_run_start.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]
         L.cpsd_dev_run_finish.argtypes = [C.c_void_p, _dp, _DPTR, _DPTR, _DPTR]
         L.cpsd_dev_sizes.argtypes = [C.c_void_p, _ip]
@@ -141,6 +165,42 @@ def _err():
 def _check(rc, what):
     if rc != 0:
         raise CpsdError("%s failed: %s" % (what, _err()))
+
+
+class Batch:
+    """independent cells advanced together, one graph launch per call
+    (cpsd_batch_* of include/cpsd_abi.h)"""
+
+    def __init__(self, devs):
+        self.devs = list(devs)
+        arr = (C.c_void_p * len(self.devs))(*[d.h.value for d in self.devs])
+        h = C.c_void_p()
+        _check(hip_lib().cpsd_batch_create(C.byref(h), arr, len(self.devs)), "cpsd_batch_create")
+        self.h = h
+
+    def run(self, inner, dt=5.8, fmass=400000.0):
+        n = len(self.devs)
+        E = np.zeros(10 * n)
+        for i, d in enumerate(self.devs):
+            E[10 * i] = d.E[0]
+        de, dc, dr = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(hip_lib().cpsd_batch_run(self.h, inner, dt, fmass, E, de, dc, dr), "cpsd_batch_run")
+        out = []
+        for i, d in enumerate(self.devs):
+            d.E[:] = E[10 * i:10 * i + 10]
+            d.deltar = dr[i]
+            out.append((d.E.copy(), float(de[i]), float(dc[i])))
+        return out
+
+    def info(self):
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        _check(hip_lib().cpsd_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c)), "cpsd_batch_info")
+        return dict(graph_launches=a.value, captures=b.value, eager=c.value)
+
+    def close(self):
+        if self.h:
+            hip_lib().cpsd_batch_destroy(self.h)
+            self.h = None
 
 
 def run_batch(devs, inner, dt=5.8, fmass=400000.0):
@@ -438,6 +498,55 @@ class DeviceSD:
             raise CpsdError("preconditioner needs %d factors, got %d" % (self.cell.nfft3d, K.size))
         _check(hip_lib().cpsd_dev_set_preconditioner(self.h, K.ctypes.data), "cpsd_dev_set_preconditioner")
 
+    def autotune(self, steps=3):
+        """measure and set the fastest launch plan of this size class (the
+        orbitals are kept, the run state restarts)"""
+        p = np.zeros(5, dtype=np.int32)
+        _check(hip_lib().cpsd_dev_autotune(self.h, int(steps), p), "cpsd_dev_autotune")
+        return dict(fft_batch=int(p[0]), density_plane_orbitals=(int(p[1]), int(p[2])), splitk_target=int(p[3]),
+                    from_table=bool(p[4]))
+
+    def set_refresh(self, period):
+        """FP64 refresh every `period` steps of the mixed modes (0: last step of each call)"""
+        _check(hip_lib().cpsd_dev_set_refresh(self.h, int(period)), "cpsd_dev_set_refresh")
+
+    def set_fallback(self, enable=True, rise=1e-8):
+        """fall back to the FP64 step when a call's energy exceeds the previous
+        call's by more than rise * max(1, |E|)"""
+        _check(hip_lib().cpsd_dev_set_fallback(self.h, 1 if enable else 0, float(rise)), "cpsd_dev_set_fallback")
+
+    def fallback_info(self):
+        f, m, p = C.c_int(), C.c_int(), C.c_int()
+        _check(hip_lib().cpsd_dev_fallback_info(self.h, C.byref(f), C.byref(m), C.byref(p)),
+               "cpsd_dev_fallback_info")
+        return dict(fallbacks=f.value, mode=m.value, precond=p.value)
+
+    def set_learned_preconditioner(self, params, alpha, kmin=0.0, kmax=1e3):
+        """K = clamp(alpha exp(MLP(features)), kmin, kmax) on this cell's G"""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != MLP_NPARAM:
+            raise CpsdError("the network has %d parameters, got %d" % (MLP_NPARAM, p.size))
+        _check(hip_lib().cpsd_dev_set_learned_preconditioner(self.h, p, alpha, kmin, kmax),
+               "cpsd_dev_set_learned_preconditioner")
+
+    def precond_features(self):
+        X = np.zeros((self.sizes()["ng"], 4), dtype=np.float32)
+        _check(hip_lib().cpsd_dev_precond_features(self.h, X.reshape(-1)), "cpsd_dev_precond_features")
+        return X
+
+    def stats_begin(self, cinf):
+        _check(hip_lib().cpsd_dev_stats_begin(self.h, np.ascontiguousarray(cinf, dtype=np.float64)),
+               "cpsd_dev_stats_begin")
+
+    def stats_end(self):
+        _check(hip_lib().cpsd_dev_stats_end(self.h), "cpsd_dev_stats_end")
+
+    def stats_read(self):
+        ng = self.sizes()["ng"]
+        A, B = np.zeros(ng), np.zeros(ng)
+        _check(hip_lib().cpsd_dev_stats_read(self.h, A, B), "cpsd_dev_stats_read")
+        return A, B
+
     def set_graphs(self, enable=True):
         """capture each run() into a HIP graph and replay it on later calls
         with the same arguments and state parity"""
@@ -532,6 +641,40 @@ def syev_sorted(a):
     eig = np.zeros(n)
     _check(hip_lib().cpsd_syev_sorted(n, v, eig), "cpsd_syev_sorted")
     return eig, v
+
+
+MLP_NPARAM = 1249
+
+
+def mlp_loss_grad(X, A, B, params):
+    """loss L = sum A K^2 - 2 B K (K = exp(MLP(X))) and dL/dparams, on the GPU"""
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = C.c_double()
+    _check(hip_lib().cpsd_mlp_loss_grad(X, np.ascontiguousarray(A, dtype=np.float64),
+                                        np.ascontiguousarray(B, dtype=np.float64), X.size // 4,
+                                        np.ascontiguousarray(params, dtype=np.float32), g, C.byref(loss)),
+           "cpsd_mlp_loss_grad")
+    return loss.value, g
+
+
+def mlp_train(X, A, B, params, iters, lr):
+    """iters Adam steps on the GPU; returns (params, loss history)"""
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1)
+    p = np.array(params, dtype=np.float32)
+    hist = np.zeros(iters)
+    _check(hip_lib().cpsd_mlp_train(X, np.ascontiguousarray(A, dtype=np.float64),
+                                    np.ascontiguousarray(B, dtype=np.float64), X.size // 4, p, int(iters),
+                                    float(lr), hist), "cpsd_mlp_train")
+    return p, hist
+
+
+def mlp_predict(X, params):
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1)
+    y = np.zeros(X.size // 4, dtype=np.float32)
+    _check(hip_lib().cpsd_mlp_predict(X, X.size // 4, np.ascontiguousarray(params, dtype=np.float32), y),
+           "cpsd_mlp_predict")
+    return y
 
 
 def fft3d(direction, nfft, a):

@@ -193,6 +193,15 @@ int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const do
                        double *partial, size_t partial_cap, hipStream_t s);
 int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s);
 
+/* ---- learned preconditioner (precond_mlp.hip) ----------------------------- */
+#define CPSD_MLP_NPARAM 1249
+int launch_mlp_predict(const float *X, long n, const float *P, float *y, double *K, double alpha, double kmin,
+                       double kmax, hipStream_t s);
+int launch_precond_features(const double *tg, const double2 *B, const double *pnrm, int nproj,
+                            const double2 *vall, double omega, int ng, int ngp, float *X, hipStream_t s);
+int launch_precond_stats(const double2 *CP, const double2 *Rh, const double2 *C1, const double2 *Cinf, int ne,
+                         int ngp, double dte, double *A, double *B, hipStream_t s);
+
 /* ---- epilogue (host) ---------------------------------------------------- */
 /* eigenpairs of the symmetric n x n column-major matrix a (upper triangle
  * read): a <- eigenvectors in columns, eig <- eigenvalues, both in the order

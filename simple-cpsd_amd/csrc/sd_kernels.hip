@@ -1059,19 +1059,51 @@ int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double sca
    return 0;
 }
 
+/* The persistent Lagrange kernels meet at grid barriers, so every workgroup
+ * of the grid must be resident at once: the grid may not exceed what the
+ * device holds of the kernel (occupancy per CU times CUs).  Contexts that
+ * run concurrently share the CUs, which is why the barrier wait is bounded
+ * (err) as well. */
+static int lagrange_grid_fits(const void *k, int grid, int lds)
+{
+   int dev = 0, cus = 0, per = 0;
+   CPSD_CHECK(hipGetDevice(&dev));
+   CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+   CPSD_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, lds));
+   if ((long)per * cus < grid) {
+      cpsd_set_error("lagrange: %d workgroups must be resident at once but the device holds %d x %d", grid, per,
+                     cus);
+      return -1;
+   }
+   return 0;
+}
+
 int sd_lagrange(const LagArgs &a, hipStream_t s)
 {
+   static std::atomic<int> checked[3] = {0, 0, 0};   /* largest grid shown to fit, per kernel */
    if (a.ne <= 128) {
       const int nt = (a.ne + 15) / 16;
       const int lds = 6 * 16 * (128 + 2) * (int)sizeof(double);
       static std::atomic<unsigned long long> attr{0};
       if (dyn_lds_once((const void *)k_lagrange_res<128>, lds, attr) != 0) return -1;
+      if (nt * nt > checked[0].load()) {
+         if (lagrange_grid_fits((const void *)k_lagrange_res<128>, nt * nt, lds) != 0) return -1;
+         checked[0].store(nt * nt);
+      }
       hipLaunchKernelGGL((k_lagrange_res<128>), dim3(nt * nt), dim3(256), lds, s, a);
    } else if (a.ne <= 256) {
       const int nt = (a.ne + 15) / 16;
+      if (nt * nt > checked[1].load()) {
+         if (lagrange_grid_fits((const void *)k_lagrange<16>, nt * nt, 0) != 0) return -1;
+         checked[1].store(nt * nt);
+      }
       hipLaunchKernelGGL(k_lagrange<16>, dim3(nt * nt), dim3(256), 0, s, a);
    } else if (a.ne <= 512) {
       const int nt = (a.ne + 31) / 32;
+      if (nt * nt > checked[2].load()) {
+         if (lagrange_grid_fits((const void *)k_lagrange<32>, nt * nt, 0) != 0) return -1;
+         checked[2].store(nt * nt);
+      }
       hipLaunchKernelGGL(k_lagrange<32>, dim3(nt * nt), dim3(256), 0, s, a);
    } else {
       cpsd_set_error("lagrange: ne=%d above 512 is not supported", a.ne);

@@ -64,3 +64,69 @@ def test_run_finish_needs_a_started_call(gpu):
         d.run_start(2)
     d.run_finish()
     d.close()
+
+
+@pytest.mark.parametrize("mode,learned", [(0, False), (1, True)])
+def test_one_graph_batch_equals_sequential(gpu, mode, learned):
+    """cpsd_batch: after the first (eager) call, every call of all cells is
+    one captured graph replayed by a single launch; results equal the cells
+    run one after the other, bit for bit, with uneven grids and orbital
+    counts, and with the mixed mode and a preconditioner"""
+    say("one-graph batch, mode %d, preconditioner %s" % (mode, learned))
+    seq = make(False, mode)
+    bat = make(False, mode)
+    if learned:
+        for d in seq + bat:
+            d.set_preconditioner(np.full(d.cell.nfft3d, 2.0))
+    b = cpsd_amd.Batch(bat)
+    for call in range(5):
+        ref = [d.run(5) for d in seq]
+        got = b.run(5)
+        for (Er, der, dcr), (Eg, deg, dcg) in zip(ref, got):
+            assert np.array_equal(Er, Eg), call
+            assert der == deg and dcr == dcg
+    info = b.info()
+    # call 1 eager; an odd inner alternates the orbital block between two
+    # start states, each captured once and then replayed
+    assert info == dict(graph_launches=4, captures=2, eager=1)
+    b.close()
+    for a, c in zip(seq, bat):
+        x = a.get(c2=True, dn=True, hml=True)
+        y = c.get(c2=True, dn=True, hml=True)
+        for k in x:
+            assert np.array_equal(x[k], y[k]), k
+    for d in seq + bat:
+        d.close()
+
+
+def test_autotune_keeps_parity_and_is_remembered(gpu):
+    """the autotuner's plan for a 128-orbital cell: the orbitals survive the
+    measurement, the steps afterwards still match the oracle, and a second
+    context of the same size class takes the remembered plan (no
+    measurement) and runs the same bits"""
+    import oracle_lib as O
+    from test_gpu_parity import compare
+    say("autotune")
+    nfft, L, ne = 24, 12.0, 128
+    cell = cpsd_amd.water_cell(2, L, nfft, seed=7)
+    psi = cell.random_orbitals(ne, seed=3, decay=0.05)
+    a = cpsd_amd.DeviceSD(cell, ne)
+    a.set_orbitals(psi)
+    plan = a.autotune(2)
+    assert not plan["from_table"] and plan["splitk_target"] in (1024, 2048, 4096)
+    st = O.State(O.Setup(1, L, nfft, cell.katm, cell.zv), ne, cell.lmmax, cell.vl, cell.vnl, cell.vnlnrm,
+                 cell.r, psi, cell.amass)
+    for _ in range(2):
+        compare(a.run(3)[0], st.step(3)[0])
+    assert np.max(np.abs(a.get(c2=True)["c2"] - st.c2)) < 1e-10
+    b = cpsd_amd.DeviceSD(cell, ne)
+    b.set_orbitals(psi)
+    plan_b = b.autotune(2)
+    assert plan_b["from_table"] and {k: v for k, v in plan_b.items() if k != "from_table"} == \
+        {k: v for k, v in plan.items() if k != "from_table"}
+    a.set_orbitals(psi)
+    Ea = [a.run(3)[0] for _ in range(2)]
+    Eb = [b.run(3)[0] for _ in range(2)]
+    assert np.array_equal(np.array(Ea), np.array(Eb))
+    a.close()
+    b.close()

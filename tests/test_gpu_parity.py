@@ -145,6 +145,42 @@ def test_dropin_cpsd_example0(gpu):
         c2, E = out["c2"], out["E"]
 
 
+def test_dropin_context_kept_and_rebuilt(gpu):
+    """The drop-in keeps its device context while the static inputs stay
+    the same (two Example0 calls), rebuilds it for other inputs (water
+    between them), and moves the kept context's fixed ions when r2 changes;
+    every call against the oracle"""
+    say("dropin context reuse")
+    ctl, st = example0_oracle_state()
+    cell = cell_from_state(st)
+    wst = water_oracle_state()
+    wcell = cell_from_state(wst)
+    c2, E = st.c2.copy(), np.zeros(10)
+    wc2, wE = wst.c2.copy(), np.zeros(10)
+    for cl, stt, key in ((cell, st, "e"), (cell, st, "e"), (wcell, wst, "w"), (cell, st, "e"), (cell, st, "e")):
+        if key == "e":
+            out = cpsd_amd.cpsd_dropin(cl, 4, 5, c2, E, ctl["dt"], ctl["fmass"])
+            c2, E = out["c2"], out["E"]
+        else:
+            out = cpsd_amd.cpsd_dropin(cl, 4, 5, wc2, wE, 5.8, 400000.0)
+            wc2, wE = out["c2"], out["E"]
+        Er, der, dcr = stt.step(5, ctl["dt"], ctl["fmass"])
+        compare(out["E"], Er)
+        assert np.max(np.abs(out["c2"] - stt.c2)) < 1e-10
+        assert np.max(np.abs(out["dn"] - stt.dn)) < 1e-10
+    # the same tables with the ions moved (move = 0): the kept context must
+    # take the new positions
+    r = st.r2.copy()
+    r[0] += 0.05
+    st.r2[:] = r
+    st.r1[:] = r
+    cell.r = r.copy()
+    out = cpsd_amd.cpsd_dropin(cell, 4, 5, c2, E, ctl["dt"], ctl["fmass"])
+    Er, _, _ = st.step(5, ctl["dt"], ctl["fmass"])
+    compare(out["E"], Er)
+    assert np.max(np.abs(out["c2"] - st.c2)) < 1e-10
+
+
 def test_device_water_vs_oracle(gpu):
     say("device water 32^3")
     st = water_oracle_state()
