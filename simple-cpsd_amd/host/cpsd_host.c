@@ -263,15 +263,17 @@ int cpsdh_kbpp(double zv, int lmax, int nrho, double drho, const double *rho, co
       }
    }
 
-   /* G = 0: vl = 4 pi S[r^2 dv_lmax] + 2 pi zv r_last^2, vnl_s = sqrt(4 pi) S[r wp_0 dv_0] */
+   /* G = 0: vl = 4 pi S[r^2 dv_lmax] + 2 pi zv r_last^2, vnl_s = sqrt(4 pi) S[r wp_0 dv_0].
+    * The reference writes vnl_s(G=0) for every lmax (ckbppv3/dkbppv3.c:436-437);
+    * with lmax = 0 its channel 0 is the local one, not a difference. */
    {
       double a = 0.0, b = 0.0;
       for (j = 0; j < nrho; ++j) {
          a += tab[3 * (size_t)nrho + j] * rho[j];
-         b += tab[j] * rho[j];
+         b += lmax > 0 ? tab[j] * rho[j] : simpson_weight(j, nrho, drho) * rho[j] * wp[j] * vp_in[j];
       }
       vl[0] = fourpi * a + 2.0 * pi * zv * rho[nrho - 1] * rho[nrho - 1];
-      if (lmax > 0) vnl[0] = c0 * b;
+      vnl[0] = c0 * b;
       for (l = 1; l < lmmax; ++l) vnl[(size_t)l * nfft3d] = 0.0;
    }
    free(tab);

@@ -181,5 +181,8 @@ int main(int argc, char **argv)
       printf("fft roundtrip %.3e\n", err);
       free(a); free(b); free(tmp);
    }
+   free(katm); free(r1); free(r2); free(fion);
+   free(vl); free(vnl); free(G); free(masker); free(tg); free(vc); free(vg);
+   free(c1); free(c2); free(cpsi); free(dn); free(hml); free(lmd); free(work);
    return 0;
 }
