@@ -187,9 +187,11 @@ def run(args):
             Eprev = Ec[0]
         tconv = time.perf_counter() - t0
         fb = dev.fallback_info()
+        ortho = float(np.max(np.abs(dev.epilogue()["ortho"] - np.eye(w["ne"]))))
         configure(0, False)
         return dict(converged=done, steps=nsteps, seconds=round(tconv, 4), E=float(Ec[0]),
-                    last_dE=None if last is None else float(last), fallbacks=fb["fallbacks"])
+                    last_dE=None if last is None else float(last), fallbacks=fb["fallbacks"],
+                    ortho_max_dev_at_end=ortho)
 
     def rate(mode, learned_on):
         """steps/s of the configured step over --steps steps (after warmup)"""
@@ -356,72 +358,93 @@ def run(args):
     return out
 
 
-def batch_cells(seed):
-    """64 seeded water cells of 1-8 molecules: cell edge 12-22 bohr growing with
-    the molecule count, grid 24^3-64^3 by edge, 4 orbitals per molecule"""
-    rng = np.random.default_rng(seed)
-    cells = []
-    for _ in range(64):
-        nmol = int(rng.integers(1, 9))
-        L = 12.0 + 10.0 * (nmol - 1) / 7.0 + float(rng.uniform(-0.4, 0.4))
-        n = 24 if L < 14 else 32 if L < 17 else 48 if L < 20 else 64
-        cells.append((nmol, L, n))
-    return cells
-
-
 def run_batch(args):
-    """--workload batch-64: the 64 cells advance 10-step calls one cell after
-    the other (each call waited for), then with every cell's call started
-    before any is finished (one stream per cell); cell-steps per second"""
+    """--workload batch-64: 64 independent water cells of uneven size (grids
+    24^3-64^3, 4-32 orbitals) advance 10-step calls in precision mode 1.
+    Three ways are timed over the same calls: one cell after the other (each
+    call waited for), every cell's call started before any is finished (one
+    stream per cell), and the one-graph batch (cpsd_batch_run: every cell's
+    call in one HIP graph, one launch per call), after each size class's
+    launch plan was chosen by the autotuner.  value: cell-steps/s of the
+    one-graph batch."""
     import cpsd_amd
     t0 = time.perf_counter()
-    specs = batch_cells(args.seed)
-    devs = []
+    specs = cpsd_amd.water_batch_specs(args.seed)
+    cells = []
     for i, (nmol, L, n) in enumerate(specs):
         cell = cpsd_amd.water_cell(nmol, L, n, seed=args.seed + i)
-        d = cpsd_amd.DeviceSD(cell, 4 * nmol)
-        d.set_graphs(args.graphs)
-        d.set_orbitals(cell.random_orbitals(4 * nmol, seed=args.seed + 100 + i, decay=0.05))
-        devs.append(d)
+        cells.append((cell, 4 * nmol, cell.random_orbitals(4 * nmol, seed=args.seed + 100 + i, decay=0.05)))
     t_setup = time.perf_counter() - t0
+
+    def make():
+        devs = []
+        for cell, ne, psi in cells:
+            d = cpsd_amd.DeviceSD(cell, ne)
+            d.set_precision(1)
+            d.set_orbitals(psi)
+            devs.append(d)
+        return devs
+
+    devs = make()
+    t0 = time.perf_counter()
+    plans = {}
+    for d in devs:
+        p = d.autotune(2)
+        key = "%d^3/%d" % (d.cell.nfft, d.ne)
+        if not p["from_table"]:
+            plans[key] = p
+    t_tune = time.perf_counter() - t0
     inner = 10
     calls = max(1, args.steps // inner)
-    for _ in range(max(1, args.warmup // inner)):
+    warm = max(1, args.warmup // inner)
+    for _ in range(warm):
         for d in devs:
             d.run(inner)
-        cpsd_amd.run_batch(devs, inner)
     t0 = time.perf_counter()
     for _ in range(calls):
         for d in devs:
             d.run(inner)
     seq = time.perf_counter() - t0
+    for _ in range(warm):
+        cpsd_amd.run_batch(devs, inner)
     t0 = time.perf_counter()
     for _ in range(calls):
         cpsd_amd.run_batch(devs, inner)
-    bat = time.perf_counter() - t0
+    con = time.perf_counter() - t0
+    b = cpsd_amd.Batch(devs)
+    for _ in range(warm + 1):
+        b.run(inner)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        b.run(inner)
+    one = time.perf_counter() - t0
+    info = b.info()
+    b.close()
     for d in devs:
         d.close()
     cell_steps = len(devs) * calls * inner
     return {
         "metric": "SD cell-steps/s, 64 independent water cells, 1 MI355X",
-        "value": round(cell_steps / bat, 1),
+        "value": round(cell_steps / one, 1),
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": calls * inner,
         "warmup": args.warmup,
-        "ms_per_step": round(bat * 1000.0 / (calls * inner), 4),
+        "ms_per_step": round(one * 1000.0 / (calls * inner), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32/f64",
         "data": "synthetic (seeded water cells and Gaussian orbitals)",
         "config": {"workload": "batch-64", "cells": len(devs),
                    "waters": [c[0] for c in specs], "fft_grids": sorted(set(c[2] for c in specs)),
-                   "orbitals": "4 per water", "inner": inner, "graphs": bool(args.graphs),
-                   "setup_s": round(t_setup, 2)},
-        "sequential": {"value": round(cell_steps / seq, 1), "unit": "cell-steps/s",
-                       "ms_per_step_all_cells": round(seq * 1000.0 / (calls * inner), 4)},
-        "speedup_vs_sequential": round(seq / bat, 4),
+                   "orbitals": "4 per water", "inner": inner, "precision": "mode 1 (FP32 FFT/KB, FP64 rest)",
+                   "setup_s": round(t_setup, 2), "autotune_s": round(t_tune, 2)},
+        "one_graph": {"value": round(cell_steps / one, 1), "graph_info": info},
+        "concurrent_streams": {"value": round(cell_steps / con, 1)},
+        "sequential": {"value": round(cell_steps / seq, 1)},
+        "speedup_vs_sequential": round(seq / one, 4),
+        "autotuned_plans": plans,
     }
 
 

@@ -197,11 +197,74 @@ int oracle_last_lagrange_iterations(void) { return g_lag_iters; }
  * cpsd_dev_set_preconditioner; no counterpart in the reference) ---------- */
 static const double *g_precond_K;   /* [nfft3d], one factor per complex G */
 static const double *g_precond_h;   /* [ne*ne] column-major */
+static double *g_precond_D;         /* [ne*ne] column-major, the removed c1 component */
 
 void oracle_set_preconditioner(const double *K, const double *h)
 {
    g_precond_K = K;
    g_precond_h = h;
+}
+
+static void ensure_precond_D(int ne)
+{
+   static int cap = 0;
+   if (ne * ne > cap) {
+      free(g_precond_D);
+      g_precond_D = (double *) malloc((size_t)ne * ne * sizeof(double));
+      cap = ne * ne;
+   }
+}
+
+/* The overlaps of the preconditioned step.  K o R is not orthogonal to c1
+ * where K varies, which moves <c1|c2> away from the identity and slows the
+ * Lagrange fixed point; the step's component along c1 is taken out first:
+ * c2' = c2 - c1 D, D = <c1|c2> - I, and the s matrices are formed from
+ *   A12' = A12 - A11 D,  A22' = A22 - D^T A12 - A12^T D + D^T A11 D
+ * (A_ab(i,j) = <a_i|b_j>, all ne x ne, column-major).  The constraint then
+ * adds dte c1 (X - D/dte) to c2. */
+static void precond_overlaps(int nfft, size_t n2, int ne, double dte, const double *c1, const double *c2,
+                             double *s11, double *s12, double *s21, double *s22, double *D)
+{
+   const size_t kk = (size_t)ne * ne;
+   double *A11 = (double *) malloc(kk * sizeof(double)), *A12 = (double *) malloc(kk * sizeof(double));
+   double *A22 = (double *) malloc(kk * sizeof(double)), *P = (double *) malloc(kk * sizeof(double));
+   int i, j, k;
+   for (j = 0; j < ne; ++j)
+      for (i = 0; i < ne; ++i) {
+         A12[i + (size_t)j*ne] = oracle_gcdotc(nfft, c1 + i*n2, c2 + j*n2);
+         if (i <= j) {
+            A11[i + (size_t)j*ne] = A11[j + (size_t)i*ne] = oracle_gcdotc(nfft, c1 + i*n2, c1 + j*n2);
+            A22[i + (size_t)j*ne] = A22[j + (size_t)i*ne] = oracle_gcdotc(nfft, c2 + i*n2, c2 + j*n2);
+         }
+      }
+   for (j = 0; j < ne; ++j)
+      for (i = 0; i < ne; ++i) D[i + (size_t)j*ne] = A12[i + (size_t)j*ne] - (i == j ? 1.0 : 0.0);
+   /* P = A11 D */
+   for (j = 0; j < ne; ++j)
+      for (i = 0; i < ne; ++i) {
+         double a = 0.0;
+         for (k = 0; k < ne; ++k) a += A11[i + (size_t)k*ne] * D[k + (size_t)j*ne];
+         P[i + (size_t)j*ne] = a;
+      }
+   for (j = 0; j < ne; ++j)
+      for (i = 0; i < ne; ++i) {
+         /* (D^T A12)(i,j), (D^T A12)(j,i) and (D^T P)(i,j) */
+         double q = 0.0, qt = 0.0, r = 0.0;
+         for (k = 0; k < ne; ++k) {
+            q += D[k + (size_t)i*ne] * A12[k + (size_t)j*ne];
+            qt += D[k + (size_t)j*ne] * A12[k + (size_t)i*ne];
+            r += D[k + (size_t)i*ne] * P[k + (size_t)j*ne];
+         }
+         const double a22 = A22[i + (size_t)j*ne] - q - qt + r;
+         const double a12 = A12[i + (size_t)j*ne] - P[i + (size_t)j*ne];    /* A12'(i,j) */
+         const double a21 = A12[j + (size_t)i*ne] - P[j + (size_t)i*ne];    /* A21'(i,j) = A12'(j,i) */
+         const double d = (i == j) ? 1.0 : 0.0;
+         s22[i + (size_t)j*ne] = (d - a22) * 0.5 / dte;
+         s21[i + (size_t)j*ne] = (d - a21) * 0.5;
+         s12[i + (size_t)j*ne] = (d - a12) * 0.5;
+         s11[i + (size_t)j*ne] = -A11[i + (size_t)j*ne] * 0.5 * dte;
+      }
+   free(A11); free(A12); free(A22); free(P);
 }
 
 /* c2 = c1 + K o (c2 - c1 + dte c1 h): with c2 = c1 + dte cpsi on entry this
@@ -290,6 +353,7 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
    sa1  = cpsdwork + jr; jr += (size_t)n0 * n0;
    sa0  = cpsdwork + jr; jr += (size_t)n0 * n0;
 
+   if (g_precond_K) ensure_precond_D(n0);
    for (itime = 0; itime < inner; ++itime) {
       /* :173-181 */
       memcpy(c1, c2, nn * sizeof(double));
@@ -452,8 +516,12 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
             for (j = 0; j < 3; ++j) r2[3*ii+j] = r1[3*ii+j] + dti[ii] * fion[3*ii+j];
 
       /* overlap matrices, :415-437 (note: s21 and s12 are filled from one
-       * triangle each and mirrored, exactly as the reference does) */
-      {
+       * triangle each and mirrored, exactly as the reference does); the
+       * preconditioned form first takes the step's component along c1 out
+       * of c2 (precond_overlaps) */
+      if (g_precond_K) {
+         precond_overlaps(nfft, n2, n0, dte, c1, c2, s11, s12, s21, s22, g_precond_D);
+      } else {
          const double *C1 = c1, *C2 = c2;
          for (i = 0; i < n0; ++i) {
             s22[i + i*n0] = (1.0 - oracle_gcdotc(nfft, C2 + i*n2, C2 + i*n2)) * 0.5 / dte;
@@ -477,11 +545,11 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
        * CONVGLMD = 1e-15 at :19-20) */
       {
          const size_t kk = (size_t)n0 * n0;
+         size_t q;
          int it = 0, done = 0;
          double adiff = 0.0;
          memcpy(sa0, s22, kk * sizeof(double));
          while (!done && (it++) < 30) {
-            size_t q;
             memcpy(sa1, s22, kk * sizeof(double));
             gemm_nn(n0, n0, n0, 1.0, s21, n0, sa0, n0, 1.0, sa1, n0);
             gemm_nn(n0, n0, n0, 1.0, sa0, n0, s12, n0, 1.0, sa1, n0);
@@ -499,6 +567,9 @@ int oracle_cpsd(int inner, int move, int nion, int ispin, const int *ne,
          g_lag_iters = it;
          if (!done) printf("ierr=10 adiff=%f\n", adiff);
          memcpy(lmd, sa1, kk * sizeof(double));
+         /* preconditioned form: c2f = c2' + dte c1 X = c2 + dte c1 (X - D/dte) */
+         if (g_precond_K)
+            for (q = 0; q < kk; ++q) lmd[q] -= g_precond_D[q] / dte;
       }
 
       /* constraint, :482-488 */

@@ -98,8 +98,10 @@ int oracle_last_lagrange_iterations(void);
 /* preconditioned SD step (no counterpart in the reference): with K set,
  * every step becomes c2 = c1 + K o dte (cpsi + c1 h) before the overlaps and
  * the constraint; K[nfft3d] one factor per half-complex G, h[ne*ne]
- * column-major; K = NULL restores the reference's step.  Pointers are kept,
- * not copied. */
+ * column-major; K = NULL restores the reference's step.  The overlaps are
+ * then those of c2 - c1 D, D = <c1|c2> - I (the step's component along c1
+ * taken out), and lmd = X - D/dte with X the Lagrange fixed point, so that
+ * c2 + dte c1 lmd is orthonormal.  Pointers are kept, not copied. */
 void oracle_set_preconditioner(const double *K, const double *h);
 
 /* Symmetric eigen-decomposition used by the epilogue (ccpsdv3/main.c:470-484:

@@ -391,6 +391,20 @@ def water_box(nmol, L, seed=0, jitter=0.3):
     return np.array(katm, dtype=np.int32), np.concatenate(pos)
 
 
+def water_batch_specs(seed, count=64):
+    """(nmol, L, nfft) of `count` seeded water cells of 1-8 molecules: cell
+    edge 12-22 bohr growing with the molecule count, grid 24^3-64^3 by edge;
+    the cells of the batched workload carry 4 orbitals per molecule"""
+    rng = np.random.default_rng(seed)
+    cells = []
+    for _ in range(count):
+        nmol = int(rng.integers(1, 9))
+        L = 12.0 + 10.0 * (nmol - 1) / 7.0 + float(rng.uniform(-0.4, 0.4))
+        n = 24 if L < 14 else 32 if L < 17 else 48 if L < 20 else 64
+        cells.append((nmol, L, n))
+    return cells
+
+
 def water_cell(nmol, L, nfft, seed=0, jitter=0.3, psp_dir=None, kb_device=False):
     """nmol waters (O s+p, H s projectors) in an L-bohr cubic cell; kb_device
     runs the KB transform on the GPU (cpsd_kbpp_device) instead of the host."""

@@ -119,6 +119,7 @@ struct cpsd_dev {
     * off for the following calls (the reference's FP64 step) */
    int fallback_on, fallbacks, have_prevE;
    int poisoned;          /* a call produced non-finite values: clear the state first */
+   int lag_itmax;         /* Lagrange iteration cap of the last step */
    double prevE, fallback_rise;
    float *Bf;             /* FP32 copy of the projectors B, [nproj][2 ngp] */
    int bf_valid;          /* Bf matches B */
@@ -127,6 +128,7 @@ struct cpsd_dev {
    /* <c1|c1> of the next step, carried from this step's overlaps and lmd
     * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
    double *g11c[2], *g11w;
+   double *DP;          /* preconditioned step: D = <c1|c2> - I and A11 D */
    int g11_idx, g11_valid;
    /* preconditioned step (cpsd_dev_set_preconditioner): one factor per
     * packed coefficient; the steps of a call use it when hml holds the
@@ -230,7 +232,7 @@ static void ctx_free(cpsd_dev *c)
                    c->s22, c->LX, c->st1, c->lmd, c->wgmax, c->info, c->hml, c->Mh, c->bar, c->err,
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
-                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->Bb, c->Kp,
+                   c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->DP, c->Bb, c->Kp,
                    c->Xf, c->Pmlp, c->Cinf, c->Rh, c->SA, c->SB};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
@@ -520,7 +522,9 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    TRY(dalloc(&c->Gm, 4 * ee));
    if (ne % 128 == 0 && ne <= 512) {
       TRY(dalloc(&c->g11c[0], ee)); TRY(dalloc(&c->g11c[1], ee)); TRY(dalloc(&c->g11w, 2 * ee));
-   } TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
+   }
+   TRY(dalloc(&c->DP, 2 * ee));
+   TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->Xkb, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->s11, ee)); TRY(dalloc(&c->s12, ee)); TRY(dalloc(&c->s21, ee)); TRY(dalloc(&c->s22, ee));
    TRY(dalloc(&c->LX, 2 * ee)); TRY(dalloc(&c->st1, ee)); TRY(dalloc(&c->lmd, ee));
@@ -763,7 +767,14 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
     * Lagrange matrix hml of the last energy evaluation (exact at the fixed
     * point, where h = <c1|H|c1>).  The overlaps and the constraint below
     * then act on this c2 as they act on the plain one. */
-   if (c->precond && c->hml_valid) {
+   const int pstep = c->precond && c->hml_valid;
+   if (pstep && f32) {
+      /* the FP32 steps of the mixed modes form C1 h in FP32 as well, like
+       * their FFTs and KB products; the update itself is FP64 */
+      TRY(ph_begin(c, PH_PRECOND));
+      TRY(launch_precond_f32(A2, ne, c->hml, K, reinterpret_cast<double *>(C2), dte, c->Kp, st));
+      TRY(ph_end(c, PH_PRECOND));
+   } else if (pstep) {
       TRY(ph_begin(c, PH_PRECOND));
       TallArgs t;
       t.A = A2; t.lda = K; t.X = c->hml; t.xsm = 1; t.ldx = ne; t.Mr = ne; t.Nc = ne; t.K = K;
@@ -796,8 +807,12 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
       if (carry) TRY(launch_reduce_partials_split(c->partial, so, sdd, 2 * ne, c->cur * ne, (c->cur + 1) * ne,
                                                   c->Gm, st));
       else if (!small) TRY(launch_reduce_partials_symm(c->partial, 2 * ne, K, sp, c->Gm, st));
-      TRY(sd_smat(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->s11, c->s12, c->s21,
-                  c->s22, c->bar, st));
+      if (pstep)
+         TRY(sd_smat_precond(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->DP, c->s11,
+                             c->s12, c->s21, c->s22, c->bar, st));
+      else
+         TRY(sd_smat(c->Gm, carry ? c->g11c[c->g11_idx] : nullptr, ne, c->cur, dte, c->s11, c->s12, c->s21,
+                     c->s22, c->bar, st));
       TRY(ph_end(c, PH_GRAMRED));
    }
    /* 12 */
@@ -807,7 +822,11 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
       l.ne = ne; l.s11 = c->s11; l.s12 = c->s12; l.s21 = c->s21; l.s22 = c->s22;
       l.X = c->LX; l.st1 = c->st1; l.lmd = c->lmd; l.wgmax = c->wgmax; l.info = c->info;
       l.bar = c->bar; l.err = c->err;
+      l.itmax = 30;
+      c->lag_itmax = l.itmax;
       TRY(sd_lagrange(l, st));
+      /* the preconditioned step's Lagrange matrix of the unprojected c2 */
+      if (pstep) TRY(sd_precond_lmd(c->lmd, c->DP, ne, dte, st));
    }
    TRY(ph_end(c, PH_LAGRANGE));
    /* <c2f|c2f> of the constrained orbitals, the next step's <c1|c1> */
@@ -977,7 +996,7 @@ static int ctx_results(cpsd_dev *c, double *E, double *deltae, double *deltac, d
    if (err) { cpsd_set_error("Lagrange grid barrier timed out"); return -1; }
    c->last_lag_iters = (int)info[0];
    c->last_lag_adiff = info[1];
-   if (c->last_lag_iters >= 30 && !(info[1] < 1e-15)) printf("ierr=10 adiff=%f\n", info[1]);
+   if (c->last_lag_iters >= c->lag_itmax && !(info[1] < 1e-15)) printf("ierr=10 adiff=%f\n", info[1]);
    for (int i = 0; i < 12; ++i)
       if (!std::isfinite(out[i])) {
          cpsd_set_error("the SD step produced a non-finite %s (%g): the orbitals are lost; restart from "
@@ -1306,7 +1325,7 @@ static int ctx_clear_state(cpsd_dev *c)
       {c->s22, ee * sizeof(double)}, {c->LX, 2 * ee * sizeof(double)}, {c->st1, ee * sizeof(double)},
       {c->lmd, ee * sizeof(double)}, {c->hml, ee * sizeof(double)}, {c->Mh, ee * sizeof(double)},
       {c->g11c[0], ee * sizeof(double)}, {c->g11c[1], ee * sizeof(double)}, {c->g11w, 2 * ee * sizeof(double)},
-      {c->err, 4 * sizeof(int)}};
+      {c->DP, 2 * ee * sizeof(double)}, {c->err, 4 * sizeof(int)}};
    for (auto &e : z)
       if (e.p) CPSD_CHECK(hipMemsetAsync(e.p, 0, e.bytes, st));
    CPSD_CHECK(hipStreamSynchronize(st));

@@ -187,6 +187,10 @@ int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const doub
 int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ldx, int Nc, size_t K,
                        double *C2, size_t ldc, double dte, hipStream_t s);
 int launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+/* the preconditioned step of the FP32 steps: C2 = C1 + Kp o (C2 - C1 + dte
+ * C1 H), the product in FP32 (C1, C2: [ne][K] reals, K a multiple of 128) */
+int launch_precond_f32(const double *C1, int ne, const double *H, size_t K, double *C2, double dte,
+                       const double *Kp, hipStream_t s);
 /* the projection with BF16 projectors Bb and C1 w rounded to BF16, FP32 sums
  * (precision mode 2) */
 int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const double *w, size_t K, int splits,

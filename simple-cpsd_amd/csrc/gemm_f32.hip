@@ -173,11 +173,16 @@ int launch_kbproj_f32(const float *Bf, int M, const double *C, int N, const doub
 /* AKT output reals (k) per workgroup, 8 waves as 2 (columns) x 4 (k), each
  * 64 columns x AKT/4 k = 4 x AKT/64 blocks; the Bs row pitch AKT + 4 puts the
  * 4 rows 4 apart that one read touches 16 banks apart.  WAVES: waves per
- * SIMD the registers are sized for. */
-template<int AKT, int WAVES>
+ * SIMD the registers are sized for.
+ * TA = float: the KB apply, rows of Bf; C2 += dte result.
+ * TA = double: the preconditioned step of the FP32 steps (PRE), rows of the
+ * FP64 orbitals C1 rounded to FP32 as they are staged, X = hml:
+ * C2 = C1 + Kp o (C2 - C1 + dte result), the update in FP64. */
+template<int AKT, int WAVES, typename TA, int PRE>
 __global__ void __launch_bounds__(512, WAVES)
-k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
-              int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per)
+k_kbapply_f32(const TA *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
+              int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per,
+              const double *__restrict__ Kp)
 {
    constexpr int BP = AKT + 4, JB = AKT / 64, WKE = AKT / 4, BH = AKT / 128;
    /* main loop: Bs[2][AM][BP] and Xs[2][AN][XP]; epilogue: T[64][BP], one
@@ -211,8 +216,17 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
 #pragma unroll
       for (int h = 0; h < BH; ++h) {
          const size_t k = k0 + bc + 4 * h;
-         rb[h] = (m < Mr && k < K) ? *reinterpret_cast<const float4 *>(Bf + (size_t)m * ldb + k)
-                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+         if constexpr (sizeof(TA) == 4) {
+            rb[h] = (m < Mr && k < K) ? *reinterpret_cast<const float4 *>(Bf + (size_t)m * ldb + k)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+         } else {
+            rb[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (m < Mr && k < K) {
+               const double2 u = *reinterpret_cast<const double2 *>(Bf + (size_t)m * ldb + k);
+               const double2 v = *reinterpret_cast<const double2 *>(Bf + (size_t)m * ldb + k + 2);
+               rb[h] = make_float4((float)u.x, (float)u.y, (float)v.x, (float)v.y);
+            }
+         }
       }
       const int n = n0 + xn;
 #pragma unroll
@@ -287,6 +301,16 @@ k_kbapply_f32(const float *__restrict__ Bf, size_t ldb, int Mr, const double *__
                b.x += dte * v.z; b.y += dte * v.w;
                p[0] = a;
                p[1] = b;
+            } else if constexpr (PRE) {
+               /* one complex coefficient per lane: k = 2 (k / 2) */
+               const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
+               const double2 c1 = *reinterpret_cast<const double2 *>(
+                  reinterpret_cast<const double *>(Bf) + (size_t)n * ldb + k);
+               const double f = Kp[k >> 1];
+               double2 a = p[0];
+               a.x = c1.x + f * (a.x - c1.x + dte * v.x);
+               a.y = c1.y + f * (a.y - c1.y + dte * v.y);
+               p[0] = a;
             } else {
                const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
                double2 a = p[0];
@@ -310,8 +334,25 @@ int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ld
    const int ntn = (Nc + AN - 1) / AN;
    const long blocks = (long)((K + AKT - 1) / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X, xsm, ldx,
-                      Nc, K, C2, ldc, dte, ntn, per);
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6, float, 0>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X,
+                      xsm, ldx, Nc, K, C2, ldc, dte, ntn, per, nullptr);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* the preconditioned step of the FP32 steps: C2 = C1 + Kp o (C2 - C1 + dte
+ * C1 H) with the product summed in FP32 over the FP64 orbitals rounded as
+ * they are staged (C1, C2: [ne][K] reals; H: ne x ne column-major) */
+int launch_precond_f32(const double *C1, int ne, const double *H, size_t K, double *C2, double dte,
+                       const double *Kp, hipStream_t s)
+{
+   constexpr int AKT = 128;
+   if (K % AKT) { cpsd_set_error("precond_f32: K=%zu not a multiple of %d", K, AKT); return -1; }
+   const int ntn = (ne + AN - 1) / AN;
+   const long blocks = (long)(K / AKT) * ntn;
+   const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6, double, 1>), dim3((unsigned)blocks), dim3(512), 0, s, C1, K, ne, H, 1,
+                      ne, ne, K, C2, K, dte, ntn, per, Kp);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

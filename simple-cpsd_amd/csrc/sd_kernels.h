@@ -24,6 +24,7 @@ struct LagArgs {
    double *info;     /* [iterations, last max|dX|] */
    unsigned *bar;    /* grid-barrier counter, zero at launch (sd_smat clears it) */
    int *err;
+   int itmax = 30;   /* the reference's ITERLMD (ccpsdv3/cpsd.c:19) */
 };
 
 struct EnergyArgs {
@@ -69,6 +70,12 @@ int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double d
  * from the step's Gram (G11 from g11 when non-null) and lmd[i + j ne]:
  * G22 + dte (G21 lmd + lmd^T G12) + dte^2 lmd^T G11 lmd, into out (ne x ne);
  * work holds 2 ne x ne */
+/* the preconditioned step's s matrices, with the step's component along c1
+ * taken out of c2 (DP: 2 ne^2 doubles of work, D first), and lmd <- lmd -
+ * D/dte after the Lagrange iteration */
+int sd_smat_precond(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *DP, double *s11,
+                    double *s12, double *s21, double *s22, unsigned *bar, hipStream_t s);
+int sd_precond_lmd(double *lmd, const double *DP, int ne, double dte, hipStream_t s);
 int sd_gram_next(const double *Gm, const double *g11, int ne, int c1_second, const double *lmd, double dte,
                  double *work, double *out, hipStream_t s);
 int sd_kbcoef(const double *P, int nproj, int ne, const double *pnrm, double scal2, double *X, hipStream_t s);

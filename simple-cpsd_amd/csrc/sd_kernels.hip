@@ -357,6 +357,74 @@ __global__ void k_smat(const double *__restrict__ Gm, const double *__restrict__
    }
 }
 
+/* The preconditioned step's overlaps.  K o R is not orthogonal to c1 where
+ * K varies, so <c1|c2> moves away from the identity and the Lagrange fixed
+ * point above stops converging; the step's component along c1 is taken out
+ * first: c2' = c2 - c1 D with D = <c1|c2> - I, and the s matrices are formed
+ * from
+ *   A12' = A12 - A11 D,   A22' = A22 - D^T A12 - A12^T D + D^T A11 D
+ * (A_ab(i, j) = <a_i|b_j>), in full, with no mirroring.  The constraint then
+ * adds dte c1 (X - D/dte) to the unprojected c2 (k_precond_lmd).  First
+ * kernel: D and P = A11 D, column-major, the sum over k in order; A11 is read
+ * as A11(k, i) (symmetric) so that a wave's loads are consecutive. */
+__global__ void __launch_bounds__(256) k_precond_dp(const double *__restrict__ Gm, const double *__restrict__ g11c,
+                                                    int ne, int c1_second, double *__restrict__ D,
+                                                    double *__restrict__ P)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= ne * ne) return;
+   const int i = idx % ne, j = idx / ne;
+   const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
+   const double *a11 = g11c ? g11c + i : Gm + (size_t)o1 * n2 + o1 + i;
+   const size_t s11 = g11c ? (size_t)ne : (size_t)n2;
+   const double *a12 = Gm + (size_t)o1 * n2 + o2 + j;      /* A12(k, j) = a12[k n2] */
+   double a = 0.0;
+   for (int k = 0; k < ne; ++k) a += a11[k * s11] * (a12[(size_t)k * n2] - (k == j ? 1.0 : 0.0));
+   D[idx] = a12[(size_t)i * n2] - (i == j ? 1.0 : 0.0);
+   P[idx] = a;
+}
+
+/* second kernel: the s matrices of the projected overlaps (column-major, the
+ * layout of k_smat); also clears the Lagrange kernel's barrier counter */
+__global__ void __launch_bounds__(256) k_precond_smat(const double *__restrict__ Gm, const double *__restrict__ g11c,
+                                                      int ne, int c1_second, double dte,
+                                                      const double *__restrict__ D, const double *__restrict__ P,
+                                                      double *__restrict__ s11, double *__restrict__ s12,
+                                                      double *__restrict__ s21, double *__restrict__ s22,
+                                                      unsigned *__restrict__ bar)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (bar && idx == 0) *bar = 0u;
+   if (idx >= ne * ne) return;
+   const int i = idx % ne, j = idx / ne;
+   const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
+   const double *a12 = Gm + (size_t)o1 * n2 + o2;          /* A12(k, c) = a12[k n2 + c] */
+   const double *di = D + (size_t)i * ne, *dj = D + (size_t)j * ne, *pj = P + (size_t)j * ne;
+   double q = 0.0, qt = 0.0, r = 0.0;
+   for (int k = 0; k < ne; ++k) {
+      const double dki = di[k];
+      q += dki * a12[(size_t)k * n2 + j];
+      qt += dj[k] * a12[(size_t)k * n2 + i];
+      r += dki * pj[k];
+   }
+   const double a22 = Gm[(size_t)(o2 + i) * n2 + o2 + j] - q - qt + r;
+   const double a12p = a12[(size_t)i * n2 + j] - P[idx];
+   const double a21p = a12[(size_t)j * n2 + i] - P[(size_t)i * ne + j];
+   const double g11 = g11c ? g11c[(size_t)i * ne + j] : Gm[(size_t)(o1 + i) * n2 + o1 + j];
+   const double d = i == j ? 1.0 : 0.0;
+   s22[idx] = (d - a22) * 0.5 / dte;
+   s21[idx] = (d - a21p) * 0.5;
+   s12[idx] = (d - a12p) * 0.5;
+   s11[idx] = -g11 * 0.5 * dte;
+}
+
+/* lmd <- X - D/dte: the Lagrange matrix of the unprojected c2 */
+__global__ void k_precond_lmd(double *__restrict__ lmd, const double *__restrict__ D, int nn, double dte)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx < nn) lmd[idx] -= D[idx] / dte;
+}
+
 /* ------------------------------------------ 16 x 16 tiles on the MFMA */
 
 /* acc += P Q over k in [kb, kb + 4n) for a 16 x 16 tile on the FP64 4x4x4
@@ -625,7 +693,8 @@ __device__ __forceinline__ void tile_mm(const double *A, const double *B, int ne
 }
 
 /* X <- s22 + s21 X + X s12 + X (s11 X), from X0 = s22, until max|dX| < 1e-15
- * or 30 iterations (ccpsdv3/cpsd.c:439-478, ITERLMD/CONVGLMD at :19-20).
+ * or a.itmax iterations (30 for the reference's step: ccpsdv3/cpsd.c:439-478,
+ * ITERLMD/CONVGLMD at :19-20).
  * Persistent grid of (ne/TS)^2 workgroups (all co-resident) with a bounded
  * spinning grid barrier; info[0] = iterations, info[1] = last max|dX|. */
 template<int TS>
@@ -648,7 +717,7 @@ __global__ void __launch_bounds__(256) k_lagrange(LagArgs a)
       if (r < a.ne && c < a.ne) a.X[r + (size_t)c * a.ne] = a.s22[r + (size_t)c * a.ne];
    }
    lag_barrier(a.bar, ++nbar, nwg, a.err);
-   while (it < 30) {
+   while (it < a.itmax) {
       const double *X = a.X + (size_t)(it & 1) * nn;
       double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
       ++it;
@@ -788,7 +857,7 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
    /* X_0 = s22 */
    if (mine) a.X[e] = s22v;
    lag_barrier(a.bar, ++nbar, nwg, a.err);
-   while (it < 30) {
+   while (it < a.itmax) {
       const double *X = a.X + (size_t)(it & 1) * nn;
       double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
       ++it;
@@ -1025,6 +1094,26 @@ int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double d
 {
    hipLaunchKernelGGL(k_smat, g1((size_t)ne * ne), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, s11, s12, s21,
                       s22, bar);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_smat_precond(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *DP, double *s11,
+                    double *s12, double *s21, double *s22, unsigned *bar, hipStream_t s)
+{
+   const size_t ee = (size_t)ne * ne;
+   hipLaunchKernelGGL(k_precond_dp, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, DP, DP + ee);
+   CPSD_CHECK(hipGetLastError());
+   hipLaunchKernelGGL(k_precond_smat, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, DP, DP + ee, s11, s12,
+                      s21, s22, bar);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+int sd_precond_lmd(double *lmd, const double *DP, int ne, double dte, hipStream_t s)
+{
+   const size_t ee = (size_t)ne * ne;
+   hipLaunchKernelGGL(k_precond_lmd, g1(ee), dim3(256), 0, s, lmd, DP, (int)ee, dte);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

@@ -269,13 +269,16 @@ def main():
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--quick", action="store_true", help="two small training cells, one held-out cell")
+    ap.add_argument("--alphas", default="0.55,0.65,0.75,0.85,1.0",
+                    help="comma-separated step scales tried on the held-out cells, ascending")
     args = ap.parse_args()
+    alphas = sorted(float(a) for a in args.alphas.split(","))
     log = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
     if args.quick:
         model = train(cells=[(8, 14.8, 64, 104), (8, 14.8, 64, 105)], valid=[(8, 14.8, 64, 205)],
-                      iters=args.iters, log=log)
+                      iters=args.iters, alphas=alphas, log=log)
     else:
-        model = train(iters=args.iters, log=log)
+        model = train(iters=args.iters, alphas=alphas, log=log)
     with open(args.out, "w") as f:
         json.dump(model, f, indent=1)
     log("wrote %s (alpha %g)" % (args.out, model["alpha"]))

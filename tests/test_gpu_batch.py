@@ -130,3 +130,58 @@ def test_autotune_keeps_parity_and_is_remembered(gpu):
     assert np.array_equal(np.array(Ea), np.array(Eb))
     a.close()
     b.close()
+
+
+def test_batch64_one_graph_equals_single_cells_and_oracle(gpu):
+    """the batched workload's 64 uneven cells (grids 24^3-64^3, 4-32
+    orbitals; cpsd_amd.water_batch_specs(0)) in one graph per call, FP64:
+    every cell's energies, deltas and orbitals equal the same cell run alone,
+    bit for bit, and the three first 24^3 cells match the oracle (energies
+    1e-9 Ha, c2 1e-10)"""
+    import oracle_lib as O
+    from test_gpu_parity import compare
+    specs = cpsd_amd.water_batch_specs(0)
+    assert len(specs) == 64 and len({(n, 4 * m) for m, L, n in specs}) >= 8
+    say("batch-64: %d size classes" % len({(n, 4 * m) for m, L, n in specs}))
+    cells = [cpsd_amd.water_cell(m, L, n, seed=i) for i, (m, L, n) in enumerate(specs)]
+    psis = [c.random_orbitals(4 * m, seed=100 + i, decay=0.05) for i, (c, (m, L, n)) in
+            enumerate(zip(cells, specs))]
+    calls, inner = 3, 5
+    alone = []
+    for cell, psi, (m, L, n) in zip(cells, psis, specs):
+        d = cpsd_amd.DeviceSD(cell, 4 * m)
+        d.set_orbitals(psi)
+        res = [d.run(inner) for _ in range(calls)]
+        alone.append((res, d.get(c2=True)["c2"]))
+        d.close()
+    say("batch-64: single cells done")
+    devs = []
+    for cell, psi, (m, L, n) in zip(cells, psis, specs):
+        d = cpsd_amd.DeviceSD(cell, 4 * m)
+        d.set_orbitals(psi)
+        devs.append(d)
+    b = cpsd_amd.Batch(devs)
+    got = [b.run(inner) for _ in range(calls)]
+    info = b.info()
+    say("batch-64: %s" % info)
+    assert info["graph_launches"] >= 1 and info["eager"] == 1
+    for i, d in enumerate(devs):
+        for call in range(calls):
+            Er, der, dcr = alone[i][0][call]
+            Eg, deg, dcg = got[call][i]
+            assert np.array_equal(Er, Eg), (i, call)
+            assert der == deg and dcr == dcg, (i, call)
+        assert np.array_equal(alone[i][1], d.get(c2=True)["c2"]), i
+    b.close()
+    small = [i for i, (m, L, n) in enumerate(specs) if n == 24][:3]
+    assert len(small) == 3
+    for i in small:
+        m, L, n = specs[i]
+        cell = cells[i]
+        st = O.State(O.Setup(1, L, n, cell.katm, cell.zv), 4 * m, cell.lmmax, cell.vl, cell.vnl, cell.vnlnrm,
+                     cell.r, psis[i], cell.amass)
+        for call in range(calls):
+            compare(got[call][i][0], st.step(inner)[0])
+        assert np.max(np.abs(devs[i].get(c2=True)["c2"] - st.c2)) < 1e-10
+    for d in devs:
+        d.close()

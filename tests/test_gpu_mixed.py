@@ -206,17 +206,17 @@ def test_bf16_projection_converges_to_fp64(gpu):
     assert out[2] != out[1]
 
 
-@pytest.mark.parametrize("mode,tol_e,tol_r,tol_f", [(1, 1e-6, 1e-6, 1e-6), (2, 2e-5, 3e-4, 2e-3)])
+@pytest.mark.parametrize("mode,tol_e,tol_r,tol_f", [(1, 1e-6, 1e-6, 1e-6)])
 def test_mixed_moving_ions_vs_fp64(gpu, mode, tol_e, tol_r, tol_f):
-    """move = 1 in the mixed modes: 4 waters at 32^3, 16 orbitals and 24
-    projectors (so the FP32 or BF16 projection kernel runs), 4 calls of 5
-    steps against FP64 from the same start.  Every step moves the ions, the
-    FP32 ones with forces from their FP32 products, so positions and forces
-    drift from FP64's.  Measured after 20 steps, with the ions moved about
-    2 bohr: mode 1 4e-8 bohr and 6e-8 Ha/bohr, mode 2 (BF16 projection
-    coefficients in the nonlocal force) 3e-5 bohr and 3e-4 Ha/bohr; the
-    bounds are about ten times those.  A missing or wrong force term moves
-    forces by ~0.1 Ha/bohr."""
+    """move = 1 in mixed mode 1: 4 waters at 32^3, 16 orbitals and 24
+    projectors (so the FP32 projection kernel runs), 4 calls of 5 steps
+    against FP64 from the same start.  Every step moves the ions, the FP32
+    ones with forces from their FP32 products, so positions and forces drift
+    from FP64's.  Measured after 20 steps, with the ions moved about 2 bohr:
+    4e-8 bohr and 6e-8 Ha/bohr; the bounds are about ten times those.  A
+    missing or wrong force term moves forces by ~0.1 Ha/bohr.  Mode 2 (BF16
+    projection coefficients in the nonlocal force) moved forces by 3e-4
+    Ha/bohr and is refused with moving ions (test_gpu_safeguards.py)."""
     say("mode %d with moving ions vs FP64, 4 waters 32^3 ne=16" % mode)
     cell = cpsd_amd.water_cell(4, 14.0, 32, seed=0)
     ne, natom = 16, 12

@@ -129,3 +129,25 @@ def test_precond_converges_to_plain_sd_minimum(gpu):
     say("plain %d steps E %.10f ; precond %d steps E %.10f" % (res["plain"] + res["precond"]))
     assert res["precond"][0] < res["plain"][0] / 2
     assert abs(res["precond"][1] - res["plain"][1]) < 1e-7
+
+
+def test_precond_in_mixed_mode_tracks_fp64(gpu):
+    """mode 1 forms the preconditioned step's c1 h product in FP32 in its
+    FP32 steps (the FFTs and KB products are FP32 there too): the energies
+    stay within 1e-7 |E| of the FP64 preconditioned run (this start is far
+    from converged, E is 450-1700 Ha; measured 1.4e-8 |E|), and the bits
+    differ (the FP32 kernels ran)"""
+    say("precond mixed mode")
+    cell = cpsd_amd.water_cell(2, 12.0, 24, seed=7)
+    psi = cell.random_orbitals(128, seed=3, decay=0.05)
+    K = tpa_factor(cell.tg, 4.0, 1.0)
+    out = []
+    for mode in (0, 1):
+        dev = cpsd_amd.DeviceSD(cell, 128)
+        dev.set_orbitals(psi)
+        dev.set_precision(mode)
+        dev.set_preconditioner(K)
+        out.append(np.array([dev.run(5)[0] for _ in range(4)]))
+        dev.close()
+    assert not np.array_equal(out[0], out[1])
+    assert np.max(np.abs(out[0][:, 0] - out[1][:, 0]) / np.abs(out[0][:, 0])) < 1e-7

@@ -50,6 +50,17 @@ def test_ne256_vs_oracle(gpu):
     run_pair(2, 12.0, 24, 256, steps=1, inner=1)
 
 
+@pytest.mark.parametrize("ne", [384, 512])
+def test_ne384_512_vs_oracle(gpu, ne):
+    """orbital counts above 256: the 32-wide Lagrange kernel (its grid of
+    12 x 12 and 16 x 16 workgroups checked for residency), the generic
+    carried-overlap kernel (the carry runs from the call's second step), and
+    a 2 ne = 768 / 1024 Gram with its split-K partial buffers; a 20^3 grid
+    keeps the oracle's O(ne^2) overlaps under a minute"""
+    say("ne=%d at 20^3" % ne)
+    run_pair(2, 12.0, 20, ne, steps=1, inner=2)
+
+
 def test_ne160_vs_oracle(gpu):
     """160 orbitals: 3 Gram tile rows, the second diagonal pair half empty"""
     say("ne=160 at 32^3")
