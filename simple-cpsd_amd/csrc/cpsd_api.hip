@@ -128,7 +128,7 @@ struct cpsd_dev {
    /* <c1|c1> of the next step, carried from this step's overlaps and lmd
     * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
    double *g11c[2], *g11w;
-   double *DP;          /* preconditioned step: D = <c1|c2> - I and A11 D */
+   double *DP;          /* preconditioned step: D = <c1|c2> - I, A11 D and D^T <c1|c2> */
    int g11_idx, g11_valid;
    /* preconditioned step (cpsd_dev_set_preconditioner): one factor per
     * packed coefficient; the steps of a call use it when hml holds the
@@ -523,7 +523,7 @@ static int ctx_build(cpsd_dev *c, int nfft, int ne, int nion, const int *katm, c
    if (ne % 128 == 0 && ne <= 512) {
       TRY(dalloc(&c->g11c[0], ee)); TRY(dalloc(&c->g11c[1], ee)); TRY(dalloc(&c->g11w, 2 * ee));
    }
-   TRY(dalloc(&c->DP, 2 * ee));
+   TRY(dalloc(&c->DP, 3 * ee));
    TRY(dalloc(&c->P, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->Xkb, (size_t)(nproj > 0 ? nproj : 1) * ne));
    TRY(dalloc(&c->s11, ee)); TRY(dalloc(&c->s12, ee)); TRY(dalloc(&c->s21, ee)); TRY(dalloc(&c->s22, ee));
@@ -1325,7 +1325,7 @@ static int ctx_clear_state(cpsd_dev *c)
       {c->s22, ee * sizeof(double)}, {c->LX, 2 * ee * sizeof(double)}, {c->st1, ee * sizeof(double)},
       {c->lmd, ee * sizeof(double)}, {c->hml, ee * sizeof(double)}, {c->Mh, ee * sizeof(double)},
       {c->g11c[0], ee * sizeof(double)}, {c->g11c[1], ee * sizeof(double)}, {c->g11w, 2 * ee * sizeof(double)},
-      {c->DP, 2 * ee * sizeof(double)}, {c->err, 4 * sizeof(int)}};
+      {c->DP, 3 * ee * sizeof(double)}, {c->err, 4 * sizeof(int)}};
    for (auto &e : z)
       if (e.p) CPSD_CHECK(hipMemsetAsync(e.p, 0, e.bytes, st));
    CPSD_CHECK(hipStreamSynchronize(st));

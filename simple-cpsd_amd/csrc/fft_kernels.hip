@@ -345,7 +345,12 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
    using R = typename Cx<V>::R;
    constexpr int NH = N / 2 + 1, M = N / 2, NT = C::PLANE_THREADS;
    V *sm = reinterpret_cast<V *>(plane_lds);
-   const int z = blockIdx.x;
+   /* the V psi pass runs the orbitals of one z plane in consecutive
+    * workgroups (blockIdx.x the orbital), which the 8 XCDs take round-robin,
+    * so each XCD's L2 fetches the plane's potential once for its share of
+    * the orbitals instead of once per orbital; the other modes keep
+    * blockIdx.x = z */
+   const int z = MODE == PM_VPSI ? blockIdx.y : blockIdx.x;
    const size_t plane = (size_t)N * NH;
    /* twiddles live in LDS after the plane; every mode passes a barrier
     * before its first transform */
@@ -408,7 +413,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
          }
       }
    } else if constexpr (MODE == PM_VPSI) {
-      const int b = blockIdx.y;
+      const int b = blockIdx.x;
       V *Tp = Tb + ((size_t)b * N + z) * plane;
       /* this plane's potential, fetched before the transforms need it */
       constexpr int EPV = (N * M + NT - 1) / NT;
@@ -536,8 +541,8 @@ static int plane_launch(const PlaneArgs &a, int nblocks, hipStream_t s)
    const size_t lds = plane_lds_bytes<N, V>();
    static std::atomic<unsigned long long> attr_set{0};
    if (dyn_lds_once((const void *)k_plane<N, MODE, V>, (int)lds, attr_set) != 0) return -1;
-   hipLaunchKernelGGL((k_plane<N, MODE, V>), dim3(N, nblocks), dim3(GridCfg<N>::PLANE_THREADS),
-                      lds, s, a);
+   const dim3 grid = MODE == PM_VPSI ? dim3(nblocks, N) : dim3(N, nblocks);
+   hipLaunchKernelGGL((k_plane<N, MODE, V>), grid, dim3(GridCfg<N>::PLANE_THREADS), lds, s, a);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }

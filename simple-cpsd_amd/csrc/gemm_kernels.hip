@@ -123,13 +123,17 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    /* a single diagonal tile's second half starts khalf after the first */
    const size_t khalf = (SELF && selfdd && ke > kb) ? ((ke - kb) / KT + 1) / 2 * KT : 0;
    const size_t kend = (SELF && selfdd) ? min(ke, kb + khalf) : ke;
-   /* S[buf][0] holds the A tile, S[buf][1] the B tile; the same memory
-    * stages the output tile for the epilogue.  Row pitch KT + 2: the 16 rows
-    * x 2 k of a half-wave's operand read fall in distinct bank pairs, and
-    * rows stay 16-byte aligned for the stores. */
-   __shared__ double S[2][2][TT][KT + 2];
+   /* S[buf][0] holds the A tile, S[buf][1] the B tile, rows of KT doubles
+    * with the k index XOR-swizzled by the row: element (row, k) sits at
+    * column k ^ (row & 15).  An operand read takes 16 rows at one k per
+    * 16-lane group; unswizzled (or with any even pad) those rows would share
+    * bank pairs, swizzled they cover 16 distinct ones.  The loader's
+    * 16-byte stores keep a (k, k+1) pair together: column pair
+    * lq ^ (row & 14), its two values swapped when row is odd.  The same
+    * memory stages the output tile for the epilogue. */
+   __shared__ double Sbuf[2 * 2 * TT * KT > TT * TP ? 2 * 2 * TT * KT : TT * TP];
+   double (*const S)[2][TT][KT] = reinterpret_cast<double (*)[2][TT][KT]>(Sbuf);
    __shared__ double Ws[2][SELF ? 2 : 1][KT];   /* weights of the A rows ([0]) and B rows ([1]) */
-   static_assert(sizeof(S) >= sizeof(double) * TT * TP, "staging tile does not fit");
    /* the wave index is uniform: read it as a scalar */
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wm = wave & 1, wn = wave >> 1;
@@ -189,11 +193,14 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
          if (tid < KT / 2) rw = *reinterpret_cast<const double2 *>(w + k0 + 2 * tid);
       }
    };
+   /* rows lr + 32 h all have row & 15 == lr & 15 */
+   const int sq = lq ^ (lr & 14);
+   const bool swp = lr & 1;
    auto lstore = [&](int bf) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-         *reinterpret_cast<double2 *>(&S[bf][0][lr + 32 * h][lq]) = ra[h];
-         *reinterpret_cast<double2 *>(&S[bf][1][lr + 32 * h][lq]) = rb[h];
+         *reinterpret_cast<double2 *>(&S[bf][0][lr + 32 * h][sq]) = swp ? make_double2(ra[h].y, ra[h].x) : ra[h];
+         *reinterpret_cast<double2 *>(&S[bf][1][lr + 32 * h][sq]) = swp ? make_double2(rb[h].y, rb[h].x) : rb[h];
       }
       if constexpr (SELF) {
          if (tid < KT) {
@@ -217,28 +224,28 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    auto blk = [&](int b, int r) -> double & { return b < 8 ? acc[(b & 7) >> 1][b & 1][r] : accx[b & 1][r]; };
    auto mma_diag = [&](auto role, int bf, int ks) {
       constexpr int DM = decltype(role)::value;
-      const int kk = 4 * ks + (lane >> 4);
+      const int kk = 4 * ks + (lane >> 4), kx = kk ^ (lane & 15);
       const double wk = Ws[bf][SELF ? hs : 0][kk];
       double a[4], bv[4][4];
 #pragma unroll
-      for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kk] * wk;
+      for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kx] * wk;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) rotations(S[bf][hs][16 * (dcol0 + j) + (lane & 15)][kk], bv[j]);
+      for (int j = 0; j < 4; ++j) rotations(S[bf][hs][16 * (dcol0 + j) + (lane & 15)][kx], bv[j]);
 #pragma unroll
       for (int b = 0; b < dg_nblk(DM); ++b)
 #pragma unroll
          for (int r = 0; r < 4; ++r) blk(b, r) = mfma4(a[dg_ai(DM, b)], bv[dg_bj(DM, b)][r], blk(b, r));
    };
    auto mma_full = [&](int bf, int ks) {
-      const int kk = 4 * ks + (lane >> 4);
+      const int kk = 4 * ks + (lane >> 4), kx = kk ^ (lane & 15);
       const double wk = Ws[bf][0][kk];
       double a[BI];
 #pragma unroll
-      for (int i = 0; i < BI; ++i) a[i] = S[bf][0][(TMR / 2) * wm + 16 * i + (lane & 15)][kk] * wk;
+      for (int i = 0; i < BI; ++i) a[i] = S[bf][0][(TMR / 2) * wm + 16 * i + (lane & 15)][kx] * wk;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
          double b[4];
-         rotations(S[bf][1][32 * wn + 16 * j + (lane & 15)][kk], b);
+         rotations(S[bf][1][32 * wn + 16 * j + (lane & 15)][kx], b);
 #pragma unroll
          for (int i = 0; i < BI; ++i)
 #pragma unroll
@@ -275,7 +282,7 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    else kloop(std::integral_constant<int, 2>{});
 
    /* stage the tile in LDS, then write whole 1 KB rows of the partial */
-   double (*T)[TP] = reinterpret_cast<double (*)[TP]>(&S[0][0][0][0]);
+   double (*T)[TP] = reinterpret_cast<double (*)[TP]>(Sbuf);
    const int q = lane >> 4, ab = (lane >> 2) & 3, bb = lane & 3;
    auto put = [&](int m, int n, int ml, int zz) {
       double *dst = partial + ((size_t)zz * M + m) * N + n;

@@ -71,7 +71,7 @@ int sd_smat(const double *Gm, const double *g11, int ne, int c1_second, double d
  * G22 + dte (G21 lmd + lmd^T G12) + dte^2 lmd^T G11 lmd, into out (ne x ne);
  * work holds 2 ne x ne */
 /* the preconditioned step's s matrices, with the step's component along c1
- * taken out of c2 (DP: 2 ne^2 doubles of work, D first), and lmd <- lmd -
+ * taken out of c2 (DP: 3 ne^2 doubles of work, D first), and lmd <- lmd -
  * D/dte after the Lagrange iteration */
 int sd_smat_precond(const double *Gm, const double *g11, int ne, int c1_second, double dte, double *DP, double *s11,
                     double *s12, double *s21, double *s22, unsigned *bar, hipStream_t s);

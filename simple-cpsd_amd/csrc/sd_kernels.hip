@@ -362,14 +362,15 @@ __global__ void k_smat(const double *__restrict__ Gm, const double *__restrict__
  * point above stops converging; the step's component along c1 is taken out
  * first: c2' = c2 - c1 D with D = <c1|c2> - I, and the s matrices are formed
  * from
- *   A12' = A12 - A11 D,   A22' = A22 - D^T A12 - A12^T D + D^T A11 D
+ *   A12' = A12 - A11 D,   A22' = A22 - Q - Q^T + D^T A11 D,   Q = D^T A12
  * (A_ab(i, j) = <a_i|b_j>), in full, with no mirroring.  The constraint then
  * adds dte c1 (X - D/dte) to the unprojected c2 (k_precond_lmd).  First
- * kernel: D and P = A11 D, column-major, the sum over k in order; A11 is read
- * as A11(k, i) (symmetric) so that a wave's loads are consecutive. */
-__global__ void __launch_bounds__(256) k_precond_dp(const double *__restrict__ Gm, const double *__restrict__ g11c,
-                                                    int ne, int c1_second, double *__restrict__ D,
-                                                    double *__restrict__ P)
+ * kernel: D, P = A11 D and Q, column-major, each sum over k in order.  A
+ * wave holds 64 consecutive i at one j: A11(k, i) (symmetric) and
+ * A12(k, i) are consecutive in memory, A12(k, j) is one address. */
+__global__ void __launch_bounds__(256) k_precond_dpq(const double *__restrict__ Gm, const double *__restrict__ g11c,
+                                                     int ne, int c1_second, double *__restrict__ D,
+                                                     double *__restrict__ P, double *__restrict__ Q)
 {
    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
    if (idx >= ne * ne) return;
@@ -377,18 +378,25 @@ __global__ void __launch_bounds__(256) k_precond_dp(const double *__restrict__ G
    const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
    const double *a11 = g11c ? g11c + i : Gm + (size_t)o1 * n2 + o1 + i;
    const size_t s11 = g11c ? (size_t)ne : (size_t)n2;
-   const double *a12 = Gm + (size_t)o1 * n2 + o2 + j;      /* A12(k, j) = a12[k n2] */
-   double a = 0.0;
-   for (int k = 0; k < ne; ++k) a += a11[k * s11] * (a12[(size_t)k * n2] - (k == j ? 1.0 : 0.0));
-   D[idx] = a12[(size_t)i * n2] - (i == j ? 1.0 : 0.0);
-   P[idx] = a;
+   const double *a12 = Gm + (size_t)o1 * n2 + o2;      /* A12(k, c) = a12[k n2 + c] */
+   double p = 0.0, q = 0.0;
+   for (int k = 0; k < ne; ++k) {
+      const double dkj = a12[(size_t)k * n2 + j] - (k == j ? 1.0 : 0.0);
+      const double dki = a12[(size_t)k * n2 + i] - (k == i ? 1.0 : 0.0);
+      p += a11[k * s11] * dkj;
+      q += dki * a12[(size_t)k * n2 + j];
+   }
+   D[idx] = a12[(size_t)i * n2 + j] - (i == j ? 1.0 : 0.0);
+   P[idx] = p;
+   Q[idx] = q;
 }
 
-/* second kernel: the s matrices of the projected overlaps (column-major, the
- * layout of k_smat); also clears the Lagrange kernel's barrier counter */
+/* second kernel: R = D^T P and the s matrices of the projected overlaps
+ * (column-major, the layout of k_smat); also clears the Lagrange kernel's
+ * barrier counter */
 __global__ void __launch_bounds__(256) k_precond_smat(const double *__restrict__ Gm, const double *__restrict__ g11c,
                                                       int ne, int c1_second, double dte,
-                                                      const double *__restrict__ D, const double *__restrict__ P,
+                                                      const double *__restrict__ P, const double *__restrict__ Q,
                                                       double *__restrict__ s11, double *__restrict__ s12,
                                                       double *__restrict__ s21, double *__restrict__ s22,
                                                       unsigned *__restrict__ bar)
@@ -398,16 +406,11 @@ __global__ void __launch_bounds__(256) k_precond_smat(const double *__restrict__
    if (idx >= ne * ne) return;
    const int i = idx % ne, j = idx / ne;
    const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
-   const double *a12 = Gm + (size_t)o1 * n2 + o2;          /* A12(k, c) = a12[k n2 + c] */
-   const double *di = D + (size_t)i * ne, *dj = D + (size_t)j * ne, *pj = P + (size_t)j * ne;
-   double q = 0.0, qt = 0.0, r = 0.0;
-   for (int k = 0; k < ne; ++k) {
-      const double dki = di[k];
-      q += dki * a12[(size_t)k * n2 + j];
-      qt += dj[k] * a12[(size_t)k * n2 + i];
-      r += dki * pj[k];
-   }
-   const double a22 = Gm[(size_t)(o2 + i) * n2 + o2 + j] - q - qt + r;
+   const double *a12 = Gm + (size_t)o1 * n2 + o2;
+   const double *pj = P + (size_t)j * ne;
+   double r = 0.0;
+   for (int k = 0; k < ne; ++k) r += (a12[(size_t)k * n2 + i] - (k == i ? 1.0 : 0.0)) * pj[k];
+   const double a22 = Gm[(size_t)(o2 + i) * n2 + o2 + j] - Q[idx] - Q[(size_t)i * ne + j] + r;
    const double a12p = a12[(size_t)i * n2 + j] - P[idx];
    const double a21p = a12[(size_t)j * n2 + i] - P[(size_t)i * ne + j];
    const double g11 = g11c ? g11c[(size_t)i * ne + j] : Gm[(size_t)(o1 + i) * n2 + o1 + j];
@@ -1102,10 +1105,10 @@ int sd_smat_precond(const double *Gm, const double *g11, int ne, int c1_second, 
                     double *s12, double *s21, double *s22, unsigned *bar, hipStream_t s)
 {
    const size_t ee = (size_t)ne * ne;
-   hipLaunchKernelGGL(k_precond_dp, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, DP, DP + ee);
+   hipLaunchKernelGGL(k_precond_dpq, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, DP, DP + ee, DP + 2 * ee);
    CPSD_CHECK(hipGetLastError());
-   hipLaunchKernelGGL(k_precond_smat, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, DP, DP + ee, s11, s12,
-                      s21, s22, bar);
+   hipLaunchKernelGGL(k_precond_smat, g1(ee), dim3(256), 0, s, Gm, g11, ne, c1_second, dte, DP + ee, DP + 2 * ee,
+                      s11, s12, s21, s22, bar);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
