@@ -325,15 +325,29 @@ __device__ __forceinline__ void slot_xy(int s, int m, int &y, int &x0)
 
 /* ------------------------------------------------------- plane kernel */
 
+/* waves per SIMD the LDS lets a plane kernel hold: whole workgroups of the
+ * plane plus its twiddles in 160 KB */
+template<int N, typename V>
+constexpr int plane_lds_waves()
+{
+   constexpr long bytes = ((long)N * (N / 2 + 1) + N) * (long)sizeof(V);
+   constexpr int wg = (int)(160L * 1024 / bytes);
+   return wg * (GridCfg<N>::PLANE_THREADS / 64) / 4;
+}
+
 /* waves per SIMD the plane kernel's registers are sized for: 4 for the
  * FP64 density pass at N = 96, DENS_WAVES_F32 / VPSI_WAVES_F32 for the FP32
- * orbital passes at N >= 80 */
+ * orbital passes at N >= 80, but never more than the LDS admits (at
+ * N = 128 the FP32 planes fit two workgroups, 4 waves per SIMD: sizing the
+ * registers for 6 only spilled them) */
 template<int N, int MODE, typename V>
 constexpr int plane_min_waves()
 {
-   return (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
-        : (sizeof(V) == 8 && N >= 80 && MODE == PM_VPSI) ? VPSI_WAVES_F32
-        : (N == 96 && MODE == PM_DENSITY) ? 4 : 1;
+   constexpr int lw = plane_lds_waves<N, V>() > 0 ? plane_lds_waves<N, V>() : 1;
+   constexpr int want = (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
+                      : (sizeof(V) == 8 && N >= 80 && MODE == PM_VPSI) ? VPSI_WAVES_F32
+                      : (N == 96 && MODE == PM_DENSITY) ? 4 : 1;
+   return want < lw ? want : lw;
 }
 
 extern __shared__ __align__(16) unsigned char plane_lds[];
@@ -345,11 +359,10 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
    using R = typename Cx<V>::R;
    constexpr int NH = N / 2 + 1, M = N / 2, NT = C::PLANE_THREADS;
    V *sm = reinterpret_cast<V *>(plane_lds);
-   /* the V psi pass runs the orbitals of one z plane in consecutive
-    * workgroups (blockIdx.x the orbital), which the 8 XCDs take round-robin,
-    * so each XCD's L2 fetches the plane's potential once for its share of
-    * the orbitals instead of once per orbital; the other modes keep
-    * blockIdx.x = z */
+   /* the V psi pass runs the orbital groups of one z plane in consecutive
+    * workgroups (blockIdx.x the group), which the 8 XCDs take round-robin,
+    * so the plane's potential is fetched by few L2s at about the same time;
+    * the other modes keep blockIdx.x = z */
    const int z = MODE == PM_VPSI ? blockIdx.y : blockIdx.x;
    const size_t plane = (size_t)N * NH;
    /* twiddles live in LDS after the plane; every mode passes a barrier
@@ -413,10 +426,18 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
          }
       }
    } else if constexpr (MODE == PM_VPSI) {
-      const int b = blockIdx.x;
-      V *Tp = Tb + ((size_t)b * N + z) * plane;
-      /* this plane's potential, fetched before the transforms need it */
+      /* orbitals [bb0, bb1) of plane z; the potential is loaded once for
+       * all of them.  When an FP64 plane fills more than half the LDS (one
+       * workgroup per CU, N >= 112), the next orbital's plane is fetched
+       * into registers while the current one is transformed: its HBM reads
+       * overlap the LDS transforms that would otherwise leave the CU idle
+       * between planes.  Smaller planes leave room for two workgroups per
+       * CU, which overlap each other; they take one orbital each. */
+      constexpr bool PF = sizeof(V) == 16 && (size_t)N * NH * sizeof(V) > 80 * 1024;
+      constexpr int EPT = (N * NH + NT - 1) / NT;
       constexpr int EPV = (N * M + NT - 1) / NT;
+      const int nb = PF && a.nb > 0 ? a.nb : 1;
+      const int bb0 = blockIdx.x * nb, bb1 = min(bb0 + nb, a.nbatch);
       V vv[EPV];
 #pragma unroll
       for (int q = 0; q < EPV; ++q) {
@@ -428,22 +449,55 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
             vv[q] = cvt<V>(*reinterpret_cast<const double2 *>(a.V + ((size_t)z * N + y) * N + x0));
          }
       }
-      load_plane<N>(sm, Tp, a.nhload);
-      inverse_yx<N>(sm, tws);
+      auto apply_v = [&]() {
 #pragma unroll
-      for (int q = 0; q < EPV; ++q) {
-         const int it = threadIdx.x + NT * q;
-         if (it < N * M) {
-            const int s = it / M, m = it % M;
-            V v = sm[s * NH + m];
-            v.x *= vv[q].x;
-            v.y *= vv[q].y;
-            sm[s * NH + m] = v;
+         for (int q = 0; q < EPV; ++q) {
+            const int it = threadIdx.x + NT * q;
+            if (it < N * M) {
+               const int s = it / M, m = it % M;
+               V v = sm[s * NH + m];
+               v.x *= vv[q].x;
+               v.y *= vv[q].y;
+               sm[s * NH + m] = v;
+            }
          }
+         __syncthreads();
+      };
+      if constexpr (PF) {
+         V nx[EPT];
+         auto fetch = [&](int b) {
+            const V *Tq = Tb + ((size_t)b * N + z) * plane;
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+               const int e = threadIdx.x + NT * q;
+               nx[q] = (e < N * NH && e % NH < a.nhload) ? Tq[e] : Cx<V>::mk(0, 0);
+            }
+         };
+         if (bb0 < bb1) fetch(bb0);
+         for (int b = bb0; b < bb1; ++b) {
+            V *Tp = Tb + ((size_t)b * N + z) * plane;
+            /* the previous orbital's write-out has read the LDS plane */
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+               const int e = threadIdx.x + NT * q;
+               if (e < N * NH) sm[e] = nx[q];
+            }
+            __syncthreads();
+            if (b + 1 < bb1) fetch(b + 1);
+            inverse_yx<N>(sm, tws);
+            apply_v();
+            forward_xy<N>(sm, tws);
+            for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
+         }
+      } else {
+         V *Tp = Tb + ((size_t)bb0 * N + z) * plane;
+         load_plane<N>(sm, Tp, a.nhload);
+         inverse_yx<N>(sm, tws);
+         apply_v();
+         forward_xy<N>(sm, tws);
+         for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
       }
-      __syncthreads();
-      forward_xy<N>(sm, tws);
-      for (int e = threadIdx.x; e < N * NH; e += blockDim.x) Tp[e] = sm[e];
    } else {  /* PM_R2C_DN, PM_R2C_IN */
       const int b = blockIdx.y;
       for (int it = threadIdx.x; it < N * M; it += blockDim.x) {
