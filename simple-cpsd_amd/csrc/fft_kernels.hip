@@ -37,6 +37,12 @@
 #ifndef ZPASS_WAVES_F32
 #define ZPASS_WAVES_F32 4
 #endif
+/* waves per SIMD the FP64 G -> r z pass is sized for: at N = 128 two (one
+ * workgroup per CU) without spills ran 11% faster than four with 84 bytes
+ * per lane spilled */
+#ifndef ZPASS_G2R_WAVES_F64
+#define ZPASS_G2R_WAVES_F64(N) ((N) >= 128 ? 2 : 4)
+#endif
 
 /* ------------------------------------------------------------ z passes */
 
@@ -59,7 +65,7 @@ __device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*r
  * b is transformed and written, so each workgroup keeps one column block of
  * loads in flight */
 template<int N, int SRC, typename V>
-__global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F32 : 4) k_zpass_g2r(ZArgs a)
+__global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F32 : ZPASS_G2R_WAVES_F64(N)) k_zpass_g2r(ZArgs a)
 {
    using C = GridCfg<N>;
    constexpr int NH = N / 2 + 1, KXB = C::KXB, ZT = C::ZT;

@@ -21,6 +21,11 @@
  */
 #include "cpsd_internal.h"
 
+/* waves per SIMD the FP32 apply kernels are sized for */
+#ifndef KBAPPLY_WAVES
+#define KBAPPLY_WAVES 6
+#endif
+
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma16(float a, float b, f4 c)
@@ -334,7 +339,7 @@ int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ld
    const int ntn = (Nc + AN - 1) / AN;
    const long blocks = (long)((K + AKT - 1) / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6, float, 0>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X,
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, float, 0>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X,
                       xsm, ldx, Nc, K, C2, ldc, dte, ntn, per, nullptr);
    CPSD_CHECK(hipGetLastError());
    return 0;
@@ -351,7 +356,7 @@ int launch_precond_f32(const double *C1, int ne, const double *H, size_t K, doub
    const int ntn = (ne + AN - 1) / AN;
    const long blocks = (long)(K / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL((k_kbapply_f32<AKT, 6, double, 1>), dim3((unsigned)blocks), dim3(512), 0, s, C1, K, ne, H, 1,
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, double, 1>), dim3((unsigned)blocks), dim3(512), 0, s, C1, K, ne, H, 1,
                       ne, ne, K, C2, K, dte, ntn, per, Kp);
    CPSD_CHECK(hipGetLastError());
    return 0;

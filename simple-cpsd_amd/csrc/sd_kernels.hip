@@ -907,6 +907,154 @@ __global__ void __launch_bounds__(256) k_lagrange_res(LagArgs a)
 }
 
 
+/* 128 < ne <= NM: the kernel above, with the panels streamed through three
+ * LDS slots instead of kept (six resident panels of 256 would need 198 KB).
+ * An iteration loads s11 rows and X columns, forms st1 = s11 X, passes the
+ * grid barrier, then forms s21 X (X columns still in their slot), loads the X
+ * rows and s12 columns for X s12, and the st1 columns for X st1.  Panels are
+ * rows of NM doubles with the k index XOR-swizzled by the row (element
+ * (r, k) at column k ^ (r & 15)): a 16-lane group reads 16 rows at one k,
+ * which then fall on 16 distinct bank pairs. */
+template<int LP>
+__device__ __forceinline__ void tile16_mfma_swz(double (*Pr)[LP], double (*Qc)[LP], int kb, int ke, int lane,
+                                                double (&acc)[4])
+{
+   const int sw = lane & 15;
+   for (int m0 = kb; m0 < ke; m0 += 4) {
+      const int m = (m0 + (lane >> 4)) ^ sw;
+      const double pa = Pr[sw][m];
+      double bq[4];
+      rotations(Qc[sw][m], bq);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma4(pa, bq[t], acc[t]);
+   }
+}
+
+template<int NM>
+__global__ void __launch_bounds__(256) k_lagrange_str(LagArgs a)
+{
+   constexpr int TS = 16;
+   extern __shared__ double lpan[];
+   double (*slA)[NM] = reinterpret_cast<double (*)[NM]>(lpan);
+   double (*slB)[NM] = slA + TS;
+   double (*slC)[NM] = slB + TS;
+   __shared__ double part[4][TS][TS + 1];
+   __shared__ double red[256];
+   const int ne = a.ne, nt = (ne + TS - 1) / TS;
+   const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+   const int nwg = gridDim.x;
+   const size_t nn = (size_t)ne * ne;
+   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+   const int kp = (ne + 15) & ~15, kq = kp / 4, kb = wave * kq, ke = kb + kq;
+   const int s1 = rot_step(lane);
+   const int rl = tid % TS, cl = tid / TS, r = ti * TS + rl, c = tj * TS + cl;
+   const bool mine = r < ne && c < ne;
+   const size_t e = r + (size_t)c * ne;
+   constexpr int LQ = TS * NM / 256;
+   static_assert(TS * NM % 256 == 0, "panel loads cover TS x NM in whole rounds");
+   /* R[rr][k ^ rr] = M(ti*TS + rr, k), k < ne */
+   auto load_rows = [&](const double *M, double (*R)[NM]) {
+      double v[LQ];
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int x = tid + 256 * u;
+         const int rr = x % TS, k = x / TS, row = ti * TS + rr;
+         v[u] = (row < ne && k < ne) ? M[row + (size_t)k * ne] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int x = tid + 256 * u;
+         const int rr = x % TS, k = x / TS;
+         if (k < ne) R[rr][k ^ rr] = v[u];
+      }
+   };
+   /* Cp[cc][k ^ cc] = M(k, tj*TS + cc), k < ne */
+   auto load_cols = [&](const double *M, double (*Cp)[NM]) {
+      double v[LQ];
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int x = tid + 256 * u;
+         const int k = x % NM, cc = x / NM, col = tj * TS + cc;
+         v[u] = (col < ne && k < ne) ? M[k + (size_t)col * ne] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < LQ; ++u) {
+         const int x = tid + 256 * u;
+         const int k = x % NM, cc = x / NM;
+         if (k < ne) Cp[cc][k ^ cc] = v[u];
+      }
+   };
+   auto tile = [&](double (*Pr)[NM], double (*Qc)[NM], double (&acc)[4]) {
+      tile16_mfma_swz<NM>(Pr, Qc, kb, ke, lane, acc);
+   };
+   auto reduce = [&](const double (&acc)[4]) { return tile16_reduce(part, acc, wave, lane, s1, rl, cl); };
+
+   /* the K padding k in [ne, kp) of every slot stays zero (the loads write
+    * k < ne only; the swizzle keeps each 16-wide block of k in place) */
+   for (int x = tid; x < 3 * TS * (kp - ne); x += 256) {
+      const int sl = x / (TS * (kp - ne)), y = x % (TS * (kp - ne));
+      const int rr = y / (kp - ne), k = ne + y % (kp - ne);
+      (slA + sl * TS)[rr][k ^ rr] = 0.0;
+   }
+   unsigned nbar = 0;
+   int it = 0;
+   double gmax = 0.0;
+   const double s22v = mine ? a.s22[e] : 0.0;
+   if (mine) a.X[e] = s22v;
+   lag_barrier(a.bar, ++nbar, nwg, a.err);
+   while (it < a.itmax) {
+      const double *X = a.X + (size_t)(it & 1) * nn;
+      double *Xn = a.X + (size_t)((it + 1) & 1) * nn;
+      ++it;
+      /* st1 = s11 X */
+      load_rows(a.s11, slA);
+      load_cols(X, slB);
+      __syncthreads();
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      tile(slA, slB, acc);
+      const double st = reduce(acc);
+      if (mine) a.st1[e] = st;
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
+      /* X_new = s22 + s21 X + X s12 + X st1 */
+      load_rows(a.s21, slA);
+      __syncthreads();
+      acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+      tile(slA, slB, acc);
+      __syncthreads();
+      load_rows(X, slA);
+      load_cols(a.s12, slC);
+      __syncthreads();
+      tile(slA, slC, acc);
+      __syncthreads();
+      load_cols(a.st1, slC);
+      __syncthreads();
+      tile(slA, slC, acc);
+      const double v = s22v + reduce(acc);
+      double lmax = 0.0;
+      if (mine) {
+         Xn[e] = v;
+         lmax = fabs(v - slB[cl][r ^ cl]);   /* X(r, c) */
+      }
+      red[tid] = lmax;
+      __syncthreads();
+      for (int s2 = 128; s2 > 0; s2 >>= 1) {
+         if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
+         __syncthreads();
+      }
+      if (tid == 0) a.wgmax[blockIdx.x] = red[0];
+      lag_barrier(a.bar, ++nbar, nwg, a.err);
+      gmax = grid_max(a.wgmax, nwg, red);
+      if (gmax < 1e-15) break;
+      if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+   }
+   const double *Xf = a.X + (size_t)(it & 1) * nn;
+   if (mine) a.lmd[e] = Xf[e];
+   if (blockIdx.x == 0 && tid == 0) {
+      a.info[0] = (double)it;
+      a.info[1] = gmax;
+   }
+}
+
 /* ------------------------------------------------------ energy assembly */
 
 /* one workgroup of 256 threads: combine the partial sums into E[0..9],
@@ -1185,11 +1333,14 @@ int sd_lagrange(const LagArgs &a, hipStream_t s)
       hipLaunchKernelGGL((k_lagrange_res<128>), dim3(nt * nt), dim3(256), lds, s, a);
    } else if (a.ne <= 256) {
       const int nt = (a.ne + 15) / 16;
+      const int lds = 3 * 16 * 256 * (int)sizeof(double);
+      static std::atomic<unsigned long long> attr{0};
+      if (dyn_lds_once((const void *)k_lagrange_str<256>, lds, attr) != 0) return -1;
       if (nt * nt > checked[1].load()) {
-         if (lagrange_grid_fits((const void *)k_lagrange<16>, nt * nt, 0) != 0) return -1;
+         if (lagrange_grid_fits((const void *)k_lagrange_str<256>, nt * nt, lds) != 0) return -1;
          checked[1].store(nt * nt);
       }
-      hipLaunchKernelGGL(k_lagrange<16>, dim3(nt * nt), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_lagrange_str<256>), dim3(nt * nt), dim3(256), lds, s, a);
    } else if (a.ne <= 512) {
       const int nt = (a.ne + 31) / 32;
       if (nt * nt > checked[2].load()) {
