@@ -752,8 +752,14 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
       TRY(ph_end(c, PH_FORCE));
    }
    /* 10 */
+   const int pstep = c->precond && c->hml_valid;
+   /* an FP32 preconditioned step does its KB apply inside the preconditioned
+    * update (10b): one pass over C2 */
+   const int fused = pstep && f32 && c->nproj > 0;
    TRY(ph_begin(c, PH_KBAPPLY));
-   if (f32 && c->nproj > 0) {
+   if (fused) {
+      /* nothing here */
+   } else if (f32 && c->nproj > 0) {
       /* C2 += dte B X with FP32 products (never the last step) */
       TRY(launch_kbapply_f32(c->Bf, c->nproj, c->Xkb, 1, c->nproj, ne, K, reinterpret_cast<double *>(C2), K,
                              dte, st));
@@ -772,10 +778,15 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
     * Lagrange matrix hml of the last energy evaluation (exact at the fixed
     * point, where h = <c1|H|c1>).  The overlaps and the constraint below
     * then act on this c2 as they act on the plain one. */
-   const int pstep = c->precond && c->hml_valid;
-   if (pstep && f32) {
+   if (fused) {
       /* the FP32 steps of the mixed modes form C1 h in FP32 as well, like
-       * their FFTs and KB products; the update itself is FP64 */
+       * their FFTs and KB products, summed with B X in one pass; the update
+       * itself is FP64 */
+      TRY(ph_begin(c, PH_PRECOND));
+      TRY(launch_kbapply_precond_f32(c->Bf, c->nproj, c->Xkb, A2, ne, c->hml, K, reinterpret_cast<double *>(C2),
+                                     dte, c->Kp, st));
+      TRY(ph_end(c, PH_PRECOND));
+   } else if (pstep && f32) {
       TRY(ph_begin(c, PH_PRECOND));
       TRY(launch_precond_f32(A2, ne, c->hml, K, reinterpret_cast<double *>(C2), dte, c->Kp, st));
       TRY(ph_end(c, PH_PRECOND));

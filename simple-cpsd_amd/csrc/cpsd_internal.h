@@ -191,6 +191,11 @@ int launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
  * C1 H), the product in FP32 (C1, C2: [ne][K] reals, K a multiple of 128) */
 int launch_precond_f32(const double *C1, int ne, const double *H, size_t K, double *C2, double dte,
                        const double *Kp, hipStream_t s);
+/* the KB apply and the preconditioned update of an FP32 step in one pass:
+ * C2 = C1 + Kp o (C2 - C1 + dte (B X + C1 H)) */
+int launch_kbapply_precond_f32(const float *Bf, int nproj, const double *X, const double *C1, int ne,
+                               const double *H, size_t K, double *C2, double dte, const double *Kp,
+                               hipStream_t s);
 /* the projection with BF16 projectors Bb and C1 w rounded to BF16, FP32 sums
  * (precision mode 2) */
 int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const double *w, size_t K, int splits,

@@ -25,6 +25,11 @@
 #ifndef KBAPPLY_WAVES
 #define KBAPPLY_WAVES 6
 #endif
+/* the fused KB apply + preconditioned update: its second operand source
+ * spills at 6 (44 bytes per lane), not at 4 */
+#ifndef KBAPPLY_PRE_WAVES
+#define KBAPPLY_PRE_WAVES 4
+#endif
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -187,8 +192,16 @@ template<int AKT, int WAVES, typename TA, int PRE>
 __global__ void __launch_bounds__(512, WAVES)
 k_kbapply_f32(const TA *__restrict__ Bf, size_t ldb, int Mr, const double *__restrict__ X, int xsm, int ldx,
               int Nc, size_t K, double *__restrict__ C2, size_t ldc, double dte, int ntn, int per,
-              const double *__restrict__ Kp)
+              const double *__restrict__ Kp, const double *__restrict__ C1, int Mr2,
+              const double *__restrict__ X2, int ldx2)
 {
+   /* PRE = 2 (TA = float): the KB apply and the preconditioned step of the
+    * FP32 steps in one pass.  The reduction runs over the Mr projector rows
+    * of Bf (padded to whole AM steps) and then the Mr2 orbital rows of C1
+    * (FP64, rounded as they are staged) against hml X2 (X2[m + n ldx2]); the
+    * epilogue is PRE's with result = B X + C1 hml. */
+   const int Mr1 = PRE == 2 ? (Mr + AM - 1) / AM * AM : Mr;
+   const int Mtot = PRE == 2 ? Mr1 + Mr2 : Mr;
    constexpr int BP = AKT + 4, JB = AKT / 64, WKE = AKT / 4, BH = AKT / 128;
    /* main loop: Bs[2][AM][BP] and Xs[2][AN][XP]; epilogue: T[64][BP], one
     * 64-column half of the output tile at a time */
@@ -217,6 +230,27 @@ k_kbapply_f32(const TA *__restrict__ Bf, size_t ldb, int Mr, const double *__res
    float4 rb[BH];
    double rx[4];
    auto gload = [&](int m0) {
+      if (PRE == 2 && m0 >= Mr1) {
+         /* orbital segment: rows of C1, X2 = hml */
+         const int m = m0 - Mr1 + br;
+#pragma unroll
+         for (int h = 0; h < BH; ++h) {
+            const size_t k = k0 + bc + 4 * h;
+            rb[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (m < Mr2 && k < K) {
+               const double2 u = *reinterpret_cast<const double2 *>(C1 + (size_t)m * ldc + k);
+               const double2 v = *reinterpret_cast<const double2 *>(C1 + (size_t)m * ldc + k + 2);
+               rb[h] = make_float4((float)u.x, (float)u.y, (float)v.x, (float)v.y);
+            }
+         }
+         const int n = n0 + xn;
+#pragma unroll
+         for (int c = 0; c < 4; ++c) {
+            const int mm = m0 - Mr1 + xp + c;
+            rx[c] = (mm < Mr2 && n < Nc) ? X2[(size_t)mm + (size_t)n * ldx2] : 0.0;
+         }
+         return;
+      }
       const int m = m0 + br;
 #pragma unroll
       for (int h = 0; h < BH; ++h) {
@@ -266,11 +300,11 @@ k_kbapply_f32(const TA *__restrict__ Bf, size_t ldb, int Mr, const double *__res
    gload(0);
    lstore(0);
    __syncthreads();
-   if (AM < Mr) gload(AM);
+   if (AM < Mtot) gload(AM);
    int buf = 0;
-   for (int m0 = 0; m0 < Mr; m0 += AM) {
-      if (m0 + AM < Mr) lstore(buf ^ 1);
-      if (m0 + 2 * AM < Mr) gload(m0 + 2 * AM);
+   for (int m0 = 0; m0 < Mtot; m0 += AM) {
+      if (m0 + AM < Mtot) lstore(buf ^ 1);
+      if (m0 + 2 * AM < Mtot) gload(m0 + 2 * AM);
       mma(buf);
       __syncthreads();
       buf ^= 1;
@@ -309,8 +343,8 @@ k_kbapply_f32(const TA *__restrict__ Bf, size_t ldb, int Mr, const double *__res
             } else if constexpr (PRE) {
                /* one complex coefficient per lane: k = 2 (k / 2) */
                const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
-               const double2 c1 = *reinterpret_cast<const double2 *>(
-                  reinterpret_cast<const double *>(Bf) + (size_t)n * ldb + k);
+               const double *c1p = PRE == 2 ? C1 : reinterpret_cast<const double *>(Bf);
+               const double2 c1 = *reinterpret_cast<const double2 *>(c1p + (size_t)n * ldc + k);
                const double f = Kp[k >> 1];
                double2 a = p[0];
                a.x = c1.x + f * (a.x - c1.x + dte * v.x);
@@ -339,8 +373,8 @@ int launch_kbapply_f32(const float *Bf, int Mr, const double *X, int xsm, int ld
    const int ntn = (Nc + AN - 1) / AN;
    const long blocks = (long)((K + AKT - 1) / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, float, 0>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K, Mr, X,
-                      xsm, ldx, Nc, K, C2, ldc, dte, ntn, per, nullptr);
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, float, 0>), dim3((unsigned)blocks), dim3(512), 0, s, Bf, K,
+                      Mr, X, xsm, ldx, Nc, K, C2, ldc, dte, ntn, per, nullptr, nullptr, 0, nullptr, 0);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
@@ -356,8 +390,28 @@ int launch_precond_f32(const double *C1, int ne, const double *H, size_t K, doub
    const int ntn = (ne + AN - 1) / AN;
    const long blocks = (long)(K / AKT) * ntn;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
-   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, double, 1>), dim3((unsigned)blocks), dim3(512), 0, s, C1, K, ne, H, 1,
-                      ne, ne, K, C2, K, dte, ntn, per, Kp);
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_WAVES, double, 1>), dim3((unsigned)blocks), dim3(512), 0, s, C1,
+                      K, ne, H, 1, ne, ne, K, C2, K, dte, ntn, per, Kp, nullptr, 0, nullptr, 0);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* the KB apply and the preconditioned step of the FP32 steps in one pass:
+ * C2 = C1 + Kp o (C2 - C1 + dte (B X + C1 H)), both products summed in one
+ * FP32 accumulator (B: FP32 projectors [nproj][K]; X: nproj x ne with
+ * X[m + n nproj]; C1, C2: [ne][K] reals; H: ne x ne column-major).  One read
+ * and one write of C2 instead of two each. */
+int launch_kbapply_precond_f32(const float *Bf, int nproj, const double *X, const double *C1, int ne,
+                               const double *H, size_t K, double *C2, double dte, const double *Kp,
+                               hipStream_t s)
+{
+   constexpr int AKT = 128;
+   if (K % AKT) { cpsd_set_error("kbapply_precond_f32: K=%zu not a multiple of %d", K, AKT); return -1; }
+   const int ntn = (ne + AN - 1) / AN;
+   const long blocks = (long)(K / AKT) * ntn;
+   const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
+   hipLaunchKernelGGL((k_kbapply_f32<AKT, KBAPPLY_PRE_WAVES, float, 2>), dim3((unsigned)blocks), dim3(512), 0, s, Bf,
+                      K, nproj, X, 1, nproj, ne, K, C2, K, dte, ntn, per, Kp, C1, ne, H, ne);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
