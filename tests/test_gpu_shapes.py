@@ -127,6 +127,14 @@ def test_grid128_step_vs_oracle(gpu):
     run_pair(1, 20.0, 128, 4, steps=2, inner=1, seed=3)
 
 
+def test_grid128_vpsi_prefetch_vs_oracle(gpu):
+    """128^3 with 16 orbitals: the FP64 V psi pass takes two orbitals per
+    workgroup and prefetches the second one's plane into registers while the
+    first is transformed (with 4 orbitals above it takes one)"""
+    say("128^3, ne=16")
+    run_pair(4, 20.0, 128, 16, steps=1, inner=1, seed=3)
+
+
 def test_full_h2o32_properties(gpu):
     """bench.py's workload (32 waters, 96^3, 128 orbitals) at full size:
     orthonormality (the GPU Gram against a host Gram), hml symmetric, the
