@@ -165,3 +165,29 @@ def test_learned_factors_are_the_explicit_preconditioner(gpu):
         dev.close()
     assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-11
     assert np.max(np.abs(out[0][1] - out[1][1])) < 1e-11
+
+
+def test_learned_reaches_plain_sd_minimum_on_held_out_cell(gpu):
+    """the committed model on a held-out cell (16 waters, 80^3, seed 202: a
+    step-scale validation cell that no training step saw), mode 1: plain FP64
+    SD and the learned run each to |dE| < 1e-7 Ha between evaluations from
+    the same Gaussian start; the learned run takes under a quarter of the
+    steps and its energy is within 1e-6 Ha per atom of plain SD's"""
+    import learned
+    say("learned on a held-out cell")
+    model = learned.load_model()
+    nmol, L, nfft, seed = learned.VALID_CELLS[1]
+    assert (nmol, nfft, seed) == (16, 80, 202)
+    cell = cpsd_amd.water_cell(nmol, L, nfft, seed=seed)
+    psi = cell.random_orbitals(4 * nmol, seed=seed + 1000, decay=0.0)
+    dev = cpsd_amd.DeviceSD(cell, 4 * nmol)
+    ok0, n0, E0, _ = learned.converge(dev, psi)
+    dev.set_learned_preconditioner(model["params"], model["alpha"], model["kmin"], model["kmax"])
+    dev.set_precision(1)
+    ok1, n1, E1, _ = learned.converge(dev, psi)
+    fb = dev.fallback_info()["fallbacks"]
+    dev.close()
+    say("plain %d steps E %.9f, learned %d steps E %.9f" % (n0, E0, n1, E1))
+    assert ok0 and ok1 and fb == 0
+    assert n1 < n0 / 4
+    assert abs(E1 - E0) / (3 * nmol) < 1e-6
