@@ -46,7 +46,9 @@ WORKLOADS = {
     "h2o-32": dict(nmol=32, L=18.6, nfft=96, ne=128),
     "h2o-64": dict(nmol=64, L=23.5, nfft=128, ne=256),
 }
-# the reference binary timed during the survey, same host type
+# the reference's serial FP64 build timed on one core of the same host type
+# at 96^3 x 128 orbitals; the 128^3 x 256-orbital figure scales it by the
+# ratio of dense flops of the two cells
 REFERENCE_CPU_S_PER_STEP = {"h2o-32": (113.0, "measured, 1 core Xeon 2.7 GHz"),
                             "h2o-64": (1080.0, "extrapolated from h2o-32 by the dense-flop ratio 9.5")}
 HBM_PEAK_GBS = 8000.0
@@ -102,8 +104,8 @@ def algorithmic(phase, w, ng, nproj, ndnblk):
 
 
 def step_floor_ms(w, ng, nproj):
-    """the per-step roofline floor of the plain FP64 step (SURVEY.md
-    section 8(d) formulas): FFT traffic (2ne+2) 2 n2 8 B and elementwise
+    """the per-step roofline floor of the plain FP64 step: FFT traffic
+    (2ne+2) 2 n2 8 B and elementwise
     traffic 6 ne n2 8 B at 8 TB/s, contractions 8 nproj ne NG + 8 ne^2 NG +
     4 ne^2 NG flop at 78.6 TFLOP/s"""
     n, ne = w["nfft"], w["ne"]

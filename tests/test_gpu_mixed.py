@@ -1,6 +1,6 @@
 """The FP32-transform precision mode (cpsd_dev_set_precision(h, 1)) on the GPU.
 
-The reference's FP64 build has no such mode, so it is judged by its error
+The reference's FP64 build has no such mode, so it is measured by its error
 against FP64: against the FP64 oracle over a fixed number of steps, against
 the reference's own Example0 run (E after 1000 steps), and against the FP64
 device path on the 32-water workload.
