@@ -776,8 +776,9 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
    /* 10b: the preconditioned step.  C2 holds c1 + dte cpsi; it becomes
     * c1 + K o dte (cpsi + c1 h), the per-G scaled residual, with h the
     * Lagrange matrix hml of the last energy evaluation (exact at the fixed
-    * point, where h = <c1|H|c1>).  The overlaps and the constraint below
-    * then act on this c2 as they act on the plain one. */
+    * point, where h = <c1|H|c1>).  The overlaps below take this c2's
+    * component along c1 out first (sd_smat_precond), and the constraint
+    * adds it back through lmd = X - D/dte (sd_precond_lmd). */
    if (fused) {
       /* the FP32 steps of the mixed modes form C1 h in FP32 as well, like
        * their FFTs and KB products, summed with B X in one pass; the update
