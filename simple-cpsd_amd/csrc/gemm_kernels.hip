@@ -30,6 +30,14 @@
 
 /* ------------------------------------------------------- split-K TN */
 
+/* the split-K kernel's rotated B operands: 1 reads each rotation from LDS
+ * (the swizzled tiles keep those reads conflict-free), 0 forms them with DPP
+ * row rotations of one read.  At the 64-water cell the LDS reads made the
+ * carried Gram 4-6% faster; the same change in the tall kernel made it
+ * 3-5% slower, so that one keeps the DPP form. */
+#ifndef SPLITK_LDS_ROT
+#define SPLITK_LDS_ROT 1
+#endif
 #define TT 128  /* output tile edge */
 #define KT 32   /* K step */
 #define TP (TT + 2)  /* row pitch of the epilogue staging tile */
@@ -137,7 +145,16 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
    /* the wave index is uniform: read it as a scalar */
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wm = wave & 1, wn = wave >> 1;
+#if SPLITK_LDS_ROT
+   /* the rotated B operands read straight from LDS: rotation r takes column
+    * 4 ((c / 4 + r) % 4) + c % 4 of the 16-column block (c = lane % 16) */
+   const int s1 = 1;
+   int rrow[4];
+#pragma unroll
+   for (int r = 0; r < 4; ++r) rrow[r] = 4 * ((((lane & 15) >> 2) + r) & 3) + (lane & 3);
+#else
    const int s1 = rot_step(lane);
+#endif
    /* loader: rows lr + 32 h (h < 4) of the tile, k pair 2 (tid % 16): 16
     * lanes cover one 256-byte row segment, contiguous in global memory and,
     * as 16-byte stores, conflict-free in LDS */
@@ -229,8 +246,15 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
       double a[4], bv[4][4];
 #pragma unroll
       for (int i = 0; i < dg_na(DM); ++i) a[i] = S[bf][hs][16 * (drow0 + i) + (lane & 15)][kx] * wk;
+#if SPLITK_LDS_ROT
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+         for (int r = 0; r < 4; ++r) bv[j][r] = S[bf][hs][16 * (dcol0 + j) + rrow[r]][kk ^ rrow[r]];
+#else
 #pragma unroll
       for (int j = 0; j < 4; ++j) rotations(S[bf][hs][16 * (dcol0 + j) + (lane & 15)][kx], bv[j]);
+#endif
 #pragma unroll
       for (int b = 0; b < dg_nblk(DM); ++b)
 #pragma unroll
@@ -245,7 +269,12 @@ k_gemm_tn_splitk(const double *__restrict__ A, size_t lda, int M,
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
          double b[4];
+#if SPLITK_LDS_ROT
+#pragma unroll
+         for (int r = 0; r < 4; ++r) b[r] = S[bf][1][32 * wn + 16 * j + rrow[r]][kk ^ rrow[r]];
+#else
          rotations(S[bf][1][32 * wn + 16 * j + (lane & 15)][kx], b);
+#endif
 #pragma unroll
          for (int i = 0; i < BI; ++i)
 #pragma unroll
