@@ -140,6 +140,16 @@ int cpsd_dev_set_precision(cpsd_dev *h, int mode);
  * is refused with moving ions (set_ions move = 1), and moving ions with
  * mode 2 set. */
 int cpsd_dev_set_refresh(cpsd_dev *h, int period);
+/* overlaps of the FP32 steps of modes 1 and 2 (ne = 128 or 256): with
+ * split_bf16 = 1 such a step forms <c2|c1> and <c2|c2> (ccpsdv3/cpsd.c:
+ * 415-437) from the FP64 <c1|c1> carried from the previous step and the
+ * products of its step d = c2 - c1, <d|c1> and <d|d>, in split-BF16 MFMA
+ * products (each FP32 factor as hi + lo BF16, three products, FP32 sums),
+ * about 2^-16 of |d| relative.  The Lagrange iteration and the constraint
+ * stay FP64; the next FP64 step (the last of every call) forms the full FP64
+ * Gram, so the orbitals a call returns are orthonormal as in mode 0.
+ * 0 (the default) keeps FP64 overlaps in every step. */
+int cpsd_dev_set_overlaps(cpsd_dev *h, int split_bf16);
 /* Safeguards of cpsd_dev_run(): a non-finite energy or delta is an error
  * (the orbitals are lost; set them again).  With the fallback on (the
  * default, rise = 1e-8), a call whose energy exceeds the previous call's by

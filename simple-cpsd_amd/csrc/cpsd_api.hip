@@ -131,6 +131,13 @@ struct cpsd_dev {
    double *g11c[2], *g11w;
    double *DP;          /* preconditioned step: D = <c1|c2> - I, A11 D and D^T <c1|c2> */
    int g11_idx, g11_valid;
+   /* cpsd_dev_set_overlaps: the FP32 steps form the overlaps from split-BF16
+    * products of their step d = c2 - c1 and the carried <c1|c1> (Gx: the
+    * reduced products, ne x 2ne).  g11_drift: the carried <c1|c1> came from
+    * such overlaps; an FP64 step after them forms the full Gram, which
+    * returns the carried value to the FP64 one */
+   int ovl, g11_drift;
+   double *Gx;
    /* preconditioned step (cpsd_dev_set_preconditioner): one factor per
     * packed coefficient; the steps of a call use it when hml holds the
     * Lagrange matrix of the current orbitals' last energy evaluation */
@@ -153,7 +160,7 @@ struct cpsd_dev {
    int *herr;
    int pending;
    struct GraphEntry {
-      int inner, mixed, cur0, g11i0, g11v0, pre, rk, rph;
+      int inner, mixed, cur0, g11i0, g11v0, pre, rk, rph, ovl;
       double dt, dte;
       int cur1, g11i1, g11v1;
       hipGraphExec_t exec;
@@ -234,7 +241,7 @@ static void ctx_free(cpsd_dev *c)
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
                    c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->DP, c->Bb, c->Kp,
-                   c->Xf, c->Pmlp, c->Cinf, c->Rh, c->SA, c->SB};
+                   c->Gx, c->Xf, c->Pmlp, c->Cinf, c->Rh, c->SA, c->SB};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
@@ -802,10 +809,26 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
    }
    /* 11 */
    /* with <c1|c1> carried from the previous step, the Gram skips its C1 x C1
-    * block (whose reduced entries are then not read) */
-   const int carry = c->g11_valid;
+    * block (whose reduced entries are then not read); an FP32 step with
+    * cpsd_dev_set_overlaps forms the products of its step d = c2 - c1 only */
+   const int dgram = f32 && c->ovl && c->g11_valid;
+   const int carry = c->g11_valid && (dgram || !c->g11_drift);
    TRY(ph_begin(c, PH_GRAM));
-   {
+   if (dgram) {
+      int sp = 0;
+      TRY(launch_gram_dbf16(reinterpret_cast<const double *>(C1), reinterpret_cast<const double *>(C2), ne, c->w,
+                            K, c->partial, c->partial_cap, &sp, st));
+      TRY(ph_end(c, PH_GRAM));
+      TRY(ph_begin(c, PH_GRAMRED));
+      TRY(launch_reduce_partials(c->partial, sp, ne, 2 * ne, c->Gx, 0, st));
+      TRY(launch_gram_dfix(c->Gx, c->g11c[c->g11_idx], ne, c->cur, c->Gm, st));
+      if (pstep)
+         TRY(sd_smat_precond(c->Gm, c->g11c[c->g11_idx], ne, c->cur, dte, c->DP, c->s11, c->s12, c->s21, c->s22,
+                             c->bar, st));
+      else
+         TRY(sd_smat(c->Gm, c->g11c[c->g11_idx], ne, c->cur, dte, c->s11, c->s12, c->s21, c->s22, c->bar, st));
+      TRY(ph_end(c, PH_GRAMRED));
+   } else {
       const int sp = splits_for(c, 2 * ne, 2 * ne, K, 1);
       const double *Cb = reinterpret_cast<const double *>(c->Cbase);
       int so = 0, sdd = 0;
@@ -853,6 +876,7 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
                        c->g11c[c->g11_idx ^ 1], st));
       c->g11_idx ^= 1;
       c->g11_valid = 1;
+      c->g11_drift = dgram;
       TRY(ph_end(c, PH_GRAMRED));
    }
    /* 13 */
@@ -959,14 +983,14 @@ static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const 
          if (e.inner == inner && e.mixed == c->mixed && e.cur0 == c->cur && e.g11i0 == c->g11_idx &&
              e.g11v0 == c->g11_valid && e.pre == (c->precond && c->hml_valid) && e.rk == c->refresh &&
              e.rph == (c->refresh > 0 ? (int)(c->nstep % c->refresh) : 0) && e.dt == dt &&
-             e.dte == dte) { ge = &e; break; }
+             e.dte == dte && e.ovl == c->ovl) { ge = &e; break; }
       }
       if (!ge && c->ngc < 8) {
          cpsd_dev::GraphEntry e;
          e.inner = inner; e.mixed = c->mixed; e.cur0 = c->cur; e.g11i0 = c->g11_idx; e.g11v0 = c->g11_valid;
          e.pre = c->precond && c->hml_valid;
          e.rk = c->refresh; e.rph = c->refresh > 0 ? (int)(c->nstep % c->refresh) : 0;
-         e.dt = dt; e.dte = dte;
+         e.dt = dt; e.dte = dte; e.ovl = c->ovl;
          hipGraph_t g = nullptr;
          CPSD_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
          int rc = 0;
@@ -1550,6 +1574,23 @@ extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
       return -1;
    }
    h->mixed = mode;
+   return 0;
+}
+
+extern "C" int cpsd_dev_set_overlaps(cpsd_dev *h, int split_bf16)
+{
+   if (!h) { cpsd_set_error("null handle"); return -1; }
+   if (split_bf16 && !gram_dbf16_ok(h->ne)) {
+      cpsd_set_error("cpsd_dev_set_overlaps: the split-BF16 overlaps need ne = 128 or 256 (ne=%d)", h->ne);
+      return -1;
+   }
+   if (split_bf16 && !gram_dbf16_fits(h->ne, 2 * (size_t)h->ngp)) {
+      cpsd_set_error("cpsd_dev_set_overlaps: %d orbitals of %d packed coefficients exceed 4 GiB", h->ne, h->ngp);
+      return -1;
+   }
+   if (split_bf16 && !h->Gx) TRY(dalloc(&h->Gx, 2 * (size_t)h->ne * h->ne));
+   if (split_bf16 && !h->ovl) h->ran_mode[1] = h->ran_mode[2] = 0;   /* the next mixed call runs eagerly */
+   h->ovl = split_bf16 ? 1 : 0;
    return 0;
 }
 

@@ -201,6 +201,16 @@ int launch_kbapply_precond_f32(const float *Bf, int nproj, const double *X, cons
 int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const double *w, size_t K, int splits,
                        double *partial, size_t partial_cap, hipStream_t s);
 int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s);
+/* the overlaps of an FP32 step from its step d = C2 - C1 (split-BF16
+ * products, FP32 sums): partial[z][i][n] = sum_k d_i w_k [c1 | d]_n over
+ * *splits K ranges (ne 128 or 256: gram_dbf16_ok; ne K 8 bytes below 4 GiB:
+ * gram_dbf16_fits), then with the reduced
+ * Gx (ne x 2ne) and the carried <c1|c1> the Gram blocks of C2 */
+int gram_dbf16_ok(int ne);
+int gram_dbf16_fits(int ne, size_t K);
+int launch_gram_dbf16(const double *C1, const double *C2, int ne, const double *w, size_t K, double *partial,
+                      size_t partial_cap, int *splits, hipStream_t s);
+int launch_gram_dfix(const double *Gx, const double *g11, int ne, int c1_second, double *Gm, hipStream_t s);
 
 /* ---- learned preconditioner (precond_mlp.hip) ----------------------------- */
 #define CPSD_MLP_NPARAM 1249

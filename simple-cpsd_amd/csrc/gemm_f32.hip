@@ -564,6 +564,292 @@ int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
    return 0;
 }
 
+/* ------------------------------------- overlaps of the step (split BF16) */
+
+/* The overlaps of an FP32 step from its step d = c2 - c1 (ccpsdv3/cpsd.c:
+ * 415-437 with c2 = c1 + d):
+ *   <c2_i|c1_j> = <c1_i|c1_j> + <d_i|c1_j>,
+ *   <c2_i|c2_j> = <c1_i|c1_j> + <d_i|c1_j> + <d_j|c1_i> + <d_i|d_j>,
+ * with <c1|c1> the FP64 overlap carried from the previous step and the two
+ * products of d in reduced precision: d is small (it vanishes at the fixed
+ * point), so a relative error of the products is an error of size |d| in
+ * the overlaps.  The products are split BF16: an FP32 factor x is x = hi +
+ * lo + O(2^-17 |x|) with hi = bf16(x) and lo = bf16(x - hi), and
+ *   x y ~ hi_x hi_y + hi_x lo_y + lo_x hi_y
+ * (three v_mfma_f32_16x16x32_bf16, FP32 sums), about 2^-16 relative per
+ * term at six times the FP32 matrix rate, so the kernel is bound by its
+ * reads of the FP64 orbitals.
+ *
+ * partial[z][i][n] = sum_{k in split z} d_i(k) w_k V_n(k), i < R, n < 2R,
+ * V = [c1 | d]: one workgroup per (128-column block, K split), every
+ * workgroup holding all R rows of d, so that both operands come from the
+ * one read of C1 and C2 rows (the column block's rows are among them).
+ * Blocks are assigned to the 8 XCDs so that the 2R/128 column blocks of
+ * one K range run on one XCD and share its L2.  The d x d products are
+ * formed in full; the caller reads one triangle. */
+#define DQP (QK + 8)   /* LDS row pitch (bf16), as QP */
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+/* round-to-nearest-even BF16 bits of a finite FP32 value */
+__device__ __forceinline__ unsigned bf16_rne(float x)
+{
+   const unsigned u = __float_as_uint(x);
+   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+/* the split (hi, lo) of two FP32 values x, y: hi = bf16(x), lo = bf16(x -
+ * hi), each pair packed as (x, y) in one 32-bit word */
+__device__ __forceinline__ void split_pack(float x, float y, unsigned &hi, unsigned &lo)
+{
+   const unsigned hx = bf16_rne(x), hy = bf16_rne(y);
+   const float rx = x - __uint_as_float(hx << 16), ry = y - __uint_as_float(hy << 16);
+   hi = hx | (hy << 16);
+   lo = bf16_rne(rx) | (bf16_rne(ry) << 16);
+}
+
+template<int R, int CB>
+__global__ void __launch_bounds__(512, 2)
+k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_t ldc,
+             const double *__restrict__ w, size_t K, size_t kchunk, int per, double *__restrict__ partial)
+{
+   constexpr int NW = 2 * R / 128 / CB;            /* workgroups per K range, CB column blocks each */
+   constexpr int RG = R / 64, CG = 8 / RG;         /* waves: RG row groups of 64 x CG column groups */
+   constexpr int CW = 128 * CB / CG, BJ = CW / 16; /* columns per wave, 16-column blocks per wave */
+   constexpr int NP = R / 32;                      /* loader passes of 32 rows */
+   constexpr int ASZ = R * DQP, BSZ = 128 * CB * DQP;
+   extern __shared__ __bf16 dlds[];
+   /* [buffer][hi, lo]: A rows of d, B rows of w V */
+   auto Aat = [&](int bf, int hl) { return dlds + (size_t)(2 * bf + hl) * ASZ; };
+   auto Bat = [&](int bf, int hl) { return dlds + 4 * (size_t)ASZ + (size_t)(2 * bf + hl) * BSZ; };
+   const int bid = blockIdx.x;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   const int blk0 = (int)(g % NW) * CB, kz = (int)(g / NW);
+   const size_t kb = (size_t)kz * kchunk;
+   const size_t ke = min(K, kb + kchunk);
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wr = wave % RG, wc = wave / RG;
+   const int q = lane >> 4, rr = lane & 15;
+   /* loader: rows lr + 32 h, k pair lq; 16 lanes cover one row's 32 k */
+   const int lr = tid >> 4, lq = (tid & 15) * 2;
+
+   f4 acc[4][BJ];
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   /* C1 and C2 through buffer loads: one 32-bit byte offset per thread (its
+    * first row and k pair) and the part common to the workgroup (pass row
+    * and K step) in a scalar offset, so that the 2 NP loads of a K step
+    * share one address register.  Every offset stays below 2^32 (R K 8
+    * bytes, checked at launch). */
+   const __amdgpu_buffer_rsrc_t rc1 = __builtin_amdgcn_make_buffer_rsrc((void *)C1, (short)0, (int)0xffffffff,
+                                                                       0x00020000);
+   const __amdgpu_buffer_rsrc_t rc2 = __builtin_amdgcn_make_buffer_rsrc((void *)C2, (short)0, (int)0xffffffff,
+                                                                       0x00020000);
+   const unsigned vb = 8u * ((unsigned)lr * (unsigned)ldc + (unsigned)lq);
+   d2v r1[NP], r2[NP];
+   double2 rw;
+   auto gload = [&](size_t k0) {
+#pragma unroll
+      for (int h = 0; h < NP; ++h) {
+         const unsigned so = (unsigned)(8 * ((size_t)(32 * h) * ldc + k0));
+         r1[h] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rc1, vb, so, 0));
+         r2[h] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rc2, vb, so, 0));
+      }
+      rw = *reinterpret_cast<const double2 *>(w + k0 + lq);
+   };
+   auto lstore = [&](int bf) {
+#pragma unroll
+      for (int h = 0; h < NP; ++h) {
+         const int row = lr + 32 * h;
+         const double dx = r2[h].x - r1[h].x, dy = r2[h].y - r1[h].y;
+         const float fx = (float)dx, fy = (float)dy;
+         unsigned hi, lo;
+         split_pack(fx, fy, hi, lo);
+         *reinterpret_cast<unsigned *>(Aat(bf, 0) + row * DQP + lq) = hi;
+         *reinterpret_cast<unsigned *>(Aat(bf, 1) + row * DQP + lq) = lo;
+         /* the workgroup's column blocks that hold this orbital: block b is
+          * c1 (b < R/128) or d, orbitals 128 (b mod R/128) on */
+#pragma unroll
+         for (int t = 0; t < CB; ++t) {
+            const int b = blk0 + t;
+            const bool bd = b >= R / 128;
+            const int h0 = (b % (R / 128)) * 4;
+            if (h >= h0 && h < h0 + 4) {
+               /* the G-space weight (1 or 2, 0 on the padded tail) scales exactly */
+               const float vx = (bd ? fx : (float)r1[h].x) * (float)rw.x;
+               const float vy = (bd ? fy : (float)r1[h].y) * (float)rw.y;
+               const int br = 128 * t + row - 32 * h0;
+               split_pack(vx, vy, hi, lo);
+               *reinterpret_cast<unsigned *>(Bat(bf, 0) + br * DQP + lq) = hi;
+               *reinterpret_cast<unsigned *>(Bat(bf, 1) + br * DQP + lq) = lo;
+            }
+         }
+      }
+   };
+   auto mma = [&](int bf) {
+      bf8 ah[4], al[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+         const int o = (64 * wr + 16 * i + rr) * DQP + 8 * q;
+         ah[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 0) + o);
+         al[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 1) + o);
+      }
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+         const int o = (CW * wc + 16 * j + rr) * DQP + 8 * q;
+         const bf8 bh = *reinterpret_cast<const bf8 *>(Bat(bf, 0) + o);
+         const bf8 bl = *reinterpret_cast<const bf8 *>(Bat(bf, 1) + o);
+#pragma unroll
+         for (int i = 0; i < 4; ++i) {
+            acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+         }
+      }
+   };
+
+   if (kb < ke) {
+      gload(kb);
+      lstore(0);
+   }
+   __syncthreads();
+   if (kb + QK < ke) gload(kb + QK);
+   int buf = 0;
+   for (size_t k0 = kb; k0 < ke; k0 += QK) {
+      if (k0 + QK < ke) lstore(buf ^ 1);
+      if (k0 + 2 * QK < ke) gload(k0 + 2 * QK);
+      mma(buf);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+   /* 16 lanes hold 16 consecutive columns of one row: 128-byte segments */
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+         const int n = 128 * blk0 + CW * wc + 16 * j + rr;
+#pragma unroll
+         for (int r = 0; r < 4; ++r) {
+            const int row = 64 * wr + 16 * i + 4 * q + r;
+            partial[((size_t)kz * R + row) * (2 * R) + n] = (double)acc[i][j][r];
+         }
+      }
+}
+
+/* dynamic LDS of k_gram_dbf16<R, CB>: two buffers of the hi and lo rows of
+ * A (R rows) and B (128 CB rows) */
+static constexpr int gram_dbf16_lds(int R, int CB)
+{
+   return 4 * (R + 128 * CB) * DQP * (int)sizeof(__bf16);
+}
+
+int gram_dbf16_ok(int ne)
+{
+   return ne == 128 || ne == 256;
+}
+
+/* the buffer loads' 32-bit byte offsets span an orbital block */
+int gram_dbf16_fits(int ne, size_t K)
+{
+   return (double)ne * (double)K * 8.0 < 4294967296.0;
+}
+
+template<int R, int CB>
+static int launch_gram_dbf16_t(const double *C1, const double *C2, const double *w, size_t K, size_t kchunk,
+                               long pairs, int per, double *partial, hipStream_t s)
+{
+   constexpr int lds = gram_dbf16_lds(R, CB);
+   static std::atomic<unsigned long long> attr{0};
+   if (dyn_lds_once((const void *)k_gram_dbf16<R, CB>, lds, attr) != 0) return -1;
+   hipLaunchKernelGGL((k_gram_dbf16<R, CB>), dim3((unsigned)pairs), dim3(512), lds, s, C1, C2, K, w, K, kchunk,
+                      per, partial);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* partial[z][i][n] of k_gram_dbf16 for the ne orbitals of C1 and C2 (rows of
+ * K reals): two 128-column blocks per workgroup where the LDS holds their
+ * buffers (every K range read by ne / 128 workgroups), else one; two
+ * workgroups per CU over the K splits (one round of the GPU at one
+ * workgroup per CU, long K ranges, few partials); *splits receives their
+ * number */
+int launch_gram_dbf16(const double *C1, const double *C2, int ne, const double *w, size_t K, double *partial,
+                      size_t partial_cap, int *splits, hipStream_t s)
+{
+   if (!gram_dbf16_ok(ne)) { cpsd_set_error("gram_dbf16: ne=%d is not 128 or 256", ne); return -1; }
+   if (K % QK) { cpsd_set_error("gram_dbf16: K=%zu not a multiple of %d", K, QK); return -1; }
+   if (!gram_dbf16_fits(ne, K)) {
+      cpsd_set_error("gram_dbf16: %d orbitals of %zu reals exceed 4 GiB", ne, K);
+      return -1;
+   }
+   int dev = 0, maxlds = 0, cus = 0;
+   CPSD_CHECK(hipGetDevice(&dev));
+   CPSD_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+   CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+   const long wgs = 2L * (cus > 0 ? cus : 1);
+#ifdef GRAM_DBF16_CB
+   const int cb = GRAM_DBF16_CB;   /* build-time override for measurement */
+#else
+   const int cb = gram_dbf16_lds(ne, 2) <= maxlds ? 2 : 1;
+#endif
+   const int nw = 2 * ne / 128 / cb;
+   long sp = wgs / nw;
+   const long maxsp = (long)(K / (QK * 8));
+   const long capsp = (long)(partial_cap / (2 * (size_t)ne * ne));
+   if (sp > maxsp) sp = maxsp;
+   if (sp > capsp) sp = capsp;
+   if (sp < 1) {
+      cpsd_set_error("gram_dbf16: the partial buffer (%zu) holds no split of %d x %d", partial_cap, ne, 2 * ne);
+      return -1;
+   }
+   size_t kchunk = (K + sp - 1) / sp;
+   kchunk = (kchunk + QK - 1) / QK * QK;
+   sp = (long)((K + kchunk - 1) / kchunk);   /* no empty split */
+   const long pairs = (long)nw * sp;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   int rc;
+   if (ne == 256) rc = cb == 2 ? launch_gram_dbf16_t<256, 2>(C1, C2, w, K, kchunk, pairs, per, partial, s)
+                               : launch_gram_dbf16_t<256, 1>(C1, C2, w, K, kchunk, pairs, per, partial, s);
+   else rc = cb == 2 ? launch_gram_dbf16_t<128, 2>(C1, C2, w, K, kchunk, pairs, per, partial, s)
+                     : launch_gram_dbf16_t<128, 1>(C1, C2, w, K, kchunk, pairs, per, partial, s);
+   if (rc != 0) return rc;
+   *splits = (int)sp;
+   return 0;
+}
+
+/* Gx (ne x 2ne, row-major: [<d_i|c1_j> | <d_i|d_j>]) and the carried
+ * g11 = <c1|c1> (ne x ne, one triangle read) -> the C2 x C1, C1 x C2 and
+ * C2 x C2 blocks of the Gram matrix Gm (2ne x 2ne, row-major, C1 first
+ * unless c1_second), exactly symmetric where the overlaps are */
+__global__ void __launch_bounds__(256) k_gram_dfix(const double *__restrict__ Gx, const double *__restrict__ g11,
+                                                   int ne, int c1_second, double *__restrict__ Gm)
+{
+   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+   if (idx >= ne * ne) return;
+   const int i = idx / ne, j = idx % ne;
+   const int lo = min(i, j), hi = max(i, j);
+   const int n2 = 2 * ne, o1 = c1_second ? ne : 0, o2 = c1_second ? 0 : ne;
+   const double s = g11[(size_t)lo * ne + hi];
+   const double aij = Gx[(size_t)i * n2 + j], aji = Gx[(size_t)j * n2 + i];
+   const double b = Gx[(size_t)lo * n2 + ne + hi];
+   const double g21 = s + aij;
+   Gm[(size_t)(o2 + i) * n2 + o1 + j] = g21;     /* <c2_i|c1_j> */
+   Gm[(size_t)(o1 + j) * n2 + o2 + i] = g21;     /* <c1_j|c2_i> */
+   Gm[(size_t)(o2 + i) * n2 + o2 + j] = s + (aij + aji) + b;
+}
+
+int launch_gram_dfix(const double *Gx, const double *g11, int ne, int c1_second, double *Gm, hipStream_t s)
+{
+   const size_t ee = (size_t)ne * ne;
+   hipLaunchKernelGGL(k_gram_dfix, dim3((unsigned)((ee + 255) / 256)), dim3(256), 0, s, Gx, g11, ne, c1_second,
+                      Gm);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
 /* ----------------------------------------------------- FP32 projectors */
 
 __global__ void __launch_bounds__(256) k_to_f32(const double *__restrict__ a, float *__restrict__ b, size_t n)
