@@ -19,16 +19,21 @@ stream during a second pass of min(--steps, 100) steps.
 The metric's comparison, time to |E(k) - E(k-1)| < 1e-7 Ha between energy
 evaluations from the same start:
   time_to_converge.fp64          the reference's FP64 step (plain SD);
-  time_to_converge.mixed_learned precision mode 1 (FP32 orbital FFTs and KB
-                                 products in 9 of 10 steps, their overlaps
-                                 from split-BF16 products of the step c2 - c1
-                                 and the carried FP64 <c1|c1>, everything
-                                 else and every energy FP64) with the learned
-                                 per-G preconditioner (data/precond_mlp.json),
-                                 its converged energy's error per atom
-                                 against the FP64 run's, and the speedup.
---all-legs adds the same with FP64 overlaps in every step, FP64 + learned,
-mode 1 alone and mode 2 (BF16 KB projection).
+  time_to_converge.mixed_learned precision mode 3 (in 9 of 10 steps FP32
+                                 orbital FFTs and every product in split
+                                 BF16 MFMA: KB projection and apply, the
+                                 preconditioned update, the overlaps of the
+                                 step c2 - c1 with the carried FP64 <c1|c1>,
+                                 the constraint; the Lagrange iteration,
+                                 potentials and every energy FP64) with the
+                                 learned per-G preconditioner
+                                 (data/precond_mlp.json), its converged
+                                 energy's error per atom against the FP64
+                                 run's, and the speedup (mode 1, FP32
+                                 products, where ne is not 128 or 256).
+--all-legs adds mode 1 + learned (FP32 KB products and update, FP64 overlaps
+and constraint), FP64 + learned, mode 1 alone and mode 2 (BF16 KB
+projection).
 The drop-in leg times two cpsd() calls on host arrays (the reference's
 host-pointer contract, PCIe copies included); the second reuses the device
 context.
@@ -167,15 +172,14 @@ def run(args):
 
     model = learned.load_model() if args.converge_steps > 0 else None
 
-    def configure(mode, learned_on, ovl=False):
+    def configure(mode, learned_on):
         dev.set_precision(mode)
-        dev.set_overlaps(ovl)
         dev.set_preconditioner(None)
         if learned_on:
             dev.set_learned_preconditioner(model["params"], model["alpha"], model["kmin"], model["kmax"])
 
-    def converge(mode, learned_on, tag, ovl=False):
-        configure(mode, learned_on, ovl)
+    def converge(mode, learned_on, tag):
+        configure(mode, learned_on)
         dev.set_orbitals(psi)
         dev.E[:] = 0.0
         t0 = time.perf_counter()
@@ -199,9 +203,9 @@ def run(args):
                     last_dE=None if last is None else float(last), fallbacks=fb["fallbacks"],
                     ortho_max_dev_at_end=ortho)
 
-    def rate(mode, learned_on, ovl=False):
+    def rate(mode, learned_on):
         """steps/s of the configured step over --steps steps (after warmup)"""
-        configure(mode, learned_on, ovl)
+        configure(mode, learned_on)
         dev.set_orbitals(psi)
         steps(max(args.warmup, 2 * inner))   # the preconditioner starts with the second call
         t0 = time.perf_counter()
@@ -218,19 +222,18 @@ def run(args):
     conv = {}
     if args.converge_steps > 0:
         conv["fp64"] = converge(0, False, "fp64")
-        ovl = w["ne"] in (128, 256)
-        ml = converge(1, True, "mixed+learned", ovl)
-        ml.update(rate(1, True, ovl))
-        ml["overlaps"] = "split-BF16 step products" if ovl else "fp64"
+        hmode = 3 if w["ne"] in (128, 256) else 1
+        ml = converge(hmode, True, "mixed+learned")
+        ml.update(rate(hmode, True))
+        ml["precision_mode"] = hmode
         if conv["fp64"]["converged"] and ml["converged"]:
             ml["abs_E_error_vs_fp64_per_atom"] = abs(ml["E"] - conv["fp64"]["E"]) / natoms
             ml["speedup_vs_fp64"] = round(conv["fp64"]["seconds"] / ml["seconds"], 4)
         conv["mixed_learned"] = ml
         if args.all_legs:
-            for key, mode, lon, o in (("mixed_learned_fp64_overlaps", 1, True, False),
-                                      ("fp64_learned", 0, True, False), ("mixed", 1, False, False),
-                                      ("mixed_bf16", 2, False, False)):
-                r = converge(mode, lon, key, o)
+            for key, mode, lon in (("mixed1_learned", 1, True), ("fp64_learned", 0, True), ("mixed", 1, False),
+                                   ("mixed_bf16", 2, False)):
+                r = converge(mode, lon, key)
                 if conv["fp64"]["converged"] and r["converged"]:
                     r["abs_E_error_vs_fp64_per_atom"] = abs(r["E"] - conv["fp64"]["E"]) / natoms
                     r["speedup_vs_fp64"] = round(conv["fp64"]["seconds"] / r["seconds"], 4)

@@ -128,28 +128,29 @@ int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
  * step, whose density and -H psi the energies are formed from, is FP64, so
  * E, deltae and deltac are FP64 values of the current orbitals.  Mode 2 is
  * mode 1 with the KB projection's inputs rounded to BF16 (FP32 sums), which
- * moves the converged energy by up to a few 1e-7 Ha per atom.  The reference
- * has no such mode; its all-FP32 build (float_single.h with sfftpack3d) is
- * the nearest. */
+ * moves the converged energy by up to a few 1e-7 Ha per atom.  Mode 3 (ne =
+ * 128 or 256) runs the FFTs as mode 1 and every product of those steps in
+ * split BF16: each FP32 factor as hi + lo BF16 (hi = bf16(x), lo = bf16(x -
+ * hi)), three BF16 MFMA products, FP32 sums, about 2^-16 relative per term.
+ * That covers the KB projection and apply, the preconditioned update
+ * (cpsd_dev_set_preconditioner) and the constraint, and the overlaps
+ * (ccpsdv3/cpsd.c:415-437): those are formed as <c2|c1> = <c1|c1> + <d|c1>
+ * and <c2|c2> = <c1|c1> + <d|c1> + <c1|d> + <d|d> from the FP64 <c1|c1>
+ * carried from the previous step and the products of the step d = c2 - c1,
+ * so that their error scales with |d|.  The Lagrange iteration stays FP64,
+ * and the next FP64 step forms the full FP64 Gram, so a call returns
+ * orthonormal orbitals as in mode 0.  Moving ions are refused with modes 2
+ * and 3.  The reference has no such modes; its all-FP32 build (float_single.h
+ * with sfftpack3d) is the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
 /* FP64 refresh period of modes 1 and 2: step k of a run (counted from the
  * last cpsd_dev_set_orbitals, k = 1, 2, ...) runs in FP64 when k is a
  * multiple of `period`, besides the last step of every call, which the
  * energies need in FP64 (so a call with inner = 1 runs only FP64 steps);
- * 0 (the default) refreshes on the last step of each call only.  Mode 2
- * is refused with moving ions (set_ions move = 1), and moving ions with
- * mode 2 set. */
+ * 0 (the default) refreshes on the last step of each call only.  Modes 2
+ * and 3 are refused with moving ions (set_ions move = 1), and moving ions
+ * with mode 2 or 3 set. */
 int cpsd_dev_set_refresh(cpsd_dev *h, int period);
-/* overlaps of the FP32 steps of modes 1 and 2 (ne = 128 or 256): with
- * split_bf16 = 1 such a step forms <c2|c1> and <c2|c2> (ccpsdv3/cpsd.c:
- * 415-437) from the FP64 <c1|c1> carried from the previous step and the
- * products of its step d = c2 - c1, <d|c1> and <d|d>, in split-BF16 MFMA
- * products (each FP32 factor as hi + lo BF16, three products, FP32 sums),
- * about 2^-16 of |d| relative.  The Lagrange iteration and the constraint
- * stay FP64; the next FP64 step (the last of every call) forms the full FP64
- * Gram, so the orbitals a call returns are orthonormal as in mode 0.
- * 0 (the default) keeps FP64 overlaps in every step. */
-int cpsd_dev_set_overlaps(cpsd_dev *h, int split_bf16);
 /* Safeguards of cpsd_dev_run(): a non-finite energy or delta is an error
  * (the orbitals are lost; set them again).  With the fallback on (the
  * default, rise = 1e-8), a call whose energy exceeds the previous call's by

@@ -43,7 +43,7 @@ ABI_SYMBOLS = ("cpsd", "cr_fft3b_", "rc_fft3f_", "cpsd_last_error", "cpsd_set_ab
                "cpsd_dev_stats_end", "cpsd_dev_stats_read", "cpsd_mlp_loss_grad", "cpsd_mlp_train",
                "cpsd_mlp_predict", "cpsd_dev_set_refresh", "cpsd_dev_set_fallback", "cpsd_dev_fallback_info",
                "cpsd_batch_create", "cpsd_batch_run", "cpsd_batch_info", "cpsd_batch_destroy",
-               "cpsd_dev_autotune", "cpsd_dev_set_overlaps")
+               "cpsd_dev_autotune")
 HOST_SYMBOLS = ("cpsdh_setup", "cpsdh_read_psp", "cpsdh_kbpp", "cpsdh_makepsi",
                 "cpsdh_random_orbitals", "cpsdh_gcdotc", "cpsdh_write_elc", "cpsdh_read_elc",
                 "cpsdh_write_ionout", "cpsdh_write_vpp", "cpsdh_read_vpp")
@@ -82,7 +82,6 @@ def hip_lib():
         L.cpsd_dev_set_graphs.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_set_preconditioner.argtypes = [C.c_void_p, C.c_void_p]
         L.cpsd_dev_set_refresh.argtypes = [C.c_void_p, C.c_int]
-        L.cpsd_dev_set_overlaps.argtypes = [C.c_void_p, C.c_int]
         L.cpsd_dev_autotune.argtypes = [C.c_void_p, C.c_int, _ip]
         L.cpsd_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
         L.cpsd_batch_run.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
@@ -498,7 +497,10 @@ class DeviceSD:
     def set_precision(self, mode):
         """0: FP64 throughout; 1: FP32 orbital FFTs and KB products in every
         step of a run() but its last (energies stay FP64 values of the current
-        orbitals); 2: as 1 with a BF16-input KB projection"""
+        orbitals); 2: as 1 with a BF16-input KB projection; 3: FP32 FFTs and
+        every product of those steps (KB projection and apply, preconditioned
+        update, overlaps of the step c2 - c1, constraint) in split BF16
+        (ne 128 or 256)"""
         _check(hip_lib().cpsd_dev_set_precision(self.h, int(mode)), "cpsd_dev_set_precision")
 
     def set_preconditioner(self, K):
@@ -520,12 +522,6 @@ class DeviceSD:
         _check(hip_lib().cpsd_dev_autotune(self.h, int(steps), p), "cpsd_dev_autotune")
         return dict(fft_batch=int(p[0]), density_plane_orbitals=(int(p[1]), int(p[2])), splitk_target=int(p[3]),
                     from_table=bool(p[4]))
-
-    def set_overlaps(self, split_bf16):
-        """True: the FP32 steps of modes 1 and 2 form their overlaps from the
-        carried <c1|c1> and split-BF16 products of the step c2 - c1 (ne 128
-        or 256); False: FP64 overlaps in every step"""
-        _check(hip_lib().cpsd_dev_set_overlaps(self.h, 1 if split_bf16 else 0), "cpsd_dev_set_overlaps")
 
     def set_refresh(self, period):
         """FP64 refresh every `period` steps of the mixed modes (0: last step of each call)"""

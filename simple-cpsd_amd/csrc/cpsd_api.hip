@@ -108,7 +108,9 @@ struct cpsd_dev {
    size_t dnpart_cap;
    /* precision of the orbital transforms and KB products of every step but
     * the last of a call: 0 FP64; 1 FP32; 2 FP32 with a BF16-input KB
-    * projection (the last step, which feeds the energy, is FP64) */
+    * projection; 3 FP32 transforms with every product of the step (KB
+    * projection and apply, preconditioned update, overlaps, constraint) in
+    * split BF16 (the last step, which feeds the energy, is FP64) */
    int mixed;
    /* FP64 refresh period of the mixed modes: step k of the run (counted from
     * the last cpsd_dev_set_orbitals) is FP64 when (k + 1) % refresh == 0,
@@ -126,17 +128,22 @@ struct cpsd_dev {
    int bf_valid;          /* Bf matches B */
    __bf16 *Bb;            /* BF16 copy of the projectors (mode 2) */
    int bb_valid;
+   /* mode 3: the split-BF16 projectors, [nproj][2 ngp] (projection) and
+    * transposed [2 ngp][npad] (apply), npad = nproj rounded up to 32; the
+    * split X^T scratch of the apply, ne (npad + ne) each */
+   __bf16 *BsH, *BsL, *BtH, *BtL, *XTh, *XTl;
+   int bs_valid, npad;
    /* <c1|c1> of the next step, carried from this step's overlaps and lmd
     * (ne a multiple of 128): g11c[g11_idx] is valid when g11_valid */
    double *g11c[2], *g11w;
    double *DP;          /* preconditioned step: D = <c1|c2> - I, A11 D and D^T <c1|c2> */
    int g11_idx, g11_valid;
-   /* cpsd_dev_set_overlaps: the FP32 steps form the overlaps from split-BF16
-    * products of their step d = c2 - c1 and the carried <c1|c1> (Gx: the
-    * reduced products, ne x 2ne).  g11_drift: the carried <c1|c1> came from
+   /* mode 3: the FP32 steps form the overlaps from split-BF16 products of
+    * their step d = c2 - c1 and the carried <c1|c1> (Gx: the reduced
+    * products, ne x 2ne).  g11_drift: the carried <c1|c1> came from
     * such overlaps; an FP64 step after them forms the full Gram, which
     * returns the carried value to the FP64 one */
-   int ovl, g11_drift;
+   int g11_drift;
    double *Gx;
    /* preconditioned step (cpsd_dev_set_preconditioner): one factor per
     * packed coefficient; the steps of a call use it when hml holds the
@@ -153,14 +160,14 @@ struct cpsd_dev {
    /* HIP graphs of whole calls (inner steps + energy tail), keyed on what
     * the captured launches bake in; see ctx_run */
    int graphs;
-   int ran_mode[3];       /* an eager call has run in this precision mode */
+   int ran_mode[4];       /* an eager call has run in this precision mode */
    /* a started call's results (E[0..11], Lagrange info, error flag) land in
     * pinned host memory; pending until cpsd_dev_run_finish */
    double *hout;
    int *herr;
    int pending;
    struct GraphEntry {
-      int inner, mixed, cur0, g11i0, g11v0, pre, rk, rph, ovl;
+      int inner, mixed, cur0, g11i0, g11v0, pre, rk, rph;
       double dt, dte;
       int cur1, g11i1, g11v1;
       hipGraphExec_t exec;
@@ -241,7 +248,7 @@ static void ctx_free(cpsd_dev *c)
                    c->flag, c->gsum_part, c->xc_part, c->orb_part, c->ewald_part, c->E,
                    c->fullbuf, c->realbuf, c->r2d, c->fion, c->dti, c->Gp, c->fgpart,
                    c->fnlpart, c->fnlsum, c->poff, c->Bf, c->g11c[0], c->g11c[1], c->g11w, c->DP, c->Bb, c->Kp,
-                   c->Gx, c->Xf, c->Pmlp, c->Cinf, c->Rh, c->SA, c->SB};
+                   c->Gx, c->BsH, c->BsL, c->BtH, c->BtL, c->XTh, c->XTl, c->Xf, c->Pmlp, c->Cinf, c->Rh, c->SA, c->SB};
    for (void *p : ptrs)
       if (p) (void)hipFree(p);
    free(c->katm_h);
@@ -578,6 +585,7 @@ static int run_structure(cpsd_dev *c)
    TRY(sd_structure(a, st));
    c->bf_valid = 0;
    c->bb_valid = 0;
+   c->bs_valid = 0;
    return 0;
 }
 
@@ -590,6 +598,25 @@ static int ensure_bb(cpsd_dev *c)
       TRY(launch_to_bf16(reinterpret_cast<const double *>(c->B), c->Bb, n, c->stream));
       c->bb_valid = 1;
    }
+   return 0;
+}
+
+/* the split-BF16 projectors of mode 3: rows for the projection, transposed
+ * for the apply */
+static int ensure_bs(cpsd_dev *c)
+{
+   const size_t K = 2 * (size_t)c->ngp, n = (size_t)c->nproj * K;
+   c->npad = (c->nproj + 31) / 32 * 32;
+   if (c->nproj > 0 && !c->BsH) {
+      TRY(dalloc(&c->BsH, n)); TRY(dalloc(&c->BsL, n));
+      TRY(dalloc(&c->BtH, K * c->npad)); TRY(dalloc(&c->BtL, K * c->npad));
+   }
+   if (!c->bs_valid && c->nproj > 0) {
+      TRY(launch_to_bf16x2(reinterpret_cast<const double *>(c->B), c->BsH, c->BsL, n, c->stream));
+      TRY(launch_split_bt(reinterpret_cast<const double *>(c->B), c->nproj, c->npad, K, c->BtH, c->BtL,
+                          c->stream));
+   }
+   c->bs_valid = 1;
    return 0;
 }
 
@@ -715,15 +742,20 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
    }
    /* 9 */
    const double *A2 = reinterpret_cast<const double *>(C1);
+   /* mode 3: every product of an FP32 step in split BF16 (ne 128 or 256) */
+   const int sb = f32 && c->mixed == 3;
    if (c->nproj > 0) {
       TRY(ph_begin(c, PH_KBPROJ));
       const int sp = splits_for(c, c->nproj, ne, K, 0);
       const int small = gemm_tn_small_ok(c->nproj, ne);
-      if (f32) TRY(ensure_bf(c));
+      if (f32 && c->mixed != 3) TRY(ensure_bf(c));
       if (f32 && c->mixed == 2) TRY(ensure_bb(c));
+      if (sb) TRY(ensure_bs(c));
       if (small) {
          TRY(launch_gemm_tn_small(reinterpret_cast<const double *>(c->B), K, c->nproj, A2, K, ne, c->w, K, 0,
                                   c->P, st));
+      } else if (sb) {
+         TRY(launch_kbproj_dbf16(c->BsH, c->BsL, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
       } else if (f32 && c->mixed == 2) {
          TRY(launch_kbproj_bf16(c->Bb, c->nproj, A2, ne, c->w, K, sp, c->partial, c->partial_cap, st));
       } else if (f32) {
@@ -764,8 +796,11 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
     * update (10b): one pass over C2 */
    const int fused = pstep && f32 && c->nproj > 0;
    TRY(ph_begin(c, PH_KBAPPLY));
-   if (fused) {
+   if (fused || (sb && pstep)) {
       /* nothing here */
+   } else if (sb && c->nproj > 0) {
+      TRY(launch_apply_dbf16(c->BtH, c->BtL, c->npad, c->Xkb, c->nproj, c->nproj, nullptr, 0, nullptr, 0, ne, K,
+                             reinterpret_cast<double *>(C2), dte, nullptr, 0, c->XTh, c->XTl, st));
    } else if (f32 && c->nproj > 0) {
       /* C2 += dte B X with FP32 products (never the last step) */
       TRY(launch_kbapply_f32(c->Bf, c->nproj, c->Xkb, 1, c->nproj, ne, K, reinterpret_cast<double *>(C2), K,
@@ -786,7 +821,14 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
     * point, where h = <c1|H|c1>).  The overlaps below take this c2's
     * component along c1 out first (sd_smat_precond), and the constraint
     * adds it back through lmd = X - D/dte (sd_precond_lmd). */
-   if (fused) {
+   if (sb && pstep) {
+      /* mode 3: B X + C1 h in split BF16, one pass */
+      TRY(ph_begin(c, PH_PRECOND));
+      TRY(launch_apply_dbf16(c->BtH, c->BtL, c->nproj > 0 ? c->npad : 0, c->Xkb, c->nproj, c->nproj, A2, ne,
+                             c->hml, ne, ne, K, reinterpret_cast<double *>(C2), dte, c->Kp, 1, c->XTh, c->XTl,
+                             st));
+      TRY(ph_end(c, PH_PRECOND));
+   } else if (fused) {
       /* the FP32 steps of the mixed modes form C1 h in FP32 as well, like
        * their FFTs and KB products, summed with B X in one pass; the update
        * itself is FP64 */
@@ -809,9 +851,9 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
    }
    /* 11 */
    /* with <c1|c1> carried from the previous step, the Gram skips its C1 x C1
-    * block (whose reduced entries are then not read); an FP32 step with
-    * cpsd_dev_set_overlaps forms the products of its step d = c2 - c1 only */
-   const int dgram = f32 && c->ovl && c->g11_valid;
+    * block (whose reduced entries are then not read); an FP32 step of mode 3
+    * forms the products of its step d = c2 - c1 only */
+   const int dgram = sb && c->g11_valid;
    const int carry = c->g11_valid && (dgram || !c->g11_drift);
    TRY(ph_begin(c, PH_GRAM));
    if (dgram) {
@@ -881,7 +923,10 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
    }
    /* 13 */
    TRY(ph_begin(c, PH_CONSTRAINT));
-   {
+   if (sb) {
+      TRY(launch_apply_dbf16(nullptr, nullptr, 0, nullptr, 0, 0, A2, ne, c->lmd, ne, ne, K,
+                             reinterpret_cast<double *>(C2), dte, nullptr, 0, c->XTh, c->XTl, st));
+   } else {
       TallArgs t;
       t.A = A2; t.lda = K; t.X = c->lmd; t.ldx = ne; t.Mr = ne; t.Nc = ne; t.K = K;
       t.C2 = reinterpret_cast<double *>(C2); t.C1 = nullptr; t.CP = nullptr; t.ldc = K;
@@ -975,7 +1020,8 @@ static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const 
     * mode's first call (one-time kernel attributes, FP32 projectors) run
     * eagerly. */
    const bool use_graph = c->graphs && !c->move && !c->prof && !c->stats_on && c->ran_mode[c->mixed] &&
-                          (!c->mixed || c->bf_valid) && (c->mixed != 2 || c->bb_valid);
+                          (c->mixed == 0 || c->mixed == 3 || c->bf_valid) && (c->mixed != 2 || c->bb_valid) &&
+                          (c->mixed != 3 || c->bs_valid);
    cpsd_dev::GraphEntry *ge = nullptr;
    if (use_graph) {
       for (int i = 0; i < c->ngc; ++i) {
@@ -983,14 +1029,14 @@ static int ctx_run_start(cpsd_dev *c, int inner, double dt, double fmass, const 
          if (e.inner == inner && e.mixed == c->mixed && e.cur0 == c->cur && e.g11i0 == c->g11_idx &&
              e.g11v0 == c->g11_valid && e.pre == (c->precond && c->hml_valid) && e.rk == c->refresh &&
              e.rph == (c->refresh > 0 ? (int)(c->nstep % c->refresh) : 0) && e.dt == dt &&
-             e.dte == dte && e.ovl == c->ovl) { ge = &e; break; }
+             e.dte == dte) { ge = &e; break; }
       }
       if (!ge && c->ngc < 8) {
          cpsd_dev::GraphEntry e;
          e.inner = inner; e.mixed = c->mixed; e.cur0 = c->cur; e.g11i0 = c->g11_idx; e.g11v0 = c->g11_valid;
          e.pre = c->precond && c->hml_valid;
          e.rk = c->refresh; e.rph = c->refresh > 0 ? (int)(c->nstep % c->refresh) : 0;
-         e.dt = dt; e.dte = dte; e.ovl = c->ovl;
+         e.dt = dt; e.dte = dte;
          hipGraph_t g = nullptr;
          CPSD_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
          int rc = 0;
@@ -1140,7 +1186,8 @@ static bool same_start(const BatchCellState &a, const BatchCellState &b)
 static bool cell_capturable(const cpsd_dev *c)
 {
    return !c->move && !c->prof && !c->stats_on && !c->pending && !c->poisoned && c->struct_valid &&
-          c->ran_mode[c->mixed] && (!c->mixed || c->bf_valid) && (c->mixed != 2 || c->bb_valid);
+          c->ran_mode[c->mixed] && (c->mixed == 0 || c->mixed == 3 || c->bf_valid) &&
+          (c->mixed != 2 || c->bb_valid) && (c->mixed != 3 || c->bs_valid);
 }
 
 static void batch_drop_graphs(cpsd_batch *b)
@@ -1307,8 +1354,9 @@ static int ctx_set_ions(cpsd_dev *c, int move, const double *amass, const double
    hipStream_t st = c->stream;
    if (move != 0 && move != 1) { cpsd_set_error("move=%d must be 0 or 1", move); return -1; }
    if (!r2 || (move && !amass)) { cpsd_set_error("set_ions needs positions and, for move=1, masses"); return -1; }
-   if (move && c->mixed == 2) {
-      cpsd_set_error("set_ions: moving ions need precision mode 0 or 1 (mode 2 moves forces by ~3e-4 Ha/bohr)");
+   if (move && c->mixed >= 2) {
+      cpsd_set_error("set_ions: moving ions need precision mode 0 or 1 (mode 2 moves forces by ~3e-4 Ha/bohr; "
+                     "mode 3 is for electronic convergence)");
       return -1;
    }
    const size_t n3i = 3 * (size_t)c->nion;
@@ -1563,34 +1611,34 @@ extern "C" int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double
 extern "C" int cpsd_dev_set_precision(cpsd_dev *h, int mode)
 {
    if (!h) { cpsd_set_error("null handle"); return -1; }
-   if (mode < 0 || mode > 2) {
-      cpsd_set_error("cpsd_dev_set_precision: mode %d is not 0 (FP64), 1 (FP32) or 2 (FP32, BF16 projection)",
-                     mode);
+   if (mode < 0 || mode > 3) {
+      cpsd_set_error("cpsd_dev_set_precision: mode %d is not 0 (FP64), 1 (FP32), 2 (FP32, BF16 projection) or 3 "
+                     "(FP32 transforms, split-BF16 products)", mode);
       return -1;
    }
-   if (mode == 2 && h->move) {
-      cpsd_set_error("cpsd_dev_set_precision: mode 2 (BF16 projection) moves forces by ~3e-4 Ha/bohr; with "
-                     "moving ions use mode 0 or 1");
+   if (mode >= 2 && h->move) {
+      cpsd_set_error("cpsd_dev_set_precision: mode %d is for electronic convergence (mode 2's BF16 projection "
+                     "moves forces by ~3e-4 Ha/bohr); with moving ions use mode 0 or 1", mode);
       return -1;
+   }
+   if (mode == 3) {
+      const size_t K = 2 * (size_t)h->ngp;
+      if (!gram_dbf16_ok(h->ne) || !apply_dbf16_ok(h->ne)) {
+         cpsd_set_error("cpsd_dev_set_precision: mode 3 needs ne = 128 or 256 (ne=%d)", h->ne);
+         return -1;
+      }
+      if (!gram_dbf16_fits(h->ne, K)) {
+         cpsd_set_error("cpsd_dev_set_precision: mode 3: %d orbitals of %zu reals exceed 4 GiB", h->ne, K);
+         return -1;
+      }
+      const int npad = (h->nproj + 31) / 32 * 32;
+      if (!h->Gx) TRY(dalloc(&h->Gx, 2 * (size_t)h->ne * h->ne));
+      if (!h->XTh) {
+         TRY(dalloc(&h->XTh, (size_t)h->ne * (npad + h->ne)));
+         TRY(dalloc(&h->XTl, (size_t)h->ne * (npad + h->ne)));
+      }
    }
    h->mixed = mode;
-   return 0;
-}
-
-extern "C" int cpsd_dev_set_overlaps(cpsd_dev *h, int split_bf16)
-{
-   if (!h) { cpsd_set_error("null handle"); return -1; }
-   if (split_bf16 && !gram_dbf16_ok(h->ne)) {
-      cpsd_set_error("cpsd_dev_set_overlaps: the split-BF16 overlaps need ne = 128 or 256 (ne=%d)", h->ne);
-      return -1;
-   }
-   if (split_bf16 && !gram_dbf16_fits(h->ne, 2 * (size_t)h->ngp)) {
-      cpsd_set_error("cpsd_dev_set_overlaps: %d orbitals of %d packed coefficients exceed 4 GiB", h->ne, h->ngp);
-      return -1;
-   }
-   if (split_bf16 && !h->Gx) TRY(dalloc(&h->Gx, 2 * (size_t)h->ne * h->ne));
-   if (split_bf16 && !h->ovl) h->ran_mode[1] = h->ran_mode[2] = 0;   /* the next mixed call runs eagerly */
-   h->ovl = split_bf16 ? 1 : 0;
    return 0;
 }
 
@@ -1705,8 +1753,9 @@ extern "C" int cpsd_dev_autotune(cpsd_dev *h, int steps, int *plan)
       int rc = hipMemcpyAsync(backup, C2, orb * sizeof(double2), hipMemcpyDeviceToDevice, h->stream) == hipSuccess
                   ? 0 : -1;
       if (rc == 0 && !h->struct_valid) { rc = run_structure(h); h->struct_valid = rc == 0; }
-      if (h->mixed && rc == 0) rc = ensure_bf(h);
+      if (h->mixed && h->mixed != 3 && rc == 0) rc = ensure_bf(h);
       if (h->mixed == 2 && rc == 0) rc = ensure_bb(h);
+      if (h->mixed == 3 && rc == 0) rc = ensure_bs(h);
       Plan cur{h->ne, {0, 0}, 2048, 0.0};
       if (rc == 0) rc = time_plan(h, backup, cur, steps, &cur.ms);
       best = cur;

@@ -211,6 +211,20 @@ int gram_dbf16_fits(int ne, size_t K);
 int launch_gram_dbf16(const double *C1, const double *C2, int ne, const double *w, size_t K, double *partial,
                       size_t partial_cap, int *splits, hipStream_t s);
 int launch_gram_dfix(const double *Gx, const double *g11, int ne, int c1_second, double *Gm, hipStream_t s);
+/* precision mode 3: the split-BF16 projectors (rows BsH + BsL; transposed
+ * [K][npad] BtH + BtL), the KB projection on them, and the apply, the
+ * preconditioned update and the constraint:
+ *   C2 (+)= dte (B^T X + C1^T X2)   (X[p + n ldx], X2[m + n ldx2])
+ * or with precond C2 = C1 + Kp o (C2 - C1 + dte v); ne 128 or 256
+ * (apply_dbf16_ok); XTh, XTl: ne (npad + Mr2) bf16 each of scratch */
+int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStream_t s);
+int launch_split_bt(const double *B, int nproj, int npad, size_t K, __bf16 *BtH, __bf16 *BtL, hipStream_t s);
+int launch_kbproj_dbf16(const __bf16 *BsH, const __bf16 *BsL, int M, const double *C, int N, const double *w,
+                        size_t K, int splits, double *partial, size_t partial_cap, hipStream_t s);
+int apply_dbf16_ok(int ne);
+int launch_apply_dbf16(const __bf16 *BtH, const __bf16 *BtL, int npad, const double *X, int nproj, int ldx,
+                       const double *C1, int Mr2, const double *X2, int ldx2, int ne, size_t K, double *C2,
+                       double dte, const double *Kp, int precond, __bf16 *XTh, __bf16 *XTl, hipStream_t s);
 
 /* ---- learned preconditioner (precond_mlp.hip) ----------------------------- */
 #define CPSD_MLP_NPARAM 1249

@@ -428,6 +428,23 @@ __device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c)
 #define QK 32           /* K step: one 16x16x32 instruction per block */
 #define QP (QK + 8)     /* row pitch (bf16): a 16-byte read of 16 rows at one k hits distinct banks */
 
+/* round-to-nearest-even BF16 bits of a finite FP32 value */
+__device__ __forceinline__ unsigned bf16_rne(float x)
+{
+   const unsigned u = __float_as_uint(x);
+   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+/* the split (hi, lo) of two FP32 values x, y: hi = bf16(x), lo = bf16(x -
+ * hi), each pair packed as (x, y) in one 32-bit word */
+__device__ __forceinline__ void split_pack(float x, float y, unsigned &hi, unsigned &lo)
+{
+   const unsigned hx = bf16_rne(x), hy = bf16_rne(y);
+   const float rx = x - __uint_as_float(hx << 16), ry = y - __uint_as_float(hy << 16);
+   hi = hx | (hy << 16);
+   lo = bf16_rne(rx) | (bf16_rne(ry) << 16);
+}
+
 /* partial[z][p][n] = sum_{k in split z} w_k Bb[p][k] C1[n][k] with BF16
  * projectors Bb, C1 w rounded to BF16 as it is staged in LDS, FP32 sums;
  * the layout, grid and split reduction of k_kbproj_f32.  Lane l supplies 8
@@ -550,6 +567,159 @@ int launch_kbproj_bf16(const __bf16 *Bb, int M, const double *C, int N, const do
    return 0;
 }
 
+/* ------------------------------------ split-BF16 projection (mode 3) */
+
+/* partial[z][p][n] = sum_{k in split z} w_k B[p][k] C1[n][k], both factors
+ * split BF16 (B as BsH + BsL, made once per set of projectors; C1 w split as
+ * it is staged), three v_mfma_f32_16x16x32_bf16 per block (hi hi, hi lo,
+ * lo hi), FP32 sums: the layout, grid and split reduction of
+ * k_kbproj_f32 */
+template<int TM>
+__global__ void __launch_bounds__(512, 4)
+k_kbproj_dbf16(const __bf16 *__restrict__ BsH, const __bf16 *__restrict__ BsL, size_t ldb, int M,
+               const double *__restrict__ C, size_t ldc, int N, const double *__restrict__ w, size_t K,
+               size_t kchunk, int ntm, int ntile, int per, double *__restrict__ partial)
+{
+   constexpr int BI = TM / 32;
+   constexpr int ASZ = TM * QP, BSZ = PN * QP;
+   extern __shared__ __bf16 plds[];
+   /* [buffer][hi, lo] */
+   auto Aat = [&](int bf, int hl) { return plds + (size_t)(2 * bf + hl) * ASZ; };
+   auto Bat = [&](int bf, int hl) { return plds + 4 * (size_t)ASZ + (size_t)(2 * bf + hl) * BSZ; };
+   const int bid = blockIdx.x;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   const int t = (int)(g % ntile), kz = (int)(g / ntile);
+   const int tm = t % ntm, tn = t / ntm;
+   const size_t kb = (size_t)kz * kchunk;
+   const size_t ke = min(K, kb + kchunk);
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wm = wave & 1, wn = wave >> 1;
+   const int q = lane >> 4, rr = lane & 15;
+   /* loaders: A row ar (8 bf16 at aq, 4 threads per 32-k row); B rows
+    * lr + 32 h, k pair lq (16 threads per row) */
+   const int ar = tid >> 2, aq = (tid & 3) * 8;
+   const int lr = tid >> 4, lq = (tid & 15) * 2;
+
+   f4 acc[BI][2];
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   uint4 rah, ral;
+   double2 rb[4], rw;
+   auto gload = [&](size_t k0) {
+      const int am = tm * TM + ar;
+      rah = ral = make_uint4(0u, 0u, 0u, 0u);
+      if (ar < TM && am < M) {
+         rah = *reinterpret_cast<const uint4 *>(BsH + (size_t)am * ldb + k0 + aq);
+         ral = *reinterpret_cast<const uint4 *>(BsL + (size_t)am * ldb + k0 + aq);
+      }
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+         const int bn = tn * PN + lr + 32 * h;
+         rb[h] = bn < N ? *reinterpret_cast<const double2 *>(C + (size_t)bn * ldc + k0 + lq)
+                        : make_double2(0.0, 0.0);
+      }
+      rw = *reinterpret_cast<const double2 *>(w + k0 + lq);
+   };
+   auto lstore = [&](int bf) {
+      if (ar < TM) {
+         *reinterpret_cast<uint4 *>(Aat(bf, 0) + ar * QP + aq) = rah;
+         *reinterpret_cast<uint4 *>(Aat(bf, 1) + ar * QP + aq) = ral;
+      }
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+         /* the G-space weight (1 or 2, 0 on the padded tail) scales exactly */
+         unsigned hi, lo;
+         split_pack((float)rb[h].x * (float)rw.x, (float)rb[h].y * (float)rw.y, hi, lo);
+         const int o = (lr + 32 * h) * QP + lq;
+         *reinterpret_cast<unsigned *>(Bat(bf, 0) + o) = hi;
+         *reinterpret_cast<unsigned *>(Bat(bf, 1) + o) = lo;
+      }
+   };
+   auto mma = [&](int bf) {
+      bf8 ah[BI], al[BI];
+#pragma unroll
+      for (int i = 0; i < BI; ++i) {
+         const int o = ((TM / 2) * wm + 16 * i + rr) * QP + 8 * q;
+         ah[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 0) + o);
+         al[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 1) + o);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+         const int o = (32 * wn + 16 * j + rr) * QP + 8 * q;
+         const bf8 bh = *reinterpret_cast<const bf8 *>(Bat(bf, 0) + o);
+         const bf8 bl = *reinterpret_cast<const bf8 *>(Bat(bf, 1) + o);
+#pragma unroll
+         for (int i = 0; i < BI; ++i) {
+            acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+         }
+      }
+   };
+
+   if (kb < ke) {
+      gload(kb);
+      lstore(0);
+   }
+   __syncthreads();
+   if (kb + QK < ke) gload(kb + QK);
+   int buf = 0;
+   for (size_t k0 = kb; k0 < ke; k0 += QK) {
+      if (k0 + QK < ke) lstore(buf ^ 1);
+      if (k0 + 2 * QK < ke) gload(k0 + 2 * QK);
+      mma(buf);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+#pragma unroll
+   for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+         const int n = tn * PN + 32 * wn + 16 * j + rr;
+#pragma unroll
+         for (int r = 0; r < 4; ++r) {
+            const int p = tm * TM + (TM / 2) * wm + 16 * i + 4 * q + r;
+            if (p < M && n < N) partial[((size_t)kz * M + p) * N + n] = (double)acc[i][j][r];
+         }
+      }
+}
+
+int launch_kbproj_dbf16(const __bf16 *BsH, const __bf16 *BsL, int M, const double *C, int N, const double *w,
+                        size_t K, int splits, double *partial, size_t partial_cap, hipStream_t s)
+{
+   if (K % QK) { cpsd_set_error("kbproj_dbf16: K=%zu not a multiple of %d", K, QK); return -1; }
+   if (splits < 1 || (size_t)splits * M * N > partial_cap) {
+      cpsd_set_error("kbproj_dbf16: %d splits of %d x %d do not fit the partial buffer", splits, M, N);
+      return -1;
+   }
+   const int pad128 = (M + 127) / 128 * 128 - M, pad96 = (M + 95) / 96 * 96 - M;
+   const int tm = pad96 < pad128 ? 96 : 128;
+   const int ntm = (M + tm - 1) / tm, ntile = ntm * ((N + PN - 1) / PN);
+   size_t kchunk = (K + splits - 1) / splits;
+   kchunk = (kchunk + QK - 1) / QK * QK;
+   const long pairs = (long)ntile * splits;
+   const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
+   if (tm == 96) {
+      constexpr int lds = 4 * (96 + PN) * QP * (int)sizeof(__bf16);
+      static std::atomic<unsigned long long> attr{0};
+      if (dyn_lds_once((const void *)k_kbproj_dbf16<96>, lds, attr) != 0) return -1;
+      hipLaunchKernelGGL((k_kbproj_dbf16<96>), dim3((unsigned)pairs), dim3(512), lds, s, BsH, BsL, K, M, C, K, N,
+                         w, K, kchunk, ntm, ntile, per, partial);
+   } else {
+      constexpr int lds = 4 * (128 + PN) * QP * (int)sizeof(__bf16);
+      static std::atomic<unsigned long long> attr{0};
+      if (dyn_lds_once((const void *)k_kbproj_dbf16<128>, lds, attr) != 0) return -1;
+      hipLaunchKernelGGL((k_kbproj_dbf16<128>), dim3((unsigned)pairs), dim3(512), lds, s, BsH, BsL, K, M, C, K,
+                         N, w, K, kchunk, ntm, ntile, per, partial);
+   }
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
 __global__ void __launch_bounds__(256) k_to_bf16(const double *__restrict__ a, __bf16 *__restrict__ b, size_t n)
 {
    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -590,22 +760,6 @@ int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
 #define DQP (QK + 8)   /* LDS row pitch (bf16), as QP */
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-/* round-to-nearest-even BF16 bits of a finite FP32 value */
-__device__ __forceinline__ unsigned bf16_rne(float x)
-{
-   const unsigned u = __float_as_uint(x);
-   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-}
-
-/* the split (hi, lo) of two FP32 values x, y: hi = bf16(x), lo = bf16(x -
- * hi), each pair packed as (x, y) in one 32-bit word */
-__device__ __forceinline__ void split_pack(float x, float y, unsigned &hi, unsigned &lo)
-{
-   const unsigned hx = bf16_rne(x), hy = bf16_rne(y);
-   const float rx = x - __uint_as_float(hx << 16), ry = y - __uint_as_float(hy << 16);
-   hi = hx | (hy << 16);
-   lo = bf16_rne(rx) | (bf16_rne(ry) << 16);
-}
 
 template<int R, int CB>
 __global__ void __launch_bounds__(512, 2)
@@ -846,6 +1000,306 @@ int launch_gram_dfix(const double *Gx, const double *g11, int ne, int c1_second,
    const size_t ee = (size_t)ne * ne;
    hipLaunchKernelGGL(k_gram_dfix, dim3((unsigned)((ee + 255) / 256)), dim3(256), 0, s, Gx, g11, ne, c1_second,
                       Gm);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+__global__ void __launch_bounds__(256) k_to_bf16x2(const double *__restrict__ a, unsigned *__restrict__ hi,
+                                                   unsigned *__restrict__ lo, size_t npair)
+{
+   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+   if (i < npair) {
+      const double2 v = *reinterpret_cast<const double2 *>(a + 2 * i);
+      split_pack((float)v.x, (float)v.y, hi[i], lo[i]);
+   }
+}
+
+/* BsH, BsL: the split-BF16 pair of n values (n even) */
+int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStream_t s)
+{
+   if (n % 2) { cpsd_set_error("to_bf16x2: n=%zu is odd", n); return -1; }
+   const size_t np = n / 2;
+   if (np == 0) return 0;
+   hipLaunchKernelGGL(k_to_bf16x2, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, a,
+                      reinterpret_cast<unsigned *>(BsH), reinterpret_cast<unsigned *>(BsL), np);
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* --------------------------- split-BF16 apply, update and constraint */
+
+/* C2[n][k] (+)= dte sum_p XT[n][p] V[p][k] over P = npad + Mr2 rows: the
+ * projector rows (p < npad: V = B, from BtH/BtL, the split projectors
+ * transposed to [K][npad] once per set of projectors) and then the orbital
+ * rows (p = npad + m: V = C1 row m, FP64, split and transposed as it is
+ * staged); XT = [X^T | X2^T] split (k_split_xt).  Both factors split BF16,
+ * three v_mfma_f32_16x16x32_bf16 per block, FP32 sums.
+ *   PRE = 0: C2 += dte v (the KB apply, the constraint);
+ *   PRE = 1: C2 = C1 + Kp o (C2 - C1 + dte v) (the preconditioned update).
+ * One workgroup per 128 reals holds every orbital (NO = ne), so that each
+ * V row is read once.  MFMA D[i][j] with i an orbital (A = XT rows, 8
+ * consecutive p per lane) and j a real (B = the staged V^T rows, 8
+ * consecutive p per lane); lane (rr, q), register r holds orbital 4q + r,
+ * real rr of its block: 16 lanes cover 128 consecutive bytes of a C2 row,
+ * updated in place from the registers. */
+#define XQ (QK + 8)    /* LDS row pitch (bf16) of the XT and V^T tiles */
+#define VK 128         /* reals per workgroup */
+
+template<int NO, int PRE>
+__global__ void __launch_bounds__(512, 2)
+k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, int npad,
+              const double *__restrict__ C1, int Mr2, const __bf16 *__restrict__ XTh,
+              const __bf16 *__restrict__ XTl, int P, size_t K, double *__restrict__ C2, double dte,
+              const double *__restrict__ Kp, int per)
+{
+   constexpr int NG = NO / 64, KG = 8 / NG;        /* waves: NG orbital groups of 64 x KG real groups */
+   constexpr int KW = VK / KG, BJ = KW / 16;       /* reals per wave, 16-real blocks per wave */
+   constexpr int XSZ = NO * XQ, VSZ = VK * XQ;
+   extern __shared__ __bf16 alds[];
+   auto Xat = [&](int bf, int hl) { return alds + (size_t)(2 * bf + hl) * XSZ; };
+   auto Vat = [&](int bf, int hl) { return alds + 4 * (size_t)XSZ + (size_t)(2 * bf + hl) * VSZ; };
+   const int bid = blockIdx.x;
+   const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
+   const size_t kt = (size_t)g * VK;
+   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+   const int wnn = wave % NG, wk = wave / NG;
+   const int q = lane >> 4, rr = lane & 15;
+   /* orbital-row loader: lane = 4 r + j covers rows m0 + 2r, m0 + 2r + 1 at
+    * the k pairs 4 b + j, b = wave + 8 u (64-byte pieces of one row per 4
+    * lanes); projector-row loader: real tid / 4, 8 rows at 8 (tid % 4) */
+   const int orr = lane >> 2, okp = lane & 3;
+
+   f4 acc[4][BJ];
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+   constexpr int XU = NO * 4 / 512;               /* 16-byte XT pieces per thread and array */
+   uint4 rxh[XU], rxl[XU], rvh, rvl;
+   double2 rc[4];
+   auto gload = [&](int p0) {
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+         const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
+         rxh[u] = *reinterpret_cast<const uint4 *>(XTh + (size_t)n * P + p0 + pc);
+         rxl[u] = *reinterpret_cast<const uint4 *>(XTl + (size_t)n * P + p0 + pc);
+      }
+      if (p0 < npad) {
+         const size_t o = (kt + (tid >> 2)) * npad + p0 + (tid & 3) * 8;
+         rvh = *reinterpret_cast<const uint4 *>(BtH + o);
+         rvl = *reinterpret_cast<const uint4 *>(BtL + o);
+      } else {
+         const int m = p0 - npad + 2 * orr;
+#pragma unroll
+         for (int u = 0; u < 2; ++u) {
+            const size_t k = kt + 2 * (4 * (wave + 8 * u) + okp);
+            rc[2 * u] = *reinterpret_cast<const double2 *>(C1 + (size_t)m * K + k);
+            rc[2 * u + 1] = *reinterpret_cast<const double2 *>(C1 + (size_t)(m + 1) * K + k);
+         }
+      }
+   };
+   auto lstore = [&](int bf, int p0) {
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+         const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
+         *reinterpret_cast<uint4 *>(Xat(bf, 0) + n * XQ + pc) = rxh[u];
+         *reinterpret_cast<uint4 *>(Xat(bf, 1) + n * XQ + pc) = rxl[u];
+      }
+      if (p0 < npad) {
+         const int o = (tid >> 2) * XQ + (tid & 3) * 8;
+         *reinterpret_cast<uint4 *>(Vat(bf, 0) + o) = rvh;
+         *reinterpret_cast<uint4 *>(Vat(bf, 1) + o) = rvl;
+      } else {
+#pragma unroll
+         for (int u = 0; u < 2; ++u) {
+            const int k = 2 * (4 * (wave + 8 * u) + okp);
+            unsigned h0, l0, h1, l1;
+            /* (row 2r, row 2r + 1) at reals k and k + 1 */
+            split_pack((float)rc[2 * u].x, (float)rc[2 * u + 1].x, h0, l0);
+            split_pack((float)rc[2 * u].y, (float)rc[2 * u + 1].y, h1, l1);
+            *reinterpret_cast<unsigned *>(Vat(bf, 0) + k * XQ + 2 * orr) = h0;
+            *reinterpret_cast<unsigned *>(Vat(bf, 1) + k * XQ + 2 * orr) = l0;
+            *reinterpret_cast<unsigned *>(Vat(bf, 0) + (k + 1) * XQ + 2 * orr) = h1;
+            *reinterpret_cast<unsigned *>(Vat(bf, 1) + (k + 1) * XQ + 2 * orr) = l1;
+         }
+      }
+   };
+   auto mma = [&](int bf) {
+      bf8 ah[4], al[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+         const int o = (64 * wnn + 16 * i + rr) * XQ + 8 * q;
+         ah[i] = *reinterpret_cast<const bf8 *>(Xat(bf, 0) + o);
+         al[i] = *reinterpret_cast<const bf8 *>(Xat(bf, 1) + o);
+      }
+#pragma unroll
+      for (int j = 0; j < BJ; ++j) {
+         const int o = (KW * wk + 16 * j + rr) * XQ + 8 * q;
+         const bf8 bh = *reinterpret_cast<const bf8 *>(Vat(bf, 0) + o);
+         const bf8 bl = *reinterpret_cast<const bf8 *>(Vat(bf, 1) + o);
+#pragma unroll
+         for (int i = 0; i < 4; ++i) {
+            acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+         }
+      }
+   };
+
+   gload(0);
+   lstore(0, 0);
+   __syncthreads();
+   if (QK < P) gload(QK);
+   int buf = 0;
+   for (int p0 = 0; p0 < P; p0 += QK) {
+      if (p0 + QK < P) lstore(buf ^ 1, p0 + QK);
+      if (p0 + 2 * QK < P) gload(p0 + 2 * QK);
+      mma(buf);
+      __syncthreads();
+      buf ^= 1;
+   }
+
+   /* epilogue by halves of NO/2 orbitals: the waves of the half stage their
+    * sums in LDS ([n][VK + 4] FP32), then every wave updates whole 1 KB row
+    * pieces, lane l the complex coefficient at reals 2l, 2l + 1, with the
+    * loads of ER rows issued before their stores (C2 is read and written) */
+   constexpr int HN = NO / 2, TP = VK + 4, ER = 8;
+   float (*T)[TP] = reinterpret_cast<float (*)[TP]>(alds);
+   const size_t kl = kt + 2 * lane;
+   double f = 0.0;
+   if (PRE) f = Kp[kl >> 1];
+#pragma unroll
+   for (int half = 0; half < 2; ++half) {
+      if (half > 0) __syncthreads();
+      if ((64 * wnn) / HN == half) {
+#pragma unroll
+         for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < BJ; ++j)
+#pragma unroll
+               for (int r = 0; r < 4; ++r)
+                  T[64 * wnn - HN * half + 16 * i + 4 * q + r][KW * wk + 16 * j + rr] = acc[i][j][r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t0 = 0; t0 < HN / 8; t0 += ER) {
+         double2 c2v[ER], c1v[ER];
+#pragma unroll
+         for (int u = 0; u < ER; ++u) {
+            const size_t o = (size_t)(HN * half + wave + 8 * (t0 + u)) * K + kl;
+            c2v[u] = *reinterpret_cast<const double2 *>(C2 + o);
+            if (PRE) c1v[u] = *reinterpret_cast<const double2 *>(C1 + o);
+         }
+#pragma unroll
+         for (int u = 0; u < ER; ++u) {
+            const int nl = wave + 8 * (t0 + u);
+            const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
+            double2 a = c2v[u];
+            if (PRE) {
+               a.x = c1v[u].x + f * (a.x - c1v[u].x + dte * (double)v.x);
+               a.y = c1v[u].y + f * (a.y - c1v[u].y + dte * (double)v.y);
+            } else {
+               a.x += dte * (double)v.x;
+               a.y += dte * (double)v.y;
+            }
+            *reinterpret_cast<double2 *>(C2 + (size_t)(HN * half + nl) * K + kl) = a;
+         }
+      }
+   }
+}
+
+/* XTh, XTl [ne][P] (P = npad + Mr2): the split of X^T (X[p + n ldx], p <
+ * nproj; zero for nproj <= p < npad) followed by X2^T (X2[m + n ldx2],
+ * m < Mr2) */
+__global__ void __launch_bounds__(256) k_split_xt(const double *__restrict__ X, int nproj, int ldx, int npad,
+                                                  const double *__restrict__ X2, int Mr2, int ldx2, int ne,
+                                                  unsigned *__restrict__ XTh, unsigned *__restrict__ XTl)
+{
+   const int P = npad + Mr2, hp = P / 2;
+   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+   if (idx >= (long)ne * hp) return;
+   const int n = (int)(idx / hp), p = 2 * (int)(idx % hp);
+   float v[2];
+#pragma unroll
+   for (int c = 0; c < 2; ++c) {
+      const int pp = p + c;
+      v[c] = pp < npad ? (pp < nproj ? (float)X[pp + (size_t)n * ldx] : 0.f)
+                       : (float)X2[(pp - npad) + (size_t)n * ldx2];
+   }
+   split_pack(v[0], v[1], XTh[idx], XTl[idx]);
+}
+
+/* BtH, BtL [K][npad]: the split projectors transposed (B: [nproj][K] FP64;
+ * zero rows nproj <= p < npad); one thread per (real, projector pair), the
+ * reads of B coalesced along the reals */
+__global__ void __launch_bounds__(256) k_split_bt(const double *__restrict__ B, int nproj, int npad, size_t K,
+                                                  unsigned *__restrict__ BtH, unsigned *__restrict__ BtL)
+{
+   const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+   const int hp = npad / 2;
+   if (idx >= K * hp) return;
+   const size_t k = idx % K;
+   const int p = 2 * (int)(idx / K);
+   const float x = p < nproj ? (float)B[(size_t)p * K + k] : 0.f;
+   const float y = p + 1 < nproj ? (float)B[(size_t)(p + 1) * K + k] : 0.f;
+   split_pack(x, y, BtH[k * hp + p / 2], BtL[k * hp + p / 2]);
+}
+
+int apply_dbf16_ok(int ne)
+{
+   return ne == 128 || ne == 256;
+}
+
+int launch_split_bt(const double *B, int nproj, int npad, size_t K, __bf16 *BtH, __bf16 *BtL, hipStream_t s)
+{
+   if (npad % 32 || npad < nproj) { cpsd_set_error("split_bt: npad=%d for %d projectors", npad, nproj); return -1; }
+   const size_t tot = K * (size_t)(npad / 2);
+   if (tot == 0) return 0;
+   hipLaunchKernelGGL(k_split_bt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, B, nproj, npad, K,
+                      reinterpret_cast<unsigned *>(BtH), reinterpret_cast<unsigned *>(BtL));
+   CPSD_CHECK(hipGetLastError());
+   return 0;
+}
+
+/* C2 (+)= dte (B^T X + C1^T X2) over the ne orbitals (ne = 128 or 256), the
+ * products in split BF16: npad projector rows of the transposed split
+ * projectors (0 for none; X[p + n ldx], p < nproj) and Mr2 orbital rows of C1
+ * (0 for none; X2[m + n ldx2]); precond: C2 = C1 + Kp o (C2 - C1 + dte v).
+ * XTh, XTl: ne (npad + Mr2) bf16 each of scratch.  C1, C2: [ne][K] reals,
+ * K a multiple of 128. */
+int launch_apply_dbf16(const __bf16 *BtH, const __bf16 *BtL, int npad, const double *X, int nproj, int ldx,
+                       const double *C1, int Mr2, const double *X2, int ldx2, int ne, size_t K, double *C2,
+                       double dte, const double *Kp, int precond, __bf16 *XTh, __bf16 *XTl, hipStream_t s)
+{
+   if (!apply_dbf16_ok(ne)) { cpsd_set_error("apply_dbf16: ne=%d is not 128 or 256", ne); return -1; }
+   if (K % VK) { cpsd_set_error("apply_dbf16: K=%zu not a multiple of %d", K, VK); return -1; }
+   if (npad % QK || Mr2 % QK || npad + Mr2 == 0 || npad < nproj) {
+      cpsd_set_error("apply_dbf16: %d projector rows (%d padded) and %d orbital rows", nproj, npad, Mr2);
+      return -1;
+   }
+   if (precond && !Kp) { cpsd_set_error("apply_dbf16: the preconditioned update needs Kp"); return -1; }
+   const int P = npad + Mr2;
+   const long nx = (long)ne * (P / 2);
+   hipLaunchKernelGGL(k_split_xt, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, X, nproj, ldx, npad, X2,
+                      Mr2, ldx2, ne, reinterpret_cast<unsigned *>(XTh), reinterpret_cast<unsigned *>(XTl));
+   CPSD_CHECK(hipGetLastError());
+   const long blocks = (long)(K / VK);
+   const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
+#define APPLY_DBF16(AN_, PRE_)                                                                              \
+   do {                                                                                                     \
+      constexpr int lds = 4 * (AN_ + VK) * XQ * (int)sizeof(__bf16);                                      \
+      static std::atomic<unsigned long long> attr{0};                                                       \
+      if (dyn_lds_once((const void *)k_apply_dbf16<AN_, PRE_>, lds, attr) != 0) return -1;                  \
+      hipLaunchKernelGGL((k_apply_dbf16<AN_, PRE_>), dim3((unsigned)blocks), dim3(512), lds, s, BtH, BtL,    \
+                         npad, C1, Mr2, XTh, XTl, P, K, C2, dte, Kp, per);                                  \
+   } while (0)
+   if (ne == 256) {
+      if (precond) APPLY_DBF16(256, 1); else APPLY_DBF16(256, 0);
+   } else {
+      if (precond) APPLY_DBF16(128, 1); else APPLY_DBF16(128, 0);
+   }
+#undef APPLY_DBF16
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
