@@ -445,6 +445,17 @@ __device__ __forceinline__ void split_pack(float x, float y, unsigned &hi, unsig
    lo = bf16_rne(rx) | (bf16_rne(ry) << 16);
 }
 
+/* element offset of (row, col) in a tile of 32-bf16 (64-byte) rows whose
+ * 16-byte pieces are XOR-swizzled by row: piece c of row r sits at piece
+ * c ^ ((r >> 1) & 3).  A 16-byte read of 8 consecutive rows at one piece
+ * then touches 8 distinct 16-byte bank groups (two rows share each 128-byte
+ * bank line, rows r >> 1 apart are permuted), and the 4-byte writes of two
+ * consecutive rows (r even) fill the 32 banks once. */
+__device__ __forceinline__ int swz(int row, int col)
+{
+   return row * 32 + ((((col >> 3) ^ (row >> 1)) & 3) << 3) + (col & 7);
+}
+
 /* partial[z][p][n] = sum_{k in split z} w_k Bb[p][k] C1[n][k] with BF16
  * projectors Bb, C1 w rounded to BF16 as it is staged in LDS, FP32 sums;
  * the layout, grid and split reduction of k_kbproj_f32.  Lane l supplies 8
@@ -581,7 +592,7 @@ k_kbproj_dbf16(const __bf16 *__restrict__ BsH, const __bf16 *__restrict__ BsL, s
                size_t kchunk, int ntm, int ntile, int per, double *__restrict__ partial)
 {
    constexpr int BI = TM / 32;
-   constexpr int ASZ = TM * QP, BSZ = PN * QP;
+   constexpr int ASZ = TM * 32, BSZ = PN * 32;
    extern __shared__ __bf16 plds[];
    /* [buffer][hi, lo] */
    auto Aat = [&](int bf, int hl) { return plds + (size_t)(2 * bf + hl) * ASZ; };
@@ -625,15 +636,15 @@ k_kbproj_dbf16(const __bf16 *__restrict__ BsH, const __bf16 *__restrict__ BsL, s
    };
    auto lstore = [&](int bf) {
       if (ar < TM) {
-         *reinterpret_cast<uint4 *>(Aat(bf, 0) + ar * QP + aq) = rah;
-         *reinterpret_cast<uint4 *>(Aat(bf, 1) + ar * QP + aq) = ral;
+         *reinterpret_cast<uint4 *>(Aat(bf, 0) + swz(ar, aq)) = rah;
+         *reinterpret_cast<uint4 *>(Aat(bf, 1) + swz(ar, aq)) = ral;
       }
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
          /* the G-space weight (1 or 2, 0 on the padded tail) scales exactly */
          unsigned hi, lo;
          split_pack((float)rb[h].x * (float)rw.x, (float)rb[h].y * (float)rw.y, hi, lo);
-         const int o = (lr + 32 * h) * QP + lq;
+         const int o = swz(lr + 32 * h, lq);
          *reinterpret_cast<unsigned *>(Bat(bf, 0) + o) = hi;
          *reinterpret_cast<unsigned *>(Bat(bf, 1) + o) = lo;
       }
@@ -642,13 +653,13 @@ k_kbproj_dbf16(const __bf16 *__restrict__ BsH, const __bf16 *__restrict__ BsL, s
       bf8 ah[BI], al[BI];
 #pragma unroll
       for (int i = 0; i < BI; ++i) {
-         const int o = ((TM / 2) * wm + 16 * i + rr) * QP + 8 * q;
+         const int o = swz((TM / 2) * wm + 16 * i + rr, 8 * q);
          ah[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 0) + o);
          al[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 1) + o);
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-         const int o = (32 * wn + 16 * j + rr) * QP + 8 * q;
+         const int o = swz(32 * wn + 16 * j + rr, 8 * q);
          const bf8 bh = *reinterpret_cast<const bf8 *>(Bat(bf, 0) + o);
          const bf8 bl = *reinterpret_cast<const bf8 *>(Bat(bf, 1) + o);
 #pragma unroll
@@ -704,13 +715,13 @@ int launch_kbproj_dbf16(const __bf16 *BsH, const __bf16 *BsL, int M, const doubl
    const long pairs = (long)ntile * splits;
    const int per = pairs % 8 == 0 ? (int)(pairs / 8) : 0;
    if (tm == 96) {
-      constexpr int lds = 4 * (96 + PN) * QP * (int)sizeof(__bf16);
+      constexpr int lds = 4 * (96 + PN) * 32 * (int)sizeof(__bf16);
       static std::atomic<unsigned long long> attr{0};
       if (dyn_lds_once((const void *)k_kbproj_dbf16<96>, lds, attr) != 0) return -1;
       hipLaunchKernelGGL((k_kbproj_dbf16<96>), dim3((unsigned)pairs), dim3(512), lds, s, BsH, BsL, K, M, C, K, N,
                          w, K, kchunk, ntm, ntile, per, partial);
    } else {
-      constexpr int lds = 4 * (128 + PN) * QP * (int)sizeof(__bf16);
+      constexpr int lds = 4 * (128 + PN) * 32 * (int)sizeof(__bf16);
       static std::atomic<unsigned long long> attr{0};
       if (dyn_lds_once((const void *)k_kbproj_dbf16<128>, lds, attr) != 0) return -1;
       hipLaunchKernelGGL((k_kbproj_dbf16<128>), dim3((unsigned)pairs), dim3(512), lds, s, BsH, BsL, K, M, C, K,
@@ -757,7 +768,6 @@ int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
  * Blocks are assigned to the 8 XCDs so that the 2R/128 column blocks of
  * one K range run on one XCD and share its L2.  The d x d products are
  * formed in full; the caller reads one triangle. */
-#define DQP (QK + 8)   /* LDS row pitch (bf16), as QP */
 typedef double d2v __attribute__((ext_vector_type(2)));
 
 
@@ -770,7 +780,7 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
    constexpr int RG = R / 64, CG = 8 / RG;         /* waves: RG row groups of 64 x CG column groups */
    constexpr int CW = 128 * CB / CG, BJ = CW / 16; /* columns per wave, 16-column blocks per wave */
    constexpr int NP = R / 32;                      /* loader passes of 32 rows */
-   constexpr int ASZ = R * DQP, BSZ = 128 * CB * DQP;
+   constexpr int ASZ = R * QP, BSZ = 128 * CB * QP;
    extern __shared__ __bf16 dlds[];
    /* [buffer][hi, lo]: A rows of d, B rows of w V */
    auto Aat = [&](int bf, int hl) { return dlds + (size_t)(2 * bf + hl) * ASZ; };
@@ -821,8 +831,8 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
          const float fx = (float)dx, fy = (float)dy;
          unsigned hi, lo;
          split_pack(fx, fy, hi, lo);
-         *reinterpret_cast<unsigned *>(Aat(bf, 0) + row * DQP + lq) = hi;
-         *reinterpret_cast<unsigned *>(Aat(bf, 1) + row * DQP + lq) = lo;
+         *reinterpret_cast<unsigned *>(Aat(bf, 0) + row * QP + lq) = hi;
+         *reinterpret_cast<unsigned *>(Aat(bf, 1) + row * QP + lq) = lo;
          /* the workgroup's column blocks that hold this orbital: block b is
           * c1 (b < R/128) or d, orbitals 128 (b mod R/128) on */
 #pragma unroll
@@ -836,8 +846,8 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
                const float vy = (bd ? fy : (float)r1[h].y) * (float)rw.y;
                const int br = 128 * t + row - 32 * h0;
                split_pack(vx, vy, hi, lo);
-               *reinterpret_cast<unsigned *>(Bat(bf, 0) + br * DQP + lq) = hi;
-               *reinterpret_cast<unsigned *>(Bat(bf, 1) + br * DQP + lq) = lo;
+               *reinterpret_cast<unsigned *>(Bat(bf, 0) + br * QP + lq) = hi;
+               *reinterpret_cast<unsigned *>(Bat(bf, 1) + br * QP + lq) = lo;
             }
          }
       }
@@ -846,13 +856,13 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
       bf8 ah[4], al[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-         const int o = (64 * wr + 16 * i + rr) * DQP + 8 * q;
+         const int o = (64 * wr + 16 * i + rr) * QP + 8 * q;
          ah[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 0) + o);
          al[i] = *reinterpret_cast<const bf8 *>(Aat(bf, 1) + o);
       }
 #pragma unroll
       for (int j = 0; j < BJ; ++j) {
-         const int o = (CW * wc + 16 * j + rr) * DQP + 8 * q;
+         const int o = (CW * wc + 16 * j + rr) * QP + 8 * q;
          const bf8 bh = *reinterpret_cast<const bf8 *>(Bat(bf, 0) + o);
          const bf8 bl = *reinterpret_cast<const bf8 *>(Bat(bf, 1) + o);
 #pragma unroll
@@ -897,7 +907,7 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
  * A (R rows) and B (128 CB rows) */
 static constexpr int gram_dbf16_lds(int R, int CB)
 {
-   return 4 * (R + 128 * CB) * DQP * (int)sizeof(__bf16);
+   return 4 * (R + 128 * CB) * QP * (int)sizeof(__bf16);
 }
 
 int gram_dbf16_ok(int ne)
@@ -1036,31 +1046,33 @@ int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStr
  * three v_mfma_f32_16x16x32_bf16 per block, FP32 sums.
  *   PRE = 0: C2 += dte v (the KB apply, the constraint);
  *   PRE = 1: C2 = C1 + Kp o (C2 - C1 + dte v) (the preconditioned update).
- * One workgroup per 128 reals holds every orbital (NO = ne), so that each
- * V row is read once.  MFMA D[i][j] with i an orbital (A = XT rows, 8
+ * One workgroup per 128 reals and NO = 128 orbitals; the ne / 128 orbital
+ * blocks of one tile of reals read the same V rows.  MFMA D[i][j] with i an orbital (A = XT rows, 8
  * consecutive p per lane) and j a real (B = the staged V^T rows, 8
  * consecutive p per lane); lane (rr, q), register r holds orbital 4q + r,
  * real rr of its block: 16 lanes cover 128 consecutive bytes of a C2 row,
  * updated in place from the registers. */
-#define XQ (QK + 8)    /* LDS row pitch (bf16) of the XT and V^T tiles */
 #define VK 128         /* reals per workgroup */
 
 template<int NO, int PRE>
-__global__ void __launch_bounds__(512, 2)
+__global__ void __launch_bounds__(512, NO == 128 ? 4 : 2)
 k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, int npad,
               const double *__restrict__ C1, int Mr2, const __bf16 *__restrict__ XTh,
               const __bf16 *__restrict__ XTl, int P, size_t K, double *__restrict__ C2, double dte,
-              const double *__restrict__ Kp, int per)
+              const double *__restrict__ Kp, int nob, int per)
 {
    constexpr int NG = NO / 64, KG = 8 / NG;        /* waves: NG orbital groups of 64 x KG real groups */
    constexpr int KW = VK / KG, BJ = KW / 16;       /* reals per wave, 16-real blocks per wave */
-   constexpr int XSZ = NO * XQ, VSZ = VK * XQ;
+   constexpr int XSZ = NO * 32, VSZ = VK * 32;
    extern __shared__ __bf16 alds[];
    auto Xat = [&](int bf, int hl) { return alds + (size_t)(2 * bf + hl) * XSZ; };
    auto Vat = [&](int bf, int hl) { return alds + 4 * (size_t)XSZ + (size_t)(2 * bf + hl) * VSZ; };
    const int bid = blockIdx.x;
    const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
-   const size_t kt = (size_t)g * VK;
+   /* the nob orbital blocks of one tile of reals are consecutive, so they
+    * run on one XCD and read its rows of V from one L2 */
+   const size_t kt = (size_t)(g / nob) * VK;
+   const int n0 = (int)(g % nob) * NO;
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wnn = wave % NG, wk = wave / NG;
    const int q = lane >> 4, rr = lane & 15;
@@ -1082,8 +1094,8 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
 #pragma unroll
       for (int u = 0; u < XU; ++u) {
          const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
-         rxh[u] = *reinterpret_cast<const uint4 *>(XTh + (size_t)n * P + p0 + pc);
-         rxl[u] = *reinterpret_cast<const uint4 *>(XTl + (size_t)n * P + p0 + pc);
+         rxh[u] = *reinterpret_cast<const uint4 *>(XTh + (size_t)(n0 + n) * P + p0 + pc);
+         rxl[u] = *reinterpret_cast<const uint4 *>(XTl + (size_t)(n0 + n) * P + p0 + pc);
       }
       if (p0 < npad) {
          const size_t o = (kt + (tid >> 2)) * npad + p0 + (tid & 3) * 8;
@@ -1103,11 +1115,11 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
 #pragma unroll
       for (int u = 0; u < XU; ++u) {
          const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
-         *reinterpret_cast<uint4 *>(Xat(bf, 0) + n * XQ + pc) = rxh[u];
-         *reinterpret_cast<uint4 *>(Xat(bf, 1) + n * XQ + pc) = rxl[u];
+         *reinterpret_cast<uint4 *>(Xat(bf, 0) + swz(n, pc)) = rxh[u];
+         *reinterpret_cast<uint4 *>(Xat(bf, 1) + swz(n, pc)) = rxl[u];
       }
       if (p0 < npad) {
-         const int o = (tid >> 2) * XQ + (tid & 3) * 8;
+         const int o = swz(tid >> 2, (tid & 3) * 8);
          *reinterpret_cast<uint4 *>(Vat(bf, 0) + o) = rvh;
          *reinterpret_cast<uint4 *>(Vat(bf, 1) + o) = rvl;
       } else {
@@ -1118,10 +1130,10 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
             /* (row 2r, row 2r + 1) at reals k and k + 1 */
             split_pack((float)rc[2 * u].x, (float)rc[2 * u + 1].x, h0, l0);
             split_pack((float)rc[2 * u].y, (float)rc[2 * u + 1].y, h1, l1);
-            *reinterpret_cast<unsigned *>(Vat(bf, 0) + k * XQ + 2 * orr) = h0;
-            *reinterpret_cast<unsigned *>(Vat(bf, 1) + k * XQ + 2 * orr) = l0;
-            *reinterpret_cast<unsigned *>(Vat(bf, 0) + (k + 1) * XQ + 2 * orr) = h1;
-            *reinterpret_cast<unsigned *>(Vat(bf, 1) + (k + 1) * XQ + 2 * orr) = l1;
+            *reinterpret_cast<unsigned *>(Vat(bf, 0) + swz(k, 2 * orr)) = h0;
+            *reinterpret_cast<unsigned *>(Vat(bf, 1) + swz(k, 2 * orr)) = l0;
+            *reinterpret_cast<unsigned *>(Vat(bf, 0) + swz(k + 1, 2 * orr)) = h1;
+            *reinterpret_cast<unsigned *>(Vat(bf, 1) + swz(k + 1, 2 * orr)) = l1;
          }
       }
    };
@@ -1129,13 +1141,13 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
       bf8 ah[4], al[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-         const int o = (64 * wnn + 16 * i + rr) * XQ + 8 * q;
+         const int o = swz(64 * wnn + 16 * i + rr, 8 * q);
          ah[i] = *reinterpret_cast<const bf8 *>(Xat(bf, 0) + o);
          al[i] = *reinterpret_cast<const bf8 *>(Xat(bf, 1) + o);
       }
 #pragma unroll
       for (int j = 0; j < BJ; ++j) {
-         const int o = (KW * wk + 16 * j + rr) * XQ + 8 * q;
+         const int o = swz(KW * wk + 16 * j + rr, 8 * q);
          const bf8 bh = *reinterpret_cast<const bf8 *>(Vat(bf, 0) + o);
          const bf8 bl = *reinterpret_cast<const bf8 *>(Vat(bf, 1) + o);
 #pragma unroll
@@ -1187,7 +1199,7 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
          double2 c2v[ER], c1v[ER];
 #pragma unroll
          for (int u = 0; u < ER; ++u) {
-            const size_t o = (size_t)(HN * half + wave + 8 * (t0 + u)) * K + kl;
+            const size_t o = (size_t)(n0 + HN * half + wave + 8 * (t0 + u)) * K + kl;
             c2v[u] = *reinterpret_cast<const double2 *>(C2 + o);
             if (PRE) c1v[u] = *reinterpret_cast<const double2 *>(C1 + o);
          }
@@ -1203,7 +1215,7 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
                a.x += dte * (double)v.x;
                a.y += dte * (double)v.y;
             }
-            *reinterpret_cast<double2 *>(C2 + (size_t)(HN * half + nl) * K + kl) = a;
+            *reinterpret_cast<double2 *>(C2 + (size_t)(n0 + HN * half + nl) * K + kl) = a;
          }
       }
    }
@@ -1284,21 +1296,21 @@ int launch_apply_dbf16(const __bf16 *BtH, const __bf16 *BtL, int npad, const dou
    hipLaunchKernelGGL(k_split_xt, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, X, nproj, ldx, npad, X2,
                       Mr2, ldx2, ne, reinterpret_cast<unsigned *>(XTh), reinterpret_cast<unsigned *>(XTl));
    CPSD_CHECK(hipGetLastError());
-   const long blocks = (long)(K / VK);
+   /* 128 orbitals per workgroup: 64 KB of LDS and at most 128 VGPRs, two
+    * workgroups per CU (one's C2 update overlaps the other's products) */
+   const int nob = ne / 128;
+   const long blocks = (long)(K / VK) * nob;
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
 #define APPLY_DBF16(AN_, PRE_)                                                                              \
    do {                                                                                                     \
-      constexpr int lds = 4 * (AN_ + VK) * XQ * (int)sizeof(__bf16);                                      \
+      constexpr int lds = 4 * (AN_ + VK) * 32 * (int)sizeof(__bf16);                                      \
       static std::atomic<unsigned long long> attr{0};                                                       \
       if (dyn_lds_once((const void *)k_apply_dbf16<AN_, PRE_>, lds, attr) != 0) return -1;                  \
       hipLaunchKernelGGL((k_apply_dbf16<AN_, PRE_>), dim3((unsigned)blocks), dim3(512), lds, s, BtH, BtL,    \
-                         npad, C1, Mr2, XTh, XTl, P, K, C2, dte, Kp, per);                                  \
+                         npad, C1, Mr2, XTh, XTl, P, K, C2, dte, Kp, nob, per);                             \
    } while (0)
-   if (ne == 256) {
-      if (precond) APPLY_DBF16(256, 1); else APPLY_DBF16(256, 0);
-   } else {
-      if (precond) APPLY_DBF16(128, 1); else APPLY_DBF16(128, 0);
-   }
+   if (precond) APPLY_DBF16(128, 1);
+   else APPLY_DBF16(128, 0);
 #undef APPLY_DBF16
    CPSD_CHECK(hipGetLastError());
    return 0;
