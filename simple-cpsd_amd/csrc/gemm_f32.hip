@@ -1046,8 +1046,9 @@ int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStr
  * three v_mfma_f32_16x16x32_bf16 per block, FP32 sums.
  *   PRE = 0: C2 += dte v (the KB apply, the constraint);
  *   PRE = 1: C2 = C1 + Kp o (C2 - C1 + dte v) (the preconditioned update).
- * One workgroup per 128 reals and NO = 128 orbitals; the ne / 128 orbital
- * blocks of one tile of reals read the same V rows.  MFMA D[i][j] with i an orbital (A = XT rows, 8
+ * One workgroup of 4 NO threads per 128 reals holds all NO = ne orbitals,
+ * so that each V row is staged once (16 waves for 256 orbitals: four per
+ * SIMD at 96 KB of LDS).  MFMA D[i][j] with i an orbital (A = XT rows, 8
  * consecutive p per lane) and j a real (B = the staged V^T rows, 8
  * consecutive p per lane); lane (rr, q), register r holds orbital 4q + r,
  * real rr of its block: 16 lanes cover 128 consecutive bytes of a C2 row,
@@ -1055,13 +1056,14 @@ int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStr
 #define VK 128         /* reals per workgroup */
 
 template<int NO, int PRE>
-__global__ void __launch_bounds__(512, NO == 128 ? 4 : 2)
+__global__ void __launch_bounds__(4 * NO, 4)
 k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, int npad,
               const double *__restrict__ C1, int Mr2, const __bf16 *__restrict__ XTh,
               const __bf16 *__restrict__ XTl, int P, size_t K, double *__restrict__ C2, double dte,
-              const double *__restrict__ Kp, int nob, int per)
+              const double *__restrict__ Kp, int per)
 {
-   constexpr int NG = NO / 64, KG = 8 / NG;        /* waves: NG orbital groups of 64 x KG real groups */
+   constexpr int NTH = 4 * NO, NWV = NTH / 64;     /* threads, waves */
+   constexpr int NG = NO / 64, KG = NWV / NG;      /* waves: NG orbital groups of 64 x KG real groups */
    constexpr int KW = VK / KG, BJ = KW / 16;       /* reals per wave, 16-real blocks per wave */
    constexpr int XSZ = NO * 32, VSZ = VK * 32;
    extern __shared__ __bf16 alds[];
@@ -1069,17 +1071,17 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
    auto Vat = [&](int bf, int hl) { return alds + 4 * (size_t)XSZ + (size_t)(2 * bf + hl) * VSZ; };
    const int bid = blockIdx.x;
    const long g = per > 0 ? (long)(bid & 7) * per + (bid >> 3) : bid;
-   /* the nob orbital blocks of one tile of reals are consecutive, so they
-    * run on one XCD and read its rows of V from one L2 */
-   const size_t kt = (size_t)(g / nob) * VK;
-   const int n0 = (int)(g % nob) * NO;
+   const size_t kt = (size_t)g * VK;
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wnn = wave % NG, wk = wave / NG;
    const int q = lane >> 4, rr = lane & 15;
    /* orbital-row loader: lane = 4 r + j covers rows m0 + 2r, m0 + 2r + 1 at
-    * the k pairs 4 b + j, b = wave + 8 u (64-byte pieces of one row per 4
-    * lanes); projector-row loader: real tid / 4, 8 rows at 8 (tid % 4) */
+    * the k pairs 4 b + j, b = wave + NWV u (64-byte pieces of one row per 4
+    * lanes); projector-row loader: piece c of the 1024 (hi, then lo) is
+    * real (c % 512) / 4, rows 8 (c % 4) on */
    const int orr = lane >> 2, okp = lane & 3;
+   constexpr int CU = 16 / NWV;                    /* orbital-row items per thread */
+   constexpr int PU = 1024 / NTH;                  /* projector pieces per thread */
 
    f4 acc[4][BJ];
 #pragma unroll
@@ -1087,45 +1089,47 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
 #pragma unroll
       for (int j = 0; j < BJ; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-   constexpr int XU = NO * 4 / 512;               /* 16-byte XT pieces per thread and array */
-   uint4 rxh[XU], rxl[XU], rvh, rvl;
-   double2 rc[4];
+   uint4 rxh, rxl, rv[PU];
+   double2 rc[2 * CU];
    auto gload = [&](int p0) {
-#pragma unroll
-      for (int u = 0; u < XU; ++u) {
-         const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
-         rxh[u] = *reinterpret_cast<const uint4 *>(XTh + (size_t)(n0 + n) * P + p0 + pc);
-         rxl[u] = *reinterpret_cast<const uint4 *>(XTl + (size_t)(n0 + n) * P + p0 + pc);
+      {
+         const int n = tid >> 2, pc = (tid & 3) * 8;
+         rxh = *reinterpret_cast<const uint4 *>(XTh + (size_t)n * P + p0 + pc);
+         rxl = *reinterpret_cast<const uint4 *>(XTl + (size_t)n * P + p0 + pc);
       }
       if (p0 < npad) {
-         const size_t o = (kt + (tid >> 2)) * npad + p0 + (tid & 3) * 8;
-         rvh = *reinterpret_cast<const uint4 *>(BtH + o);
-         rvl = *reinterpret_cast<const uint4 *>(BtL + o);
+#pragma unroll
+         for (int u = 0; u < PU; ++u) {
+            const int c = tid + NTH * u, cc = c & 511;
+            const __bf16 *src = c < 512 ? BtH : BtL;
+            rv[u] = *reinterpret_cast<const uint4 *>(src + (kt + (cc >> 2)) * npad + p0 + (cc & 3) * 8);
+         }
       } else {
          const int m = p0 - npad + 2 * orr;
 #pragma unroll
-         for (int u = 0; u < 2; ++u) {
-            const size_t k = kt + 2 * (4 * (wave + 8 * u) + okp);
+         for (int u = 0; u < CU; ++u) {
+            const size_t k = kt + 2 * (4 * (wave + NWV * u) + okp);
             rc[2 * u] = *reinterpret_cast<const double2 *>(C1 + (size_t)m * K + k);
             rc[2 * u + 1] = *reinterpret_cast<const double2 *>(C1 + (size_t)(m + 1) * K + k);
          }
       }
    };
    auto lstore = [&](int bf, int p0) {
-#pragma unroll
-      for (int u = 0; u < XU; ++u) {
-         const int c = tid + 512 * u, n = c >> 2, pc = (c & 3) * 8;
-         *reinterpret_cast<uint4 *>(Xat(bf, 0) + swz(n, pc)) = rxh[u];
-         *reinterpret_cast<uint4 *>(Xat(bf, 1) + swz(n, pc)) = rxl[u];
+      {
+         const int n = tid >> 2, pc = (tid & 3) * 8;
+         *reinterpret_cast<uint4 *>(Xat(bf, 0) + swz(n, pc)) = rxh;
+         *reinterpret_cast<uint4 *>(Xat(bf, 1) + swz(n, pc)) = rxl;
       }
       if (p0 < npad) {
-         const int o = swz(tid >> 2, (tid & 3) * 8);
-         *reinterpret_cast<uint4 *>(Vat(bf, 0) + o) = rvh;
-         *reinterpret_cast<uint4 *>(Vat(bf, 1) + o) = rvl;
+#pragma unroll
+         for (int u = 0; u < PU; ++u) {
+            const int c = tid + NTH * u, cc = c & 511;
+            *reinterpret_cast<uint4 *>(Vat(bf, c >> 9) + swz(cc >> 2, (cc & 3) * 8)) = rv[u];
+         }
       } else {
 #pragma unroll
-         for (int u = 0; u < 2; ++u) {
-            const int k = 2 * (4 * (wave + 8 * u) + okp);
+         for (int u = 0; u < CU; ++u) {
+            const int k = 2 * (4 * (wave + NWV * u) + okp);
             unsigned h0, l0, h1, l1;
             /* (row 2r, row 2r + 1) at reals k and k + 1 */
             split_pack((float)rc[2 * u].x, (float)rc[2 * u + 1].x, h0, l0);
@@ -1175,8 +1179,9 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
    /* epilogue by halves of NO/2 orbitals: the waves of the half stage their
     * sums in LDS ([n][VK + 4] FP32), then every wave updates whole 1 KB row
     * pieces, lane l the complex coefficient at reals 2l, 2l + 1, with the
-    * loads of ER rows issued before their stores (C2 is read and written) */
-   constexpr int HN = NO / 2, TP = VK + 4, ER = 8;
+    * loads of its HN / NWV rows issued before their stores (C2 is read and
+    * written) */
+   constexpr int HN = NO / 2, TP = VK + 4, ER = HN / NWV;
    float (*T)[TP] = reinterpret_cast<float (*)[TP]>(alds);
    const size_t kl = kt + 2 * lane;
    double f = 0.0;
@@ -1194,29 +1199,26 @@ k_apply_dbf16(const __bf16 *__restrict__ BtH, const __bf16 *__restrict__ BtL, in
                   T[64 * wnn - HN * half + 16 * i + 4 * q + r][KW * wk + 16 * j + rr] = acc[i][j][r];
       }
       __syncthreads();
+      double2 c2v[ER], c1v[ER];
 #pragma unroll
-      for (int t0 = 0; t0 < HN / 8; t0 += ER) {
-         double2 c2v[ER], c1v[ER];
+      for (int u = 0; u < ER; ++u) {
+         const size_t o = (size_t)(HN * half + wave + NWV * u) * K + kl;
+         c2v[u] = *reinterpret_cast<const double2 *>(C2 + o);
+         if (PRE) c1v[u] = *reinterpret_cast<const double2 *>(C1 + o);
+      }
 #pragma unroll
-         for (int u = 0; u < ER; ++u) {
-            const size_t o = (size_t)(n0 + HN * half + wave + 8 * (t0 + u)) * K + kl;
-            c2v[u] = *reinterpret_cast<const double2 *>(C2 + o);
-            if (PRE) c1v[u] = *reinterpret_cast<const double2 *>(C1 + o);
+      for (int u = 0; u < ER; ++u) {
+         const int nl = wave + NWV * u;
+         const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
+         double2 a = c2v[u];
+         if (PRE) {
+            a.x = c1v[u].x + f * (a.x - c1v[u].x + dte * (double)v.x);
+            a.y = c1v[u].y + f * (a.y - c1v[u].y + dte * (double)v.y);
+         } else {
+            a.x += dte * (double)v.x;
+            a.y += dte * (double)v.y;
          }
-#pragma unroll
-         for (int u = 0; u < ER; ++u) {
-            const int nl = wave + 8 * (t0 + u);
-            const float2 v = *reinterpret_cast<const float2 *>(&T[nl][2 * lane]);
-            double2 a = c2v[u];
-            if (PRE) {
-               a.x = c1v[u].x + f * (a.x - c1v[u].x + dte * (double)v.x);
-               a.y = c1v[u].y + f * (a.y - c1v[u].y + dte * (double)v.y);
-            } else {
-               a.x += dte * (double)v.x;
-               a.y += dte * (double)v.y;
-            }
-            *reinterpret_cast<double2 *>(C2 + (size_t)(n0 + HN * half + nl) * K + kl) = a;
-         }
+         *reinterpret_cast<double2 *>(C2 + (size_t)(HN * half + nl) * K + kl) = a;
       }
    }
 }
@@ -1296,21 +1298,21 @@ int launch_apply_dbf16(const __bf16 *BtH, const __bf16 *BtL, int npad, const dou
    hipLaunchKernelGGL(k_split_xt, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, X, nproj, ldx, npad, X2,
                       Mr2, ldx2, ne, reinterpret_cast<unsigned *>(XTh), reinterpret_cast<unsigned *>(XTl));
    CPSD_CHECK(hipGetLastError());
-   /* 128 orbitals per workgroup: 64 KB of LDS and at most 128 VGPRs, two
-    * workgroups per CU (one's C2 update overlaps the other's products) */
-   const int nob = ne / 128;
-   const long blocks = (long)(K / VK) * nob;
+   const long blocks = (long)(K / VK);
    const int per = blocks % 8 == 0 ? (int)(blocks / 8) : 0;
 #define APPLY_DBF16(AN_, PRE_)                                                                              \
    do {                                                                                                     \
       constexpr int lds = 4 * (AN_ + VK) * 32 * (int)sizeof(__bf16);                                      \
       static std::atomic<unsigned long long> attr{0};                                                       \
       if (dyn_lds_once((const void *)k_apply_dbf16<AN_, PRE_>, lds, attr) != 0) return -1;                  \
-      hipLaunchKernelGGL((k_apply_dbf16<AN_, PRE_>), dim3((unsigned)blocks), dim3(512), lds, s, BtH, BtL,    \
-                         npad, C1, Mr2, XTh, XTl, P, K, C2, dte, Kp, nob, per);                             \
+      hipLaunchKernelGGL((k_apply_dbf16<AN_, PRE_>), dim3((unsigned)blocks), dim3(4 * AN_), lds, s, BtH,    \
+                         BtL, npad, C1, Mr2, XTh, XTl, P, K, C2, dte, Kp, per);                             \
    } while (0)
-   if (precond) APPLY_DBF16(128, 1);
-   else APPLY_DBF16(128, 0);
+   if (ne == 256) {
+      if (precond) APPLY_DBF16(256, 1); else APPLY_DBF16(256, 0);
+   } else {
+      if (precond) APPLY_DBF16(128, 1); else APPLY_DBF16(128, 0);
+   }
 #undef APPLY_DBF16
    CPSD_CHECK(hipGetLastError());
    return 0;
