@@ -13,6 +13,16 @@
  * the orbitals stay FP64 in HBM and are rounded to FP32 while they are
  * staged in LDS.
  *
+ * Precision mode 2 adds the BF16-input projection (k_kbproj_bf16); mode 3
+ * runs every product of its FP32 steps in split BF16 (three
+ * v_mfma_f32_16x16x32_bf16 per block on hi/lo halves of each FP32 factor):
+ *   k_kbproj_dbf16 : the KB projection;
+ *   k_apply_dbf16  : the KB apply, the preconditioned update (B X + C1 h in
+ *                    one pass) and the constraint (C1 lmd);
+ *   k_gram_dbf16   : the overlaps of the step d = c2 - c1, <d|c1> and
+ *                    <d|d>, which k_gram_dfix assembles with the carried
+ *                    FP64 <c1|c1> (ccpsdv3/cpsd.c:415-437).
+ *
  * v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and B[l/16][l%16];
  * lane l, register r receives D[4(l/16)+r][l%16].  A sum over k may assign
  * its terms to the instruction's 4 k slots in any order, as long as A and B
