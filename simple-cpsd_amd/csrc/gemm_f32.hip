@@ -781,15 +781,17 @@ int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
 typedef double d2v __attribute__((ext_vector_type(2)));
 
 
-template<int R, int CB>
-__global__ void __launch_bounds__(512, 2)
+template<int R, int CB, int NT>
+__global__ void __launch_bounds__(NT, NT == 1024 ? 4 : 2)
 k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_t ldc,
              const double *__restrict__ w, size_t K, size_t kchunk, int per, double *__restrict__ partial)
 {
    constexpr int NW = 2 * R / 128 / CB;            /* workgroups per K range, CB column blocks each */
-   constexpr int RG = R / 64, CG = 8 / RG;         /* waves: RG row groups of 64 x CG column groups */
+   constexpr int NWV = NT / 64;                    /* waves */
+   constexpr int RG = R / 64, CG = NWV / RG;       /* waves: RG row groups of 64 x CG column groups */
    constexpr int CW = 128 * CB / CG, BJ = CW / 16; /* columns per wave, 16-column blocks per wave */
-   constexpr int NP = R / 32;                      /* loader passes of 32 rows */
+   constexpr int LR = NT / 16, NP = R / LR;        /* rows per loader pass, passes */
+   constexpr int PB = 128 / LR;                    /* loader passes per 128-row block */
    constexpr int ASZ = R * QP, BSZ = 128 * CB * QP;
    extern __shared__ __bf16 dlds[];
    /* [buffer][hi, lo]: A rows of d, B rows of w V */
@@ -803,7 +805,7 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
    const int wr = wave % RG, wc = wave / RG;
    const int q = lane >> 4, rr = lane & 15;
-   /* loader: rows lr + 32 h, k pair lq; 16 lanes cover one row's 32 k */
+   /* loader: rows lr + LR h, k pair lq; 16 lanes cover one row's 32 k */
    const int lr = tid >> 4, lq = (tid & 15) * 2;
 
    f4 acc[4][BJ];
@@ -827,7 +829,7 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
    auto gload = [&](size_t k0) {
 #pragma unroll
       for (int h = 0; h < NP; ++h) {
-         const unsigned so = (unsigned)(8 * ((size_t)(32 * h) * ldc + k0));
+         const unsigned so = (unsigned)(8 * ((size_t)(LR * h) * ldc + k0));
          r1[h] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rc1, vb, so, 0));
          r2[h] = __builtin_bit_cast(d2v, __builtin_amdgcn_raw_buffer_load_b128(rc2, vb, so, 0));
       }
@@ -836,7 +838,7 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
    auto lstore = [&](int bf) {
 #pragma unroll
       for (int h = 0; h < NP; ++h) {
-         const int row = lr + 32 * h;
+         const int row = lr + LR * h;
          const double dx = r2[h].x - r1[h].x, dy = r2[h].y - r1[h].y;
          const float fx = (float)dx, fy = (float)dy;
          unsigned hi, lo;
@@ -849,12 +851,12 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
          for (int t = 0; t < CB; ++t) {
             const int b = blk0 + t;
             const bool bd = b >= R / 128;
-            const int h0 = (b % (R / 128)) * 4;
-            if (h >= h0 && h < h0 + 4) {
+            const int h0 = (b % (R / 128)) * PB;
+            if (h >= h0 && h < h0 + PB) {
                /* the G-space weight (1 or 2, 0 on the padded tail) scales exactly */
                const float vx = (bd ? fx : (float)r1[h].x) * (float)rw.x;
                const float vy = (bd ? fy : (float)r1[h].y) * (float)rw.y;
-               const int br = 128 * t + row - 32 * h0;
+               const int br = 128 * t + row - LR * h0;
                split_pack(vx, vy, hi, lo);
                *reinterpret_cast<unsigned *>(Bat(bf, 0) + br * QP + lq) = hi;
                *reinterpret_cast<unsigned *>(Bat(bf, 1) + br * QP + lq) = lo;
@@ -931,15 +933,18 @@ int gram_dbf16_fits(int ne, size_t K)
    return (double)ne * (double)K * 8.0 < 4294967296.0;
 }
 
+/* 8 waves per workgroup (16 waves for the 256 x 256 tiles spilled 65 VGPRs
+ * at the 128 a wave has then) */
 template<int R, int CB>
 static int launch_gram_dbf16_t(const double *C1, const double *C2, const double *w, size_t K, size_t kchunk,
                                long pairs, int per, double *partial, hipStream_t s)
 {
    constexpr int lds = gram_dbf16_lds(R, CB);
+   constexpr int NT = 512;
    static std::atomic<unsigned long long> attr{0};
-   if (dyn_lds_once((const void *)k_gram_dbf16<R, CB>, lds, attr) != 0) return -1;
-   hipLaunchKernelGGL((k_gram_dbf16<R, CB>), dim3((unsigned)pairs), dim3(512), lds, s, C1, C2, K, w, K, kchunk,
-                      per, partial);
+   if (dyn_lds_once((const void *)k_gram_dbf16<R, CB, NT>, lds, attr) != 0) return -1;
+   hipLaunchKernelGGL((k_gram_dbf16<R, CB, NT>), dim3((unsigned)pairs), dim3(NT), lds, s, C1, C2, K, w, K,
+                      kchunk, per, partial);
    CPSD_CHECK(hipGetLastError());
    return 0;
 }
