@@ -64,6 +64,25 @@ __device__ __forceinline__ void stage_rows(const RowTable &rows, int ky, int (*r
  * orbital b+1 is issued into the registers it is loaded into while orbital
  * b is transformed and written, so each workgroup keeps one column block of
  * loads in flight */
+/* The z passes' logical workgroup (x, y, z) = (kx block, ky row, orbitals).
+ * The hardware hands workgroups to the 8 XCDs round-robin in linear order
+ * (x fastest), so neighbouring kx blocks and ky rows, which share cache
+ * lines of T and of the packed orbitals at their edges, would each fetch
+ * them into another L2.  Logical ids are regrouped so that every XCD runs
+ * a contiguous range of them; a grid whose size is not a multiple of 8
+ * keeps the hardware order. */
+__device__ __forceinline__ void zpass_block(int &x, int &y, int &z)
+{
+   const unsigned long nx = gridDim.x, ny = gridDim.y;
+   const unsigned long tot = nx * ny * gridDim.z;
+   unsigned long l = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+   if (tot % 8 == 0) l = (l & 7) * (tot / 8) + (l >> 3);
+   x = (int)(l % nx);
+   l /= nx;
+   y = (int)(l % ny);
+   z = (int)(l / ny);
+}
+
 template<int N, int SRC, typename V>
 __global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F32 : ZPASS_G2R_WAVES_F64(N)) k_zpass_g2r(ZArgs a)
 {
@@ -72,8 +91,10 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F
    constexpr int EPT = (N * KXB + ZT - 1) / ZT;   /* elements per thread */
    __shared__ V sm[N * KXB];
    __shared__ int rt[3][N];
-   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y;
-   const int b0 = blockIdx.z * a.bper, b1 = min(a.nbatch, b0 + a.bper);
+   int bx, ky, bz;
+   zpass_block(bx, ky, bz);
+   const int kx0 = bx * KXB;
+   const int b0 = bz * a.bper, b1 = min(a.nbatch, b0 + a.bper);
    const int ncol = min(KXB, a.rows.nhc - kx0);
    /* twiddles are read from LDS (the barrier in stage_rows publishes them) */
    __shared__ V tws[N];
@@ -141,7 +162,9 @@ __global__ void __launch_bounds__(GridCfg<N>::ZT, sizeof(V) == 8 ? ZPASS_WAVES_F
    constexpr int EPT = (N * KXB + ZT - 1) / ZT;
    __shared__ V sm[N * KXB];
    __shared__ int rt[3][N];
-   const int kx0 = blockIdx.x * KXB, ky = blockIdx.y, b = blockIdx.z;
+   int bx, ky, b;
+   zpass_block(bx, ky, b);
+   const int kx0 = bx * KXB;
    const int ncol = min(KXB, a.rows.nhc - kx0);
    const V *T = reinterpret_cast<const V *>(a.T) + (size_t)b * N * N * NH;
    __shared__ V tws[N];
