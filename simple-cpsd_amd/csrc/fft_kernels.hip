@@ -407,9 +407,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
       for (int q = 0; q < ACC; ++q) acc[q] = 0.0;
       const int bb0 = a.b0 + blockIdx.y * a.nb;
       const int bb1 = min(bb0 + a.nb, a.nbatch);
-      for (int b = bb0; b < bb1; ++b) {
-         load_plane<N>(sm, Tb + ((size_t)b * N + z) * plane, a.nhload);
-         inverse_yx<N>(sm, tws);
+      auto accumulate = [&]() {
          const R *smr = reinterpret_cast<const R *>(sm);
 #pragma unroll
          for (int q = 0; q < ACC; ++q) {
@@ -421,6 +419,41 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
             }
          }
          __syncthreads();
+      };
+      /* as in the V psi pass: an FP64 plane that fills more than half the
+       * LDS leaves one workgroup per CU, so the next orbital's plane is
+       * fetched into registers while the current one is transformed */
+      constexpr bool PF = sizeof(V) == 16 && (size_t)N * NH * sizeof(V) > 80 * 1024;
+      if constexpr (PF) {
+         constexpr int EPT = (N * NH + NT - 1) / NT;
+         V nx[EPT];
+         auto fetch = [&](int b) {
+            const V *Tq = Tb + ((size_t)b * N + z) * plane;
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+               const int e = threadIdx.x + NT * q;
+               nx[q] = (e < N * NH && e % NH < a.nhload) ? Tq[e] : Cx<V>::mk(0, 0);
+            }
+         };
+         if (bb0 < bb1) fetch(bb0);
+         for (int b = bb0; b < bb1; ++b) {
+            /* the previous orbital's sums have read the LDS plane */
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+               const int e = threadIdx.x + NT * q;
+               if (e < N * NH) sm[e] = nx[q];
+            }
+            __syncthreads();
+            if (b + 1 < bb1) fetch(b + 1);
+            inverse_yx<N>(sm, tws);
+            accumulate();
+         }
+      } else {
+         for (int b = bb0; b < bb1; ++b) {
+            load_plane<N>(sm, Tb + ((size_t)b * N + z) * plane, a.nhload);
+            inverse_yx<N>(sm, tws);
+            accumulate();
+         }
       }
       double *dst = a.dnpart + ((size_t)blockIdx.y * N + z) * N * N;
 #pragma unroll
