@@ -915,6 +915,14 @@ k_gram_dbf16(const double *__restrict__ C1, const double *__restrict__ C2, size_
       }
 }
 
+/* workgroups per CU the Gram of d is split into (its partials: splits x ne x
+ * 2ne doubles, reduced in FP64 afterwards).  One round of the GPU measured
+ * 0.05 ms faster per 64-water step than two (fewer partials to reduce) and
+ * 0.2 ms faster than four. */
+#ifndef GRAM_DBF16_WPC
+#define GRAM_DBF16_WPC 1
+#endif
+
 /* dynamic LDS of k_gram_dbf16<R, CB>: two buffers of the hi and lo rows of
  * A (R rows) and B (128 CB rows) */
 static constexpr int gram_dbf16_lds(int R, int CB)
@@ -951,10 +959,9 @@ static int launch_gram_dbf16_t(const double *C1, const double *C2, const double 
 
 /* partial[z][i][n] of k_gram_dbf16 for the ne orbitals of C1 and C2 (rows of
  * K reals): two 128-column blocks per workgroup where the LDS holds their
- * buffers (every K range read by ne / 128 workgroups), else one; two
- * workgroups per CU over the K splits (one round of the GPU at one
- * workgroup per CU, long K ranges, few partials); *splits receives their
- * number */
+ * buffers (every K range read by ne / 128 workgroups), else one;
+ * GRAM_DBF16_WPC workgroups per CU over the K splits (long K ranges, few
+ * partials); *splits receives their number */
 int launch_gram_dbf16(const double *C1, const double *C2, int ne, const double *w, size_t K, double *partial,
                       size_t partial_cap, int *splits, hipStream_t s)
 {
@@ -968,7 +975,7 @@ int launch_gram_dbf16(const double *C1, const double *C2, int ne, const double *
    CPSD_CHECK(hipGetDevice(&dev));
    CPSD_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
    CPSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-   const long wgs = 2L * (cus > 0 ? cus : 1);
+   const long wgs = (long)GRAM_DBF16_WPC * (cus > 0 ? cus : 1);
 #ifdef GRAM_DBF16_CB
    const int cb = GRAM_DBF16_CB;   /* build-time override for measurement */
 #else
