@@ -91,7 +91,8 @@ def test_h2o64_learned_preconditioner_converges(h2o64):
     plain FP64 SD's after 40 steps, and orthonormal orbitals at the end (the
     first preconditioned steps move c2 far enough from orthonormality that
     the Lagrange iteration stops at its cap of 30 with deviations of 1e-5;
-    they vanish as the steps shrink)"""
+    they vanish as the steps shrink); then mode 3 from the same start, to
+    the same minimum within 1e-8 Ha per atom in as many steps"""
     import learned
     cell, psi, dev = h2o64
     say("h2o-64 learned")
@@ -114,6 +115,23 @@ def test_h2o64_learned_preconditioner_converges(h2o64):
     assert dev.fallback_info()["fallbacks"] == 0
     assert E[0] < E0[-1][0]
     ep = dev.epilogue()
+    assert np.max(np.abs(ep["ortho"] - np.eye(NE))) < 1e-10
+    # precision mode 3 (split-BF16 products and step overlaps) from the same
+    # start: the same minimum within 1e-8 Ha per atom, in as many steps
+    E1, n1 = E[0], n
+    dev.set_precision(3)
+    dev.set_orbitals(psi)
+    dev.E[:] = 0.0
+    Eprev, n = None, 0
+    while n < 1500:
+        E = dev.run(10)[0]
+        n += 10
+        if Eprev is not None and abs(E[0] - Eprev) < 1e-7:
+            break
+        Eprev = E[0]
+    say("mode 3 + learned: %d steps, E %.10f (mode 1: %d, %.10f)" % (n, E[0], n1, E1))
+    fb = dev.fallback_info()["fallbacks"]
     dev.set_precision(0)
     dev.set_preconditioner(None)
-    assert np.max(np.abs(ep["ortho"] - np.eye(NE))) < 1e-10
+    assert n <= n1 + 20 and fb == 0
+    assert abs(E[0] - E1) / (3 * NMOL) < 1e-8
