@@ -722,10 +722,10 @@ static int sd_step(cpsd_dev *c, double dte, int last, int fp64)
       memset(&pa, 0, sizeof pa);
       pa.T = c->T + (size_t)b0 * plane; pa.nhload = c->nhc; pa.tw = c->tw; pa.V = c->V;
       pa.nbatch = bs; pa.fp32 = f32;
-      /* orbitals per workgroup: the FP64 pass at N >= 112 prefetches the
-       * next one's plane, up to 4 while the grid keeps about 4 workgroups
-       * per CU (the kernel takes one otherwise) */
-      pa.nb = (f32 || n < 112) ? 1 : (int)std::max(1L, std::min(4L, (long)bs * n / 1024));
+      /* orbitals per workgroup: an FP64 pass that prefetches the next
+       * orbital's plane (plane_prefetch_on) takes up to 4 while the grid
+       * keeps about 4 workgroups per CU (the kernel takes one otherwise) */
+      pa.nb = (f32 || !plane_prefetch_on(n)) ? 1 : (int)std::max(1L, std::min(4L, (long)bs * n / 1024));
       TRY(launch_plane(n, PM_VPSI, pa, (bs + pa.nb - 1) / pa.nb, st));
       TRY(ph_end(c, PH_VPSI));
       TRY(ph_begin(c, PH_ZR2G));

@@ -110,6 +110,9 @@ int launch_zpass_r2g(int n, int dst, const ZArgs &a, int nbatch, hipStream_t s);
 int launch_plane(int n, int mode, const PlaneArgs &a, int nblocks, hipStream_t s);
 /* workgroups of the density plane pass one CU holds at once */
 int plane_density_occupancy(int n, int fp32);
+/* the FP64 density and V psi passes at grid n prefetch the next orbital's
+ * plane (several orbitals per workgroup pay for it) */
+int plane_prefetch_on(int n);
 
 /* ---- GEMM ------------------------------------------------------------- */
 

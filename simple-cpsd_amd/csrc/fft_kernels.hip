@@ -369,13 +369,29 @@ constexpr int plane_lds_waves()
  * orbital passes at N >= 80, but never more than the LDS admits (at
  * N = 128 the FP32 planes fit two workgroups, 4 waves per SIMD: sizing the
  * registers for 6 only spilled them) */
+/* the FP64 plane passes (density, V psi) fetch the next orbital's plane
+ * into registers while the current one is transformed when a plane is
+ * larger than PLANE_PF_KB KB: such planes leave one workgroup per CU, whose
+ * HBM reads would otherwise wait on its LDS transforms.  The 75 KB planes
+ * of 96^3 fit two workgroups per CU, which overlap each other: prefetching
+ * there (one workgroup per CU for the registers) measured slower, V psi
+ * 0.51 -> 0.69 ms on 32 waters. */
+#ifndef PLANE_PF_KB
+#define PLANE_PF_KB 80
+#endif
+template<int N, typename V>
+constexpr bool plane_prefetch()
+{
+   return sizeof(V) == 16 && (size_t)N * (N / 2 + 1) * sizeof(V) > (size_t)PLANE_PF_KB * 1024;
+}
+
 template<int N, int MODE, typename V>
 constexpr int plane_min_waves()
 {
    constexpr int lw = plane_lds_waves<N, V>() > 0 ? plane_lds_waves<N, V>() : 1;
    constexpr int want = (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
                       : (sizeof(V) == 8 && N >= 80 && MODE == PM_VPSI) ? VPSI_WAVES_F32
-                      : (N == 96 && MODE == PM_DENSITY) ? 4 : 1;
+                      : (N == 96 && MODE == PM_DENSITY && !plane_prefetch<N, V>()) ? 4 : 1;
    return want < lw ? want : lw;
 }
 
@@ -423,7 +439,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
       /* as in the V psi pass: an FP64 plane that fills more than half the
        * LDS leaves one workgroup per CU, so the next orbital's plane is
        * fetched into registers while the current one is transformed */
-      constexpr bool PF = sizeof(V) == 16 && (size_t)N * NH * sizeof(V) > 80 * 1024;
+      constexpr bool PF = plane_prefetch<N, V>();
       if constexpr (PF) {
          constexpr int EPT = (N * NH + NT - 1) / NT;
          V nx[EPT];
@@ -495,7 +511,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
        * overlap the LDS transforms that would otherwise leave the CU idle
        * between planes.  Smaller planes leave room for two workgroups per
        * CU, which overlap each other; they take one orbital each. */
-      constexpr bool PF = sizeof(V) == 16 && (size_t)N * NH * sizeof(V) > 80 * 1024;
+      constexpr bool PF = plane_prefetch<N, V>();
       constexpr int EPT = (N * NH + NT - 1) / NT;
       constexpr int EPV = (N * M + NT - 1) / NT;
       const int nb = PF && a.nb > 0 ? a.nb : 1;
@@ -718,6 +734,11 @@ static int density_blocks_per_cu()
        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, GridCfg<N>::PLANE_THREADS, lds) != hipSuccess)
       return 1;
    return nb > 0 ? nb : 1;
+}
+
+int plane_prefetch_on(int n)
+{
+   return (size_t)n * (n / 2 + 1) * 16 > (size_t)PLANE_PF_KB * 1024;
 }
 
 int plane_density_occupancy(int n, int fp32)
