@@ -768,16 +768,16 @@ int launch_to_bf16(const double *a, __bf16 *b, size_t n, hipStream_t s)
  * lo + O(2^-17 |x|) with hi = bf16(x) and lo = bf16(x - hi), and
  *   x y ~ hi_x hi_y + hi_x lo_y + lo_x hi_y
  * (three v_mfma_f32_16x16x32_bf16, FP32 sums), about 2^-16 relative per
- * term at six times the FP32 matrix rate, so the kernel is bound by its
- * reads of the FP64 orbitals.
+ * term at about five times the FP32 matrix rate, so the kernel is bound by
+ * its reads of the FP64 orbitals.
  *
  * partial[z][i][n] = sum_{k in split z} d_i(k) w_k V_n(k), i < R, n < 2R,
- * V = [c1 | d]: one workgroup per (128-column block, K split), every
+ * V = [c1 | d]: one workgroup per (CB 128-column blocks, K split), every
  * workgroup holding all R rows of d, so that both operands come from the
- * one read of C1 and C2 rows (the column block's rows are among them).
- * Blocks are assigned to the 8 XCDs so that the 2R/128 column blocks of
- * one K range run on one XCD and share its L2.  The d x d products are
- * formed in full; the caller reads one triangle. */
+ * one read of C1 and C2 rows (the column blocks' rows are among them).
+ * Workgroups are assigned to the 8 XCDs so that those of one K range run
+ * on one XCD and share its L2.  The d x d products are formed in full; the
+ * caller reads one triangle. */
 typedef double d2v __attribute__((ext_vector_type(2)));
 
 
@@ -1073,8 +1073,8 @@ int launch_to_bf16x2(const double *a, __bf16 *BsH, __bf16 *BsL, size_t n, hipStr
  * SIMD at 96 KB of LDS).  MFMA D[i][j] with i an orbital (A = XT rows, 8
  * consecutive p per lane) and j a real (B = the staged V^T rows, 8
  * consecutive p per lane); lane (rr, q), register r holds orbital 4q + r,
- * real rr of its block: 16 lanes cover 128 consecutive bytes of a C2 row,
- * updated in place from the registers. */
+ * real rr of its block.  The sums are staged in LDS for the update of C2
+ * (below), which reads and writes whole 1 KB row pieces. */
 #define VK 128         /* reals per workgroup */
 
 template<int NO, int PRE>
