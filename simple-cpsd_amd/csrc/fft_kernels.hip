@@ -368,7 +368,8 @@ constexpr int plane_lds_waves()
  * FP64 density pass at N = 96, DENS_WAVES_F32 / VPSI_WAVES_F32 for the FP32
  * orbital passes at N >= 80, but never more than the LDS admits (at
  * N = 128 the FP32 planes fit two workgroups, 4 waves per SIMD: sizing the
- * registers for 6 only spilled them) */
+ * registers for 6 only spilled them), and 2 for an FP32 density pass that
+ * prefetches (plane_prefetch_mode) */
 /* the FP64 plane passes (density, V psi) fetch the next orbital's plane
  * into registers while the current one is transformed when a plane is
  * larger than PLANE_PF_KB KB: such planes leave one workgroup per CU, whose
@@ -384,12 +385,25 @@ constexpr bool plane_prefetch()
 {
    return sizeof(V) == 16 && (size_t)N * (N / 2 + 1) * sizeof(V) > (size_t)PLANE_PF_KB * 1024;
 }
+/* the FP32 density pass at N >= 112: one workgroup per CU with the next
+ * orbital's plane prefetched, as in FP64, instead of two workgroups whose
+ * 128 registers per lane spill the FP64 density sums (17 VGPRs at N = 128).
+ * At 128^3 and 256 orbitals: 1.11 -> 0.99 ms per pass */
+#ifndef DENS_PF_F32
+#define DENS_PF_F32 1
+#endif
+template<int N, int MODE, typename V>
+constexpr bool plane_prefetch_mode()
+{
+   return plane_prefetch<N, V>() || (DENS_PF_F32 && sizeof(V) == 8 && N >= 112 && MODE == PM_DENSITY);
+}
 
 template<int N, int MODE, typename V>
 constexpr int plane_min_waves()
 {
    constexpr int lw = plane_lds_waves<N, V>() > 0 ? plane_lds_waves<N, V>() : 1;
-   constexpr int want = (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
+   constexpr int want = (sizeof(V) == 8 && plane_prefetch_mode<N, MODE, V>()) ? 2
+                      : (sizeof(V) == 8 && N >= 80 && MODE == PM_DENSITY) ? DENS_WAVES_F32
                       : (sizeof(V) == 8 && N >= 80 && MODE == PM_VPSI) ? VPSI_WAVES_F32
                       : (N == 96 && MODE == PM_DENSITY && !plane_prefetch<N, V>()) ? 4 : 1;
    return want < lw ? want : lw;
@@ -439,7 +453,7 @@ __global__ void __launch_bounds__(GridCfg<N>::PLANE_THREADS, (plane_min_waves<N,
       /* as in the V psi pass: an FP64 plane that fills more than half the
        * LDS leaves one workgroup per CU, so the next orbital's plane is
        * fetched into registers while the current one is transformed */
-      constexpr bool PF = plane_prefetch<N, V>();
+      constexpr bool PF = plane_prefetch_mode<N, MODE, V>();
       if constexpr (PF) {
          constexpr int EPT = (N * NH + NT - 1) / NT;
          V nx[EPT];
