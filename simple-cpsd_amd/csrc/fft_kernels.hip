@@ -387,8 +387,10 @@ constexpr bool plane_prefetch()
 }
 /* the FP32 density pass at N >= 112: one workgroup per CU with the next
  * orbital's plane prefetched, as in FP64, instead of two workgroups whose
- * 128 registers per lane spill the FP64 density sums (17 VGPRs at N = 128).
- * At 128^3 and 256 orbitals: 1.11 -> 0.99 ms per pass */
+ * 128 registers per lane spill the FP64 density sums (17 VGPRs spilled at
+ * N = 128).  At 128^3 and 256 orbitals: 1.11 -> 0.99 ms per pass.  The
+ * FP32 V psi pass keeps two workgroups per CU: prefetching it there
+ * measured 1.61 -> 2.27 ms. */
 #ifndef DENS_PF_F32
 #define DENS_PF_F32 1
 #endif
