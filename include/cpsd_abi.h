@@ -143,9 +143,9 @@ int cpsd_dev_epilogue(cpsd_dev *h, double *ortho, double *eig, double *V);
  * and 3.  The reference has no such modes; its all-FP32 build (float_single.h
  * with sfftpack3d) is the nearest. */
 int cpsd_dev_set_precision(cpsd_dev *h, int mode);
-/* FP64 refresh period of modes 1 and 2: step k of a run (counted from the
- * last cpsd_dev_set_orbitals, k = 1, 2, ...) runs in FP64 when k is a
- * multiple of `period`, besides the last step of every call, which the
+/* FP64 refresh period of the mixed modes (1-3): step k of a run (counted
+ * from the last cpsd_dev_set_orbitals, k = 1, 2, ...) runs in FP64 when k is
+ * a multiple of `period`, besides the last step of every call, which the
  * energies need in FP64 (so a call with inner = 1 runs only FP64 steps);
  * 0 (the default) refreshes on the last step of each call only.  Modes 2
  * and 3 are refused with moving ions (set_ions move = 1), and moving ions
