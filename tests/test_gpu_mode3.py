@@ -21,6 +21,8 @@ start, and at convergence against the FP64 minimum:
   constrained with the split-BF16 overlaps: from the Gaussian start here,
   where |d| is large (E falls from 1.7e3 Ha by hundreds of Ha per call), that
   is about 1e-7 of |E|; the error shrinks with d;
+- the same bars with an FP64 refresh step every 3 steps inside the calls
+  (measured: energies 1-6e-8 of |E|, c2 1.3e-7, dn 6.5e-8 from the oracle);
 - with the learned preconditioner on the 32-water cell, the converged energy
   within 1e-8 Ha per atom of the FP64 run's (same preconditioner, same start);
 - graph replays bit-identical to eager calls.
@@ -116,6 +118,37 @@ def test_mode3_vs_mode1_and_oracle(gpu, ne, precond):
     assert worst_ab < 5e-7 and worst_or < 5e-7
     assert dc_or < 1e-6
     assert dn_or < 5e-7
+
+
+def test_mode3_refresh_period(gpu):
+    """an FP64 step every 3 steps inside mode-3 calls (cpsd_dev_set_refresh):
+    the refresh steps form the full FP64 Gram after split-BF16 steps and the
+    steps after them carry its <c1|c1>; the same bars against the oracle"""
+    say("mode 3 with an FP64 refresh every 3 steps")
+    nfft, L, ne = 24, 12.0, 128
+    cell = cpsd_amd.water_cell(2, L, nfft, seed=7)
+    psi = cell.random_orbitals(ne, seed=3, decay=0.05)
+    st = O.State(O.Setup(1, L, nfft, cell.katm, cell.zv), ne, cell.lmmax, cell.vl, cell.vnl,
+                 cell.vnlnrm, cell.r, psi, cell.amass)
+    dev = make(cell, ne, psi, 3)
+    dev.set_refresh(3)
+    worst = 0.0
+    for call in range(3):
+        Eb, deb, dcb = dev.run(5)
+        Er, _, _ = st.step(5)
+        d = float(np.max(np.abs(Eb[:10] - Er[:10]))) / max(1.0, abs(Er[0]))
+        worst = max(worst, d)
+        say("  call %d: E %.10f  |E - E(oracle)|/|E| %.2e" % (call, Eb[0], d))
+        assert np.isfinite(deb) and np.isfinite(dcb)
+    ob = dev.get(c2=True, dn=True)
+    dc = float(np.max(np.abs(ob["c2"] - st.c2)))
+    dn = float(np.max(np.abs(ob["dn"] - st.dn)))
+    orth = ortho_dev(ob["c2"], ne, nfft)
+    say("  c2 vs oracle %.2e; dn vs oracle %.2e; orthonormality %.2e" % (dc, dn, orth))
+    dev.close()
+    assert orth < 1e-12
+    assert worst < 5e-7
+    assert dc < 1e-6 and dn < 5e-7
 
 
 def test_mode3_graphs_bitwise(gpu):
